@@ -1,0 +1,63 @@
+"""Build the native parts of fk_mc_amd in-tree.
+
+  fk_mc_amd/lib/libfkgpu.so    HIP kernels + C ABI (hipcc, gfx950)
+  fk_mc_amd/lib/libfkmc_host.so  C++ host layer mirroring the reference classes
+
+Every call recompiles (the build check must exercise the compiler).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+ARCH = os.environ.get("FK_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             # packed FP32 VALU ops are kept out of every code object
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+GPU_SOURCES = ["csrc/fk_kernels.hip", "csrc/fk_vectors.hip", "csrc/fk_measures.hip", "csrc/fk_cheb.hip",
+               "csrc/fkgpu.cpp"]
+HOST_SOURCES = ["host/lattice.cpp", "host/configuration.cpp", "host/moves.cpp", "host/moves_chebyshev.cpp",
+                "host/mc_metropolis.cpp", "host/measures.cpp", "host/capi.cpp"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=HERE)
+
+
+def build_gpu():
+    os.makedirs(LIB, exist_ok=True)
+    srcs = [s for s in GPU_SOURCES if os.path.exists(os.path.join(HERE, s))]
+    out = os.path.join(LIB, "libfkgpu.so")
+    tmp = out + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs)
+    os.replace(tmp, out)
+    return out
+
+
+def build_host():
+    srcs = [s for s in HOST_SOURCES if os.path.exists(os.path.join(HERE, s))]
+    if not srcs:
+        return None
+    out = os.path.join(LIB, "libfkmc_host.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(HERE, "host"), "-o", tmp] + srcs +
+         ["-L", LIB, "-lfkgpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    build_gpu()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

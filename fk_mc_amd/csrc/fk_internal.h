@@ -1,0 +1,67 @@
+// fk_internal.h -- kernel argument blocks and launcher prototypes shared by the
+// kernels and the C ABI implementation.  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fkgpu.h"
+
+// Batched calc_hamiltonian + calc_ed.  All pointers are device pointers.
+struct eig_args {
+    int B, N;
+    const double* hop;     // N x N dense hopping template
+    const int32_t* f;      // B x N occupations
+    const double* beta;    // B
+    const double* U;       // B
+    const double* mu_c;    // B
+    double* evals;         // B x N (ascending)
+    double* logz;          // B
+    double* cexp;          // B x N or null
+    double* cfermi;        // B x N or null
+    double* ws;            // grid x N x N workspace (global-memory solver) or null
+    double* vout;          // B x N x N Householder vectors or null
+    double* tauout;        // B x N or null
+    double* d_out;         // B x N tridiagonal diagonal or null
+    double* e_out;         // B x N tridiagonal off-diagonal or null
+};
+
+// One Metropolis move for each of B chains.
+struct sweep_args {
+    int B, N;
+    const double* hop;
+    const int32_t* nbr;    // N x zmax neighbour table (hopping-graph order)
+    const int32_t* z;      // N neighbour counts
+    int zmax;
+    uint8_t* f;            // B x N
+    int32_t* nf;           // B
+    double* logz;          // B
+    double* spectrum;      // B x N or null
+    uint32_t* mt;          // B x 625
+    const double* beta;
+    const double* U;
+    const double* mu_c;
+    const double* mu_f;
+    long long* naccept;    // B
+    long long* nsolve;     // B
+    double* ws;            // grid x N x N workspace or null
+    int ncp;               // number of registered moves
+    double cp[4];          // cumulative move probabilities (libstdc++ discrete_distribution)
+    int kinds[4];          // move kind of each registered move
+    int nW;                // f-f interaction range (1-D lattices only)
+    double W[8];
+    int trace;             // record the per-move trace
+    int step;              // trace row
+    int32_t* tr_kind;
+    int32_t* tr_a;
+    int32_t* tr_b;
+    double* tr_u;
+    double* tr_logw;
+    double* tr_logz;
+    int8_t* tr_acc;
+};
+
+int fk_solver_kind(int N);          // -1: unsupported size
+int fk_solver_threads(int kind);
+int fk_solver_needs_ws(int kind);
+hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
+hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);

@@ -1,0 +1,395 @@
+// fk_kernels.hip -- HIP/CDNA4 kernels of the per-move diagonalisation path.
+//
+//   k_eig_logz_reg<P,NB>   calc_hamiltonian + calc_ed for a batch of f-configs
+//                          (src/configuration.cpp:79-91, 208-246), N <= P*NB,
+//                          matrix resident in VGPRs, one workgroup per config.
+//   k_eig_logz_glob        same for larger N: matrix in an HBM workspace.
+//   k_sweep<Solver>        one Metropolis move for every chain of a batch in one
+//                          launch: move choice, proposal, H assembly, eigensolve,
+//                          logZ, accept test, state update
+//                          (src/mc_metropolis.cpp:34-52, src/moves.cpp).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fk_internal.h"
+#include "fk_rng.h"
+#include "fk_solve.h"
+
+namespace fkk {
+
+using namespace fks;
+
+// ---------------------------------------------------------------------------
+// Solver policies.  Each provides:
+//   NT                      threads per workgroup
+//   smem                    shared-memory struct (holds ev[], e2[] etc.)
+//   solve(...) -> logZ      eigenvalues of H(f_lds) into sm.ev, returns logZ
+// ---------------------------------------------------------------------------
+template <int P, int NB>
+struct reg_solver {
+    static constexpr int NT = P * P;
+    static constexpr int NP = P * NB;
+    static constexpr int NMAX = NP;
+    struct smem {
+        tri_smem<P, NB> t;
+        double ev[NP];
+        double e2[NP];
+        double scratch[3 * (NT / WAVE) + 8];
+    };
+    __device__ static const double* dptr(const smem& sm) { return sm.t.d; }
+    __device__ static const double* eptr(const smem& sm) { return sm.t.e; }
+    __device__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
+                                   double beta, double U, double mu_c, double* ws, double* vout, double* tauout,
+                                   double* cexp, double* cfermi) {
+        const int tid = threadIdx.x;
+        const int lane = tid & (WAVE - 1);
+        const int r = (tid / WAVE) * (WAVE / P) + lane / P;
+        const int c = lane % P;
+        double A[NB * (NB + 1) / 2];
+        assemble<P, NB>(A, hop, f_lds, U, mu_c, N, r, c);
+        tridiagonalize<P, NB>(A, sm.t, N, vout, tauout);
+        tridiag_eigvals<NT>(sm.t.d, sm.t.e, sm.e2, N, sm.ev, sm.scratch);
+        return logz_block<NT>(sm.ev, N, beta, cexp, cfermi, sm.scratch);
+    }
+};
+
+// Matrix in an N x N HBM workspace (ws), one workgroup per matrix.
+struct glob_solver {
+    static constexpr int NT = 1024;
+    static constexpr int NMAX = 2048;
+    struct smem {
+        double xs[NMAX], vs[NMAX], ps[NMAX], wsv[NMAX];
+        double d[NMAX], e[NMAX], e2[NMAX], ev[NMAX];
+        double scratch[3 * (NT / WAVE) + 8];
+        double red[4];
+    };
+    __device__ static const double* dptr(const smem& sm) { return sm.d; }
+    __device__ static const double* eptr(const smem& sm) { return sm.e; }
+    __device__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
+                                   double beta, double U, double mu_c, double* __restrict__ W, double* vout,
+                                   double* tauout, double* cexp, double* cfermi) {
+        const int tid = threadIdx.x;
+        const int lane = tid & (WAVE - 1), wid = tid / WAVE;
+        constexpr int NW = NT / WAVE;
+        const size_t NN = (size_t)N * N;
+        for (size_t idx = tid; idx < NN; idx += NT) {
+            int i = (int)(idx / N), j = (int)(idx % N);
+            double v = hop[idx];
+            if (i == j) v += -mu_c + U * (double)f_lds[i];
+            W[idx] = v;
+        }
+        for (int i = tid; i < N; i += NT) { sm.d[i] = 0.0; sm.e[i] = 0.0; }
+        __syncthreads();
+        for (int k = 0; k + 2 < N; ++k) {
+            for (int i = k + 1 + tid; i < N; i += NT) sm.xs[i] = W[(size_t)i * N + k];
+            if (tid == 0) sm.d[k] = W[(size_t)k * N + k];
+            __syncthreads();
+            double alpha = sm.xs[k + 1];
+            double part = 0.0;
+            for (int i = k + 2 + lane; i < N; i += WAVE) part += sm.xs[i] * sm.xs[i];
+            double xn2 = wave_sum(part);
+            double tau = 0.0, beta_h = alpha, scal = 0.0;
+            if (xn2 != 0.0) {
+                beta_h = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                tau = (beta_h - alpha) / beta_h;
+                scal = 1.0 / (alpha - beta_h);
+            }
+            if (tid == 0) sm.e[k] = beta_h;
+            for (int i = tid; i < N; i += NT)
+                sm.vs[i] = (i == k + 1) ? 1.0 : ((i > k + 1) ? sm.xs[i] * scal : 0.0);
+            if (vout) {
+                for (int i = tid; i < N; i += NT)
+                    vout[(size_t)k * N + i] = (i == k + 1) ? 1.0 : ((i > k + 1) ? sm.xs[i] * scal : 0.0);
+                if (tid == 0) tauout[k] = tau;
+            }
+            __syncthreads();
+            if (tau != 0.0) {
+                for (int i = k + 1 + wid; i < N; i += NW) {
+                    const double* row = W + (size_t)i * N;
+                    double s = 0.0;
+                    for (int j = k + 1 + lane; j < N; j += WAVE) s = fma(row[j], sm.vs[j], s);
+                    s = wave_sum(s);
+                    if (lane == 0) sm.ps[i] = tau * s;
+                }
+                __syncthreads();
+                double pvp = 0.0;
+                for (int i = k + 1 + lane; i < N; i += WAVE) pvp += sm.ps[i] * sm.vs[i];
+                const double coef = 0.5 * tau * wave_sum(pvp);
+                for (int i = k + 1 + tid; i < N; i += NT) sm.wsv[i] = sm.ps[i] - coef * sm.vs[i];
+                __syncthreads();
+                const int m = N - k - 1;
+                const size_t mm = (size_t)m * m;
+                for (size_t idx = tid; idx < mm; idx += NT) {
+                    int i = k + 1 + (int)(idx / m), j = k + 1 + (int)(idx % m);
+                    double* wp = W + (size_t)i * N + j;
+                    *wp = fma(-sm.vs[i], sm.wsv[j], fma(-sm.wsv[i], sm.vs[j], *wp));
+                }
+            }
+            __syncthreads();
+        }
+        if (N >= 2 && tid == 0) {
+            sm.d[N - 2] = W[(size_t)(N - 2) * N + (N - 2)];
+            sm.e[N - 2] = W[(size_t)(N - 1) * N + (N - 2)];
+            sm.d[N - 1] = W[(size_t)(N - 1) * N + (N - 1)];
+        } else if (N == 1 && tid == 0) {
+            sm.d[0] = W[0];
+        }
+        if (vout) {
+            for (int i = tid; i < N; i += NT) {
+                if (N >= 2) vout[(size_t)(N - 2) * N + i] = 0.0;
+                vout[(size_t)(N - 1) * N + i] = 0.0;
+            }
+            if (tid == 0) { if (N >= 2) tauout[N - 2] = 0.0; tauout[N - 1] = 0.0; }
+        }
+        __syncthreads();
+        tridiag_eigvals<NT>(sm.d, sm.e, sm.e2, N, sm.ev, sm.scratch);
+        return logz_block<NT>(sm.ev, N, beta, cexp, cfermi, sm.scratch);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Batched calc_hamiltonian + calc_ed.
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(S::NT) k_eig_logz(eig_args a) {
+    __shared__ typename S::smem sm;
+    __shared__ uint8_t f_lds[S::NMAX];
+    const int tid = threadIdx.x;
+    const int N = a.N;
+    for (int bidx = blockIdx.x; bidx < a.B; bidx += gridDim.x) {
+        for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? (uint8_t)(a.f[(size_t)bidx * N + i] != 0) : 0;
+        __syncthreads();
+        double* ws = a.ws ? a.ws + (size_t)blockIdx.x * N * N : nullptr;
+        double* vout = a.vout ? a.vout + (size_t)bidx * N * N : nullptr;
+        double* tauout = a.tauout ? a.tauout + (size_t)bidx * N : nullptr;
+        double lz = S::solve(sm, a.hop, f_lds, N, a.beta[bidx], a.U[bidx], a.mu_c[bidx], ws, vout, tauout,
+                             a.cexp ? a.cexp + (size_t)bidx * N : nullptr,
+                             a.cfermi ? a.cfermi + (size_t)bidx * N : nullptr);
+        for (int i = tid; i < N; i += S::NT) a.evals[(size_t)bidx * N + i] = sm.ev[i];
+        if (tid == 0) a.logz[bidx] = lz;
+        if (a.d_out) {
+            // tridiagonal (d, e) for the eigenvector stage
+            const double* dd = S::dptr(sm);
+            const double* ee = S::eptr(sm);
+            for (int i = tid; i < N; i += S::NT) {
+                a.d_out[(size_t)bidx * N + i] = dd[i];
+                a.e_out[(size_t)bidx * N + i] = (i + 1 < N) ? ee[i] : 0.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One Metropolis move for every chain (one launch per move step).
+// ---------------------------------------------------------------------------
+__device__ inline double ff_energy_dev(const uint8_t* f, int V, int nW, const double* W) {
+    // src/configuration.cpp:59-77 (called only for 1-D lattices)
+    double e = 0.0;
+    for (int i = 0; i < V; ++i) {
+        if (!f[i]) continue;
+        for (int l = 0; l < nW; ++l) {
+            int left = (i - l + V) % V, right = (i + l) % V;
+            e += W[l] * f[left];
+            e += W[l] * f[right] * (l > 0);
+        }
+    }
+    return e;
+}
+
+template <class S>
+__global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
+    __shared__ typename S::smem sm;
+    __shared__ uint8_t f_lds[S::NMAX];
+    __shared__ uint32_t mt[fkrng::MT_WORDS];
+    __shared__ double ctl_d[8];
+    __shared__ int ctl_i[8];
+    const int tid = threadIdx.x;
+    const int N = a.N;
+    for (int ch = blockIdx.x; ch < a.B; ch += gridDim.x) {
+        uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
+        for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
+        for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;
+        __syncthreads();
+        const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch], mu_f = a.mu_f[ch];
+        if (tid == 0) {
+            const int nf = a.nf[ch];
+            const int mi = fkrng::discrete(mt, a.cp, a.ncp);
+            const int kind = a.kinds[mi];
+            int sa = -1, sb = -1, solve = 0, mode = 0;  // mode: 0 normal, 1 zero weight, 2 unit weight
+            int nf_new = nf;
+            double ff_old = 0.0, ff_new = 0.0;
+            if (a.nW > 0) ff_old = ff_energy_dev(f_lds, N, a.nW, a.W);
+            if (kind == FKGPU_MOVE_ADDREMOVE) {
+                sa = fkrng::uniform_int(mt, 0, N - 1);
+                f_lds[sa] ^= 1;
+                nf_new = nf + (f_lds[sa] ? 1 : -1);
+                solve = 1;
+            } else if (kind == FKGPU_MOVE_FLIP) {
+                if (nf == 0 || nf == N) {
+                    mode = 1;
+                } else {
+                    int from = fkrng::uniform_int(mt, 0, N - 1);
+                    while (f_lds[from] == 0) from = fkrng::uniform_int(mt, 0, N - 1);
+                    int to = fkrng::uniform_int(mt, 0, N - 1);
+                    while (f_lds[to] == 1) to = fkrng::uniform_int(mt, 0, N - 1);
+                    f_lds[from] = 0;
+                    f_lds[to] = 1;
+                    sa = from;
+                    sb = to;
+                    solve = 1;
+                }
+            } else if (kind == FKGPU_MOVE_HOP) {
+                int i = fkrng::uniform_int(mt, 0, N - 1);
+                int zi = a.z[i];
+                int j = a.nbr[(size_t)i * a.zmax + fkrng::uniform_int(mt, 0, zi - 1)];
+                sa = i;
+                sb = j;
+                if (f_lds[i] == f_lds[j]) {
+                    mode = 2;
+                } else {
+                    uint8_t t = f_lds[i];
+                    f_lds[i] = f_lds[j];
+                    f_lds[j] = t;
+                    solve = 1;
+                }
+            } else {  // FKGPU_MOVE_RANDOMIZE: randomize_f(rnd) with nf = 0 (src/configuration.cpp:47-56)
+                int nfs = fkrng::uniform_int(mt, 0, N - 1);
+                for (int i = 0; i < N; ++i) f_lds[i] = 0;
+                for (int s = 0; s < nfs; ++s) {
+                    int ind = fkrng::uniform_int(mt, 0, N - 1);
+                    while (f_lds[ind] == 1) ind = fkrng::uniform_int(mt, 0, N - 1);
+                    f_lds[ind] = 1;
+                }
+                nf_new = nfs;
+                solve = 1;
+            }
+            if (a.nW > 0 && solve) ff_new = ff_energy_dev(f_lds, N, a.nW, a.W);
+            const double u = fkrng::canonical(mt);
+            ctl_i[0] = kind; ctl_i[1] = sa; ctl_i[2] = sb; ctl_i[3] = solve; ctl_i[4] = mode; ctl_i[5] = nf_new;
+            ctl_d[0] = u; ctl_d[1] = ff_new - ff_old;
+        }
+        __syncthreads();
+        const int solve = ctl_i[3];
+        double lz_new = 0.0;
+        if (solve) {
+            double* ws = a.ws ? a.ws + (size_t)blockIdx.x * N * N : nullptr;
+            lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
+        }
+        if (tid == 0) {
+            const int kind = ctl_i[0], mode = ctl_i[4], sa = ctl_i[1];
+            const double u = ctl_d[0], ffdiff = ctl_d[1];
+            const double lz_old = a.logz[ch];
+            double w = 0.0, logw = -INFINITY;
+            if (mode == 1) {
+                w = 0.0;
+            } else if (mode == 2) {
+                w = 1.0;
+                logw = 0.0;
+            } else if (kind == FKGPU_MOVE_ADDREMOVE) {
+                // src/moves.cpp:63-66
+                double ebmf = exp(beta * mu_f);
+                double ratio = exp(lz_new - lz_old);
+                w = (f_lds[sa] ? ratio * ebmf : ratio / ebmf) * exp(-beta * ffdiff);
+                logw = (lz_new - lz_old) + (f_lds[sa] ? 1.0 : -1.0) * beta * mu_f - beta * ffdiff;
+            } else if (kind == FKGPU_MOVE_FLIP) {
+                // src/moves.cpp:19-20
+                w = exp(lz_new - lz_old);
+                logw = lz_new - lz_old;
+            } else if (kind == FKGPU_MOVE_HOP) {
+                w = exp(lz_new - lz_old) * exp(-beta * ffdiff);
+                logw = lz_new - lz_old - beta * ffdiff;
+            } else {
+                // src/moves.cpp:43-48 including the clamp
+                double log_ratio = lz_new - lz_old;
+                double dn = (double)(ctl_i[5] - a.nf[ch]);
+                double g = beta * mu_f * dn - ffdiff;
+                logw = log_ratio + g;
+                if (g > 2.7182818 - log_ratio) w = 1.0;
+                else if (g + log_ratio < 0) w = 0.0;
+                else w = exp(log_ratio) * exp(beta * (mu_f * dn - ffdiff));
+            }
+            const int acc = fabs(w) > u;
+            ctl_i[6] = acc;
+            if (a.trace) {
+                const size_t t = (size_t)a.step * a.B + ch;
+                a.tr_kind[t] = kind;
+                a.tr_a[t] = ctl_i[1];
+                a.tr_b[t] = ctl_i[2];
+                a.tr_u[t] = u;
+                a.tr_logw[t] = logw;
+                a.tr_logz[t] = ctl_i[3] ? lz_new : 0.0;
+                a.tr_acc[t] = acc;
+            }
+            if (acc) {
+                a.naccept[ch] += 1;
+                a.nf[ch] = ctl_i[5];
+                if (ctl_i[3]) a.logz[ch] = lz_new;
+            }
+            if (ctl_i[3]) a.nsolve[ch] += 1;
+        }
+        __syncthreads();
+        if (ctl_i[6]) {
+            for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * N + i] = f_lds[i];
+            if (ctl_i[3] && a.spectrum)
+                for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = sm.ev[i];
+        }
+        for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mtg[i] = mt[i];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers.
+// ---------------------------------------------------------------------------
+template <class S>
+static hipError_t launch_eig(const eig_args& a, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_eig_logz<S>, dim3(grid), dim3(S::NT), 0, st, a);
+    return hipGetLastError();
+}
+template <class S>
+static hipError_t launch_sweep(const sweep_args& a, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_sweep<S>, dim3(grid), dim3(S::NT), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace fkk
+
+using namespace fkk;
+
+int fk_solver_kind(int N) {
+    if (N <= 64) return 0;
+    if (N <= 128) return 1;
+    if (N <= 256) return 2;
+    if (N <= glob_solver::NMAX) return 3;
+    return -1;
+}
+int fk_solver_threads(int kind) {
+    switch (kind) {
+        case 0: return reg_solver<16, 4>::NT;
+        case 1: return reg_solver<16, 8>::NT;
+        case 2: return reg_solver<32, 8>::NT;
+        case 3: return glob_solver::NT;
+    }
+    return 0;
+}
+int fk_solver_needs_ws(int kind) { return kind == 3; }
+
+hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) {
+    switch (kind) {
+        case 0: return launch_eig<reg_solver<16, 4>>(a, grid, st);
+        case 1: return launch_eig<reg_solver<16, 8>>(a, grid, st);
+        case 2: return launch_eig<reg_solver<32, 8>>(a, grid, st);
+        case 3: return launch_eig<glob_solver>(a, grid, st);
+    }
+    return hipErrorInvalidValue;
+}
+hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st) {
+    switch (kind) {
+        case 0: return launch_sweep<reg_solver<16, 4>>(a, grid, st);
+        case 1: return launch_sweep<reg_solver<16, 8>>(a, grid, st);
+        case 2: return launch_sweep<reg_solver<32, 8>>(a, grid, st);
+        case 3: return launch_sweep<glob_solver>(a, grid, st);
+    }
+    return hipErrorInvalidValue;
+}

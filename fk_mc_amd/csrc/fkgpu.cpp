@@ -1,0 +1,671 @@
+// fkgpu.cpp -- implementation of the C ABI declared in include/fkgpu.h.
+//
+// Host-side responsibilities: lattice hopping template (built once, kept in
+// HBM), argument validation with the reference's error convention turned into
+// status codes, buffer management, kernel dispatch by matrix size, HIP events
+// for timing and HIP graph capture of move steps.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fkgpu.h"
+#include "fk_internal.h"
+#include "fk_rng.h"
+
+struct fkgpu_ctx {
+    int refs = 1;  // the owner plus one per live chain set
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cu = 0;
+    int N = 0;
+    int lat_kind = -1, lat_ndim = 0, lat_L = 0;
+    double* d_hop = nullptr;
+    int32_t* d_nbr = nullptr;
+    int32_t* d_z = nullptr;
+    int zmax = 0;
+    std::vector<double> hop;
+    std::vector<int32_t> nbr, z;
+    // grow-on-demand scratch for the host-pointer entry points
+    struct buf { void* p = nullptr; size_t n = 0; };
+    buf b_f, b_beta, b_U, b_mu, b_ev, b_lz, b_ce, b_cf, b_ws;
+};
+
+struct fkgpu_chains {
+    fkgpu_ctx* ctx = nullptr;
+    int B = 0, N = 0, kind = -1, grid = 0;
+    uint8_t* f = nullptr;
+    int32_t* nf = nullptr;
+    double* logz = nullptr;
+    double* spectrum = nullptr;
+    uint32_t* mt = nullptr;
+    double *beta = nullptr, *U = nullptr, *mu_c = nullptr, *mu_f = nullptr;
+    long long *naccept = nullptr, *nsolve = nullptr;
+    double* ws = nullptr;
+    sweep_args args{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    size_t trace_cap = 0;
+    int32_t *t_kind = nullptr, *t_a = nullptr, *t_b = nullptr;
+    double *t_u = nullptr, *t_logw = nullptr, *t_logz = nullptr;
+    int8_t* t_acc = nullptr;
+};
+
+namespace {
+
+int fail(fkgpu_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+int hipfail(fkgpu_ctx* ctx, hipError_t e, const char* where) {
+    return fail(ctx, FKGPU_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+#define FK_HIP(ctx, call)                                   \
+    do {                                                    \
+        hipError_t _e = (call);                             \
+        if (_e != hipSuccess) return hipfail(ctx, _e, #call); \
+    } while (0)
+
+template <class T>
+int ensure(fkgpu_ctx* ctx, fkgpu_ctx::buf& b, size_t count, T** out) {
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    if (b.n < bytes) {
+        if (b.p) hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        b.n = bytes;
+    }
+    *out = (T*)b.p;
+    return FKGPU_OK;
+}
+
+bool finite_all(const double* p, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+int grid_for(fkgpu_ctx* ctx, int kind, int B) {
+    int per_cu = fk_solver_threads(kind) <= 256 ? 4 : 1;
+    int g = ctx->num_cu * per_cu;
+    return B < g ? B : g;
+}
+
+// Site index order of hypercubic_lattice::index_to_pos: last coordinate fastest.
+void build_hypercubic(fkgpu_ctx* ctx, int kind, int D, int L, double t, double tp) {
+    int n = 1;
+    for (int i = 0; i < D; ++i) n *= L;
+    ctx->N = n;
+    ctx->hop.assign((size_t)n * n, 0.0);
+    ctx->zmax = 2 * D + (kind == FKGPU_LATTICE_TRIANGULAR ? 2 : 0);
+    ctx->nbr.assign((size_t)n * ctx->zmax, 0);
+    ctx->z.assign(n, 0);
+    int stride[3] = {1, 1, 1};
+    for (int d = D - 2; d >= 0; --d) stride[d] = stride[d + 1] * L;
+    for (int i = 0; i < n; ++i) {
+        int p[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) p[d] = (i / stride[d]) % L;
+        auto add = [&](int j, double v) {
+            ctx->hop[(size_t)i * n + j] += v;
+            ctx->nbr[(size_t)i * ctx->zmax + ctx->z[i]++] = j;
+        };
+        for (int d = 0; d < D; ++d) {
+            int pl = p[d] > 0 ? p[d] - 1 : L - 1;
+            int pr = p[d] < L - 1 ? p[d] + 1 : 0;
+            add(i + (pl - p[d]) * stride[d], -t);
+            add(i + (pr - p[d]) * stride[d], -t);
+        }
+        if (kind == FKGPU_LATTICE_TRIANGULAR) {
+            int xl = p[0] > 0 ? p[0] - 1 : L - 1, yl = p[1] > 0 ? p[1] - 1 : L - 1;
+            int xr = p[0] < L - 1 ? p[0] + 1 : 0, yr = p[1] < L - 1 ? p[1] + 1 : 0;
+            add(xl * stride[0] + yl, -tp);
+            add(xr * stride[0] + yr, -tp);
+        }
+    }
+}
+
+int upload_lattice(fkgpu_ctx* ctx) {
+    if (ctx->d_hop) hipFree(ctx->d_hop);
+    if (ctx->d_nbr) hipFree(ctx->d_nbr);
+    if (ctx->d_z) hipFree(ctx->d_z);
+    ctx->d_hop = nullptr;
+    ctx->d_nbr = ctx->d_z = nullptr;
+    size_t n = ctx->N;
+    FK_HIP(ctx, hipMalloc(&ctx->d_hop, sizeof(double) * n * n));
+    FK_HIP(ctx, hipMalloc(&ctx->d_nbr, sizeof(int32_t) * ctx->nbr.size()));
+    FK_HIP(ctx, hipMalloc(&ctx->d_z, sizeof(int32_t) * n));
+    FK_HIP(ctx, hipMemcpy(ctx->d_hop, ctx->hop.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+    FK_HIP(ctx, hipMemcpy(ctx->d_nbr, ctx->nbr.data(), sizeof(int32_t) * ctx->nbr.size(), hipMemcpyHostToDevice));
+    FK_HIP(ctx, hipMemcpy(ctx->d_z, ctx->z.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    return FKGPU_OK;
+}
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+}
+
+}  // namespace
+
+extern "C" {
+
+int fkgpu_create(int device, fkgpu_ctx** out) {
+    if (!out) return FKGPU_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FKGPU_EHIP;
+    if (device < 0 || device >= ndev) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = new fkgpu_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return FKGPU_EHIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+    if (ctx->num_cu <= 0) ctx->num_cu = 256;
+    *out = ctx;
+    return FKGPU_OK;
+}
+
+static void ctx_release(fkgpu_ctx* ctx) {
+    if (!ctx || --ctx->refs > 0) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto* b : {&ctx->b_f, &ctx->b_beta, &ctx->b_U, &ctx->b_mu, &ctx->b_ev, &ctx->b_lz, &ctx->b_ce, &ctx->b_cf,
+                    &ctx->b_ws})
+        if (b->p) hipFree(b->p);
+    if (ctx->d_hop) hipFree(ctx->d_hop);
+    if (ctx->d_nbr) hipFree(ctx->d_nbr);
+    if (ctx->d_z) hipFree(ctx->d_z);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+// Destroying a context while chain sets built on it are alive defers the
+// release until the last of them is destroyed.
+void fkgpu_destroy(fkgpu_ctx* ctx) { ctx_release(ctx); }
+
+const char* fkgpu_last_error(const fkgpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+void* fkgpu_stream(fkgpu_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+int fkgpu_msize(const fkgpu_ctx* ctx) { return ctx ? ctx->N : 0; }
+
+int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, double tp) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (kind != FKGPU_LATTICE_HYPERCUBIC && kind != FKGPU_LATTICE_TRIANGULAR)
+        return fail(ctx, FKGPU_EINVAL, "unknown lattice kind");
+    if (ndim < 1 || ndim > 3) return fail(ctx, FKGPU_EINVAL, "ndim must be 1..3");
+    if (kind == FKGPU_LATTICE_TRIANGULAR && ndim != 2) return fail(ctx, FKGPU_EINVAL, "triangular lattice is 2-D");
+    if (L < 3) return fail(ctx, FKGPU_EINVAL, "lattice size L must be >= 3");
+    if (!std::isfinite(t) || !std::isfinite(tp)) return fail(ctx, FKGPU_EINVAL, "non-finite hopping");
+    long n = 1;
+    for (int i = 0; i < ndim; ++i) n *= L;
+    if (fk_solver_kind((int)n) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "lattice volume beyond the dense solver range");
+    hipSetDevice(ctx->device);
+    build_hypercubic(ctx, kind, ndim, L, t, tp);
+    ctx->lat_kind = kind;
+    ctx->lat_ndim = ndim;
+    ctx->lat_L = L;
+    return upload_lattice(ctx);
+}
+
+int fkgpu_set_lattice_csr(fkgpu_ctx* ctx, int n, const int32_t* row_ptr, const int32_t* col, const double* val) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (n <= 0 || !row_ptr || !col || !val) return fail(ctx, FKGPU_EINVAL, "invalid CSR arguments");
+    if (fk_solver_kind(n) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "matrix size beyond the dense solver range");
+    std::vector<double> h((size_t)n * n, 0.0);
+    std::vector<int32_t> z(n, 0);
+    int zmax = 0;
+    for (int i = 0; i < n; ++i) {
+        if (row_ptr[i + 1] < row_ptr[i]) return fail(ctx, FKGPU_EINVAL, "row_ptr not monotone");
+        for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+            int j = col[p];
+            if (j < 0 || j >= n || !std::isfinite(val[p])) return fail(ctx, FKGPU_EINVAL, "invalid CSR entry");
+            h[(size_t)i * n + j] += val[p];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j)
+            if (h[(size_t)i * n + j] != h[(size_t)j * n + i]) return fail(ctx, FKGPU_EINVAL, "hopping matrix not symmetric");
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j)
+            if (i != j && h[(size_t)i * n + j] != 0.0) ++z[i];
+        zmax = std::max(zmax, (int)z[i]);
+    }
+    if (zmax == 0) zmax = 1;
+    ctx->N = n;
+    ctx->hop.swap(h);
+    ctx->zmax = zmax;
+    ctx->z.assign(n, 0);
+    ctx->nbr.assign((size_t)n * zmax, 0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if (i != j && ctx->hop[(size_t)i * n + j] != 0.0) ctx->nbr[(size_t)i * zmax + ctx->z[i]++] = j;
+    ctx->lat_kind = 2;
+    ctx->lat_ndim = 0;
+    ctx->lat_L = 0;
+    hipSetDevice(ctx->device);
+    return upload_lattice(ctx);
+}
+
+static int check_params(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                        const double* mu_c) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
+    if (batch <= 0 || !f || !beta || !U || !mu_c) return fail(ctx, FKGPU_EINVAL, "null argument or empty batch");
+    if (!finite_all(beta, batch) || !finite_all(U, batch) || !finite_all(mu_c, batch))
+        return fail(ctx, FKGPU_EINVAL, "non-finite beta, U or mu_c");
+    const size_t n = (size_t)batch * ctx->N;
+    for (size_t i = 0; i < n; ++i)
+        if (f[i] != 0 && f[i] != 1) return fail(ctx, FKGPU_EINVAL, "occupations must be 0 or 1");
+    return FKGPU_OK;
+}
+
+int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta, const double* d_U,
+                          const double* d_mu_c, double* d_evals, double* d_logZ) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
+    if (batch <= 0 || !d_f || !d_beta || !d_U || !d_mu_c || !d_evals || !d_logZ)
+        return fail(ctx, FKGPU_EINVAL, "null argument or empty batch");
+    const int N = ctx->N;
+    const int kind = fk_solver_kind(N);
+    const int grid = grid_for(ctx, kind, batch);
+    eig_args a{};
+    a.B = batch;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = d_f;
+    a.beta = d_beta;
+    a.U = d_U;
+    a.mu_c = d_mu_c;
+    a.evals = d_evals;
+    a.logz = d_logZ;
+    if (fk_solver_needs_ws(kind)) {
+        int r = ensure(ctx, ctx->b_ws, (size_t)grid * N * N, &a.ws);
+        if (r) return r;
+    }
+    hipSetDevice(ctx->device);
+    FK_HIP(ctx, fk_launch_eig(kind, a, grid, ctx->stream));
+    return FKGPU_OK;
+}
+
+int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                   const double* mu_c, int want_vectors, double* evals, double* evecs, double* logZ,
+                   double* cached_exp, double* cached_fermi) {
+    int r = check_params(ctx, batch, f, beta, U, mu_c);
+    if (r) return r;
+    if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
+    if (want_vectors) return fail(ctx, FKGPU_EUNSUPPORTED, "eigenvectors are not available in this build");
+    (void)evecs;
+    hipSetDevice(ctx->device);
+    const int N = ctx->N;
+    const size_t BN = (size_t)batch * N;
+    int32_t* df;
+    double *dbeta, *dU, *dmu, *dev, *dlz, *dce = nullptr, *dcf = nullptr;
+    if ((r = ensure(ctx, ctx->b_f, BN, &df)) || (r = ensure(ctx, ctx->b_beta, batch, &dbeta)) ||
+        (r = ensure(ctx, ctx->b_U, batch, &dU)) || (r = ensure(ctx, ctx->b_mu, batch, &dmu)) ||
+        (r = ensure(ctx, ctx->b_ev, BN, &dev)) || (r = ensure(ctx, ctx->b_lz, batch, &dlz)))
+        return r;
+    if (cached_exp || cached_fermi) {
+        if ((r = ensure(ctx, ctx->b_ce, BN, &dce)) || (r = ensure(ctx, ctx->b_cf, BN, &dcf))) return r;
+    }
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemcpyAsync(df, f, sizeof(int32_t) * BN, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dbeta, beta, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dU, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dmu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    const int kind = fk_solver_kind(N);
+    const int grid = grid_for(ctx, kind, batch);
+    eig_args a{};
+    a.B = batch;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = df;
+    a.beta = dbeta;
+    a.U = dU;
+    a.mu_c = dmu;
+    a.evals = dev;
+    a.logz = dlz;
+    a.cexp = dce;
+    a.cfermi = dcf;
+    if (fk_solver_needs_ws(kind)) {
+        if ((r = ensure(ctx, ctx->b_ws, (size_t)grid * N * N, &a.ws))) return r;
+    }
+    FK_HIP(ctx, fk_launch_eig(kind, a, grid, st));
+    FK_HIP(ctx, hipMemcpyAsync(evals, dev, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipMemcpyAsync(logZ, dlz, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
+    if (cached_exp) FK_HIP(ctx, hipMemcpyAsync(cached_exp, dce, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+    if (cached_fermi) FK_HIP(ctx, hipMemcpyAsync(cached_fermi, dcf, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Chains
+// ---------------------------------------------------------------------------
+void fkgpu_chains_destroy(fkgpu_chains* ch) {
+    if (!ch) return;
+    if (ch->ctx) {
+        hipSetDevice(ch->ctx->device);
+        hipStreamSynchronize(ch->ctx->stream);
+    }
+    if (ch->gexec) hipGraphExecDestroy(ch->gexec);
+    if (ch->graph) hipGraphDestroy(ch->graph);
+    void* ptrs[] = {ch->f, ch->nf, ch->logz, ch->spectrum, ch->mt, ch->beta, ch->U, ch->mu_c, ch->mu_f,
+                    ch->naccept, ch->nsolve, ch->ws, ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw,
+                    ch->t_logz, ch->t_acc};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (ch->ev0) hipEventDestroy(ch->ev0);
+    if (ch->ev1) hipEventDestroy(ch->ev1);
+    fkgpu_ctx* ctx = ch->ctx;
+    delete ch;
+    ctx_release(ctx);
+}
+
+int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, const double* beta, const double* U,
+                        const double* mu_c, const double* mu_f, const uint32_t* seeds, fkgpu_chains** out) {
+    if (!out) return FKGPU_EINVAL;
+    *out = nullptr;
+    int r = check_params(ctx, nchains, f_init, beta, U, mu_c);
+    if (r) return r;
+    if (!mu_f || !seeds || !finite_all(mu_f, nchains)) return fail(ctx, FKGPU_EINVAL, "invalid mu_f or seeds");
+    hipSetDevice(ctx->device);
+    const int N = ctx->N, B = nchains;
+    fkgpu_chains* ch = new fkgpu_chains();
+    ch->ctx = ctx;
+    ctx->refs += 1;
+    ch->B = B;
+    ch->N = N;
+    ch->kind = fk_solver_kind(N);
+    ch->grid = grid_for(ctx, ch->kind, B);
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    chk(dalloc(&ch->f, (size_t)B * N));
+    chk(dalloc(&ch->nf, B));
+    chk(dalloc(&ch->logz, B));
+    chk(dalloc(&ch->spectrum, (size_t)B * N));
+    chk(dalloc(&ch->mt, (size_t)B * fkrng::MT_WORDS));
+    chk(dalloc(&ch->beta, B));
+    chk(dalloc(&ch->U, B));
+    chk(dalloc(&ch->mu_c, B));
+    chk(dalloc(&ch->mu_f, B));
+    chk(dalloc(&ch->naccept, B));
+    chk(dalloc(&ch->nsolve, B));
+    if (fk_solver_needs_ws(ch->kind)) chk(dalloc(&ch->ws, (size_t)ch->grid * N * N));
+    chk(hipEventCreate(&ch->ev0));
+    chk(hipEventCreate(&ch->ev1));
+    if (e != hipSuccess) {
+        fkgpu_chains_destroy(ch);
+        return fail(ctx, FKGPU_ENOMEM, std::string("chain allocation: ") + hipGetErrorString(e));
+    }
+    std::vector<uint8_t> f8((size_t)B * N);
+    std::vector<int32_t> nf(B, 0);
+    for (int c = 0; c < B; ++c)
+        for (int i = 0; i < N; ++i) {
+            f8[(size_t)c * N + i] = (uint8_t)f_init[(size_t)c * N + i];
+            nf[c] += f_init[(size_t)c * N + i];
+        }
+    std::vector<uint32_t> mts((size_t)B * fkrng::MT_WORDS);
+    for (int c = 0; c < B; ++c) fkrng::mt_seed(&mts[(size_t)c * fkrng::MT_WORDS], seeds[c]);
+    std::vector<long long> zeros(B, 0);
+    hipStream_t st = ctx->stream;
+    chk(hipMemcpyAsync(ch->f, f8.data(), f8.size(), hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->nf, nf.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->mt, mts.data(), sizeof(uint32_t) * mts.size(), hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->beta, beta, sizeof(double) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->U, U, sizeof(double) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->mu_c, mu_c, sizeof(double) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->mu_f, mu_f, sizeof(double) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->naccept, zeros.data(), sizeof(long long) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->nsolve, zeros.data(), sizeof(long long) * B, hipMemcpyHostToDevice, st));
+    if (e != hipSuccess) {
+        fkgpu_chains_destroy(ch);
+        return hipfail(ctx, e, "chain upload");
+    }
+    // initial logZ and spectrum of every chain
+    int32_t* df;
+    if ((r = ensure(ctx, ctx->b_f, (size_t)B * N, &df))) { fkgpu_chains_destroy(ch); return r; }
+    chk(hipMemcpyAsync(df, f_init, sizeof(int32_t) * B * N, hipMemcpyHostToDevice, st));
+    eig_args a{};
+    a.B = B;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = df;
+    a.beta = ch->beta;
+    a.U = ch->U;
+    a.mu_c = ch->mu_c;
+    a.evals = ch->spectrum;
+    a.logz = ch->logz;
+    a.ws = ch->ws;
+    chk(fk_launch_eig(ch->kind, a, ch->grid, st));
+    chk(hipStreamSynchronize(st));
+    if (e != hipSuccess) {
+        fkgpu_chains_destroy(ch);
+        return hipfail(ctx, e, "initial eigensolve");
+    }
+    sweep_args& s = ch->args;
+    s.B = B;
+    s.N = N;
+    s.hop = ctx->d_hop;
+    s.nbr = ctx->d_nbr;
+    s.z = ctx->d_z;
+    s.zmax = ctx->zmax;
+    s.f = ch->f;
+    s.nf = ch->nf;
+    s.logz = ch->logz;
+    s.spectrum = ch->spectrum;
+    s.mt = ch->mt;
+    s.beta = ch->beta;
+    s.U = ch->U;
+    s.mu_c = ch->mu_c;
+    s.mu_f = ch->mu_f;
+    s.naccept = ch->naccept;
+    s.nsolve = ch->nsolve;
+    s.ws = ch->ws;
+    s.ncp = 1;
+    s.cp[0] = 1.0;
+    s.kinds[0] = FKGPU_MOVE_ADDREMOVE;
+    *out = ch;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, const double* weights) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    if (nmoves < 1 || nmoves > 4 || !kinds || !weights) return fail(ctx, FKGPU_EINVAL, "1..4 moves required");
+    double sum = 0.0;
+    for (int m = 0; m < nmoves; ++m) {
+        if (kinds[m] < 0 || kinds[m] > 3) return fail(ctx, FKGPU_EINVAL, "unknown move kind");
+        if (!(weights[m] >= 0.0) || !std::isfinite(weights[m])) return fail(ctx, FKGPU_EINVAL, "invalid move weight");
+        sum += weights[m];
+    }
+    if (!(sum > 0.0)) return fail(ctx, FKGPU_EINVAL, "move weights sum to zero");
+    // libstdc++ discrete_distribution::param_type::_M_initialize: normalise by
+    // division, partial sums, last entry forced to 1; a single weight draws nothing.
+    sweep_args& s = ch->args;
+    s.ncp = nmoves;
+    double acc = 0.0;
+    for (int m = 0; m < nmoves; ++m) {
+        acc += weights[m] / sum;
+        s.cp[m] = acc;
+        s.kinds[m] = kinds[m];
+    }
+    s.cp[nmoves - 1] = 1.0;
+    if (nmoves < 2) s.ncp = 1;
+    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+    ch->graph_steps = 0;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W) {
+    if (!ch) return FKGPU_EINVAL;
+    if (nW < 0 || nW > 8 || (nW > 0 && !W) || (nW > 0 && !finite_all(W, nW)))
+        return fail(ch->ctx, FKGPU_EINVAL, "W must hold 0..8 finite values");
+    if (nW > 0 && ch->ctx->lat_ndim != 1) nW = 0;  // calc_ff_energy returns 0 unless ndim == 1
+    ch->args.nW = nW;
+    for (int i = 0; i < nW; ++i) ch->args.W[i] = W[i];
+    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+    ch->graph_steps = 0;
+    return FKGPU_OK;
+}
+
+int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    if (nsteps < 0) return fail(ctx, FKGPU_EINVAL, "negative step count");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    sweep_args a = ch->args;
+    const size_t need = (size_t)nsteps * ch->B;
+    if (trace) {
+        if (ch->trace_cap < need) {
+            void* ps[] = {ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw, ch->t_logz, ch->t_acc};
+            for (void* p : ps)
+                if (p) hipFree(p);
+            ch->trace_cap = 0;
+            hipError_t e = hipSuccess;
+            auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+            chk(dalloc(&ch->t_kind, need));
+            chk(dalloc(&ch->t_a, need));
+            chk(dalloc(&ch->t_b, need));
+            chk(dalloc(&ch->t_u, need));
+            chk(dalloc(&ch->t_logw, need));
+            chk(dalloc(&ch->t_logz, need));
+            chk(dalloc(&ch->t_acc, need));
+            if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "trace allocation failed");
+            ch->trace_cap = need;
+        }
+        a.trace = 1;
+        a.tr_kind = ch->t_kind;
+        a.tr_a = ch->t_a;
+        a.tr_b = ch->t_b;
+        a.tr_u = ch->t_u;
+        a.tr_logw = ch->t_logw;
+        a.tr_logz = ch->t_logz;
+        a.tr_acc = ch->t_acc;
+    }
+    FK_HIP(ctx, hipEventRecord(ch->ev0, st));
+    for (int s = 0; s < nsteps; ++s) {
+        a.step = s;
+        FK_HIP(ctx, fk_launch_sweep(ch->kind, a, ch->grid, st));
+    }
+    FK_HIP(ctx, hipEventRecord(ch->ev1, st));
+    FK_HIP(ctx, hipEventSynchronize(ch->ev1));
+    float ms = 0.f;
+    FK_HIP(ctx, hipEventElapsedTime(&ms, ch->ev0, ch->ev1));
+    ch->last_ms = ms;
+    if (trace && nsteps > 0) {
+        auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+            return dst ? hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+        };
+        FK_HIP(ctx, cp(trace->kind, ch->t_kind, sizeof(int32_t) * need));
+        FK_HIP(ctx, cp(trace->site_a, ch->t_a, sizeof(int32_t) * need));
+        FK_HIP(ctx, cp(trace->site_b, ch->t_b, sizeof(int32_t) * need));
+        FK_HIP(ctx, cp(trace->u, ch->t_u, sizeof(double) * need));
+        FK_HIP(ctx, cp(trace->logw, ch->t_logw, sizeof(double) * need));
+        FK_HIP(ctx, cp(trace->logz_new, ch->t_logz, sizeof(double) * need));
+        FK_HIP(ctx, cp(trace->accepted, ch->t_acc, sizeof(int8_t) * need));
+    }
+    return FKGPU_OK;
+}
+
+int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    if (nsteps <= 0) return fail(ctx, FKGPU_EINVAL, "step count must be positive");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    if (!ch->gexec || ch->graph_steps != nsteps) {
+        if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+        if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+        sweep_args a = ch->args;
+        a.trace = 0;
+        FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < nsteps; ++s) {
+            a.step = s;
+            hipError_t e = fk_launch_sweep(ch->kind, a, ch->grid, st);
+            if (e != hipSuccess) {
+                hipGraph_t g;
+                hipStreamEndCapture(st, &g);
+                if (g) hipGraphDestroy(g);
+                return hipfail(ctx, e, "graph capture");
+            }
+        }
+        FK_HIP(ctx, hipStreamEndCapture(st, &ch->graph));
+        FK_HIP(ctx, hipGraphInstantiate(&ch->gexec, ch->graph, nullptr, nullptr, 0));
+        ch->graph_steps = nsteps;
+    }
+    FK_HIP(ctx, hipEventRecord(ch->ev0, st));
+    FK_HIP(ctx, hipGraphLaunch(ch->gexec, st));
+    FK_HIP(ctx, hipEventRecord(ch->ev1, st));
+    FK_HIP(ctx, hipEventSynchronize(ch->ev1));
+    float ms = 0.f;
+    FK_HIP(ctx, hipEventElapsedTime(&ms, ch->ev0, ch->ev1));
+    ch->last_ms = ms;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_last_ms(const fkgpu_chains* ch, double* ms) {
+    if (!ch || !ms) return FKGPU_EINVAL;
+    *ms = ch->last_ms;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, double* spectrum,
+                       int64_t* naccept, int64_t* nsolve) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t BN = (size_t)ch->B * ch->N;
+    if (f) {
+        std::vector<uint8_t> f8(BN);
+        FK_HIP(ctx, hipMemcpy(f8.data(), ch->f, BN, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < BN; ++i) f[i] = f8[i];
+    }
+    if (nf) FK_HIP(ctx, hipMemcpy(nf, ch->nf, sizeof(int32_t) * ch->B, hipMemcpyDeviceToHost));
+    if (logZ) FK_HIP(ctx, hipMemcpy(logZ, ch->logz, sizeof(double) * ch->B, hipMemcpyDeviceToHost));
+    if (spectrum) FK_HIP(ctx, hipMemcpy(spectrum, ch->spectrum, sizeof(double) * BN, hipMemcpyDeviceToHost));
+    if (naccept) FK_HIP(ctx, hipMemcpy(naccept, ch->naccept, sizeof(int64_t) * ch->B, hipMemcpyDeviceToHost));
+    if (nsolve) FK_HIP(ctx, hipMemcpy(nsolve, ch->nsolve, sizeof(int64_t) * ch->B, hipMemcpyDeviceToHost));
+    return FKGPU_OK;
+}
+
+int fkgpu_rng_draws(uint32_t seed, int mode, int hi, int nprobs, const double* probs, int count, double* out) {
+    if (!out || count < 0 || mode < 0 || mode > 3 || (mode == 3 && (nprobs < 1 || nprobs > 64 || !probs)))
+        return FKGPU_EINVAL;
+    std::vector<uint32_t> st(fkrng::MT_WORDS);
+    fkrng::mt_seed(st.data(), seed);
+    std::vector<double> cp;
+    if (mode == 3) {
+        double sum = 0.0;
+        for (int i = 0; i < nprobs; ++i) sum += probs[i];
+        double acc = 0.0;
+        for (int i = 0; i < nprobs; ++i) { acc += probs[i] / sum; cp.push_back(acc); }
+        cp.back() = 1.0;
+    }
+    for (int i = 0; i < count; ++i) {
+        switch (mode) {
+            case 0: out[i] = (double)fkrng::mt_next(st.data()); break;
+            case 1: out[i] = fkrng::uniform_int(st.data(), 0, hi); break;
+            case 2: out[i] = fkrng::canonical(st.data()); break;
+            case 3: out[i] = fkrng::discrete(st.data(), cp.data(), nprobs < 2 ? 1 : nprobs); break;
+        }
+    }
+    return FKGPU_OK;
+}
+
+}  // extern "C"

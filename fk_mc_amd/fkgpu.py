@@ -1,0 +1,256 @@
+"""ctypes binding of the fkgpu C ABI (include/fkgpu.h, fk_mc_amd/lib/libfkgpu.so).
+
+There is no CPU fallback: importing this module without the built HIP library
+raises, and every call reports a non-zero status as FkgpuError.
+"""
+import atexit
+import ctypes as C
+import os
+import weakref
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfkgpu.so")
+
+OK, EINVAL, ENOLATTICE, EUNSUPPORTED, EHIP, ENOMEM = 0, -1, -2, -3, -4, -5
+LATTICE_HYPERCUBIC, LATTICE_TRIANGULAR = 0, 1
+MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
+
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_u32 = C.POINTER(C.c_uint32)
+_i64 = C.POINTER(C.c_int64)
+_i8 = C.POINTER(C.c_int8)
+
+
+class FkgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fkgpu error {code}: {msg}")
+        self.code = code
+
+
+class _Trace(C.Structure):
+    _fields_ = [("kind", _i32), ("site_a", _i32), ("site_b", _i32), ("u", _d), ("logw", _d),
+                ("logz_new", _d), ("accepted", _i8)]
+
+
+_LIB = None
+_LIVE = []  # weak references to open handles, closed children-first at exit
+
+
+def _close_all():
+    for kind in ("chains", "ctx"):
+        for ref in list(_LIVE):
+            obj = ref()
+            if obj is not None and obj._kind == kind:
+                try:
+                    obj.close()
+                except Exception:
+                    pass
+
+
+atexit.register(_close_all)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.fkgpu_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.fkgpu_destroy.argtypes = [P]
+        L.fkgpu_destroy.restype = None
+        L.fkgpu_last_error.argtypes = [P]
+        L.fkgpu_last_error.restype = C.c_char_p
+        L.fkgpu_stream.argtypes = [P]
+        L.fkgpu_stream.restype = P
+        L.fkgpu_set_lattice.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.fkgpu_set_lattice_csr.argtypes = [P, C.c_int, _i32, _i32, _d]
+        L.fkgpu_msize.argtypes = [P]
+        L.fkgpu_eig_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, _d, _d, _d, _d, _d]
+        L.fkgpu_eig_logz_device.argtypes = [P, C.c_int, P, P, P, P, P, P]
+        L.fkgpu_chains_create.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _u32, C.POINTER(P)]
+        L.fkgpu_chains_destroy.argtypes = [P]
+        L.fkgpu_chains_destroy.restype = None
+        L.fkgpu_chains_set_moves.argtypes = [P, C.c_int, _i32, _d]
+        L.fkgpu_chains_set_ff.argtypes = [P, C.c_int, _d]
+        L.fkgpu_sweep.argtypes = [P, C.c_int, C.POINTER(_Trace)]
+        L.fkgpu_sweep_graph.argtypes = [P, C.c_int]
+        L.fkgpu_chains_last_ms.argtypes = [P, _d]
+        L.fkgpu_chains_state.argtypes = [P, _i32, _i32, _d, _d, _i64, _i64]
+        L.fkgpu_rng_draws.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, _d, C.c_int, _d]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """A device context holding one lattice (fkgpu_ctx)."""
+
+    _kind = "ctx"
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        _LIVE.append(weakref.ref(self))
+        r = lib().fkgpu_create(device, C.byref(self._h))
+        if r != OK:
+            raise FkgpuError(r, "fkgpu_create failed (no HIP device?)")
+        self.N = 0
+
+    def close(self):
+        if self._h:
+            lib().fkgpu_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, r):
+        if r != OK:
+            raise FkgpuError(r, lib().fkgpu_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return lib().fkgpu_stream(self._h)
+
+    def set_lattice(self, kind, ndim, L, t=1.0, tp=1.0):
+        self._check(lib().fkgpu_set_lattice(self._h, kind, ndim, L, t, tp))
+        self.N = lib().fkgpu_msize(self._h)
+        return self
+
+    def set_lattice_csr(self, hop_dense):
+        h = np.asarray(hop_dense, dtype=np.float64)
+        n = h.shape[0]
+        rp = [0]
+        cols, vals = [], []
+        for i in range(n):
+            nz = np.nonzero(h[i])[0]
+            cols.extend(nz.tolist())
+            vals.extend(h[i, nz].tolist())
+            rp.append(len(cols))
+        rp = np.array(rp, dtype=np.int32)
+        cols = np.array(cols if cols else [0], dtype=np.int32)
+        vals = np.array(vals if vals else [0.0], dtype=np.float64)
+        self._check(lib().fkgpu_set_lattice_csr(self._h, n, _p(rp, _i32), _p(cols, _i32), _p(vals, _d)))
+        self.N = lib().fkgpu_msize(self._h)
+        return self
+
+    def eig_logz(self, f, beta, U, mu_c, vectors=False, with_exp=False):
+        """calc_hamiltonian + calc_ed for a batch of configurations f[B, N]."""
+        f = np.ascontiguousarray(np.atleast_2d(f), dtype=np.int32)
+        B, N = f.shape
+        bet = np.ascontiguousarray(np.broadcast_to(np.asarray(beta, dtype=np.float64), (B,)))
+        uu = np.ascontiguousarray(np.broadcast_to(np.asarray(U, dtype=np.float64), (B,)))
+        mc = np.ascontiguousarray(np.broadcast_to(np.asarray(mu_c, dtype=np.float64), (B,)))
+        ev = np.zeros((B, N))
+        lz = np.zeros(B)
+        V = np.zeros((B, N, N)) if vectors else None
+        ce = np.zeros((B, N)) if with_exp else None
+        cf = np.zeros((B, N)) if with_exp else None
+        self._check(lib().fkgpu_eig_logz(self._h, B, _p(f, _i32), _p(bet, _d), _p(uu, _d), _p(mc, _d),
+                                         int(vectors), _p(ev, _d), _p(V, _d), _p(lz, _d), _p(ce, _d), _p(cf, _d)))
+        out = dict(evals=ev, logZ=lz)
+        if vectors:
+            out["evecs"] = V
+        if with_exp:
+            out["cexp"] = ce
+            out["cfermi"] = cf
+        return out
+
+
+class Chains:
+    """Batched persistent Metropolis chains (fkgpu_chains)."""
+
+    _kind = "chains"
+
+    def __init__(self, ctx, f_init, beta, U, mu_c, mu_f, seeds):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        _LIVE.append(weakref.ref(self))
+        f = np.ascontiguousarray(np.atleast_2d(f_init), dtype=np.int32)
+        self.B, self.N = f.shape
+        B = self.B
+
+        def vec(x, dt=np.float64):
+            return np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=dt), (B,)))
+
+        self._keep = [vec(beta), vec(U), vec(mu_c), vec(mu_f), vec(seeds, np.uint32)]
+        self._h = C.c_void_p()
+        ctx._check(lib().fkgpu_chains_create(ctx.handle, B, _p(f, _i32), _p(self._keep[0], _d),
+                                             _p(self._keep[1], _d), _p(self._keep[2], _d), _p(self._keep[3], _d),
+                                             _p(self._keep[4], _u32), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().fkgpu_chains_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_moves(self, moves):
+        kinds = np.array([m[0] for m in moves], dtype=np.int32)
+        w = np.array([m[1] for m in moves], dtype=np.float64)
+        self.ctx._check(lib().fkgpu_chains_set_moves(self._h, len(kinds), _p(kinds, _i32), _p(w, _d)))
+
+    def set_ff(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        self.ctx._check(lib().fkgpu_chains_set_ff(self._h, len(W), _p(W, _d) if len(W) else None))
+
+    def sweep(self, nsteps, trace=False):
+        if not trace:
+            self.ctx._check(lib().fkgpu_sweep(self._h, nsteps, None))
+            return None
+        n = nsteps * self.B
+        tr = dict(kind=np.zeros(n, np.int32), a=np.zeros(n, np.int32), b=np.zeros(n, np.int32),
+                  u=np.zeros(n), logw=np.zeros(n), logz=np.zeros(n), acc=np.zeros(n, np.int8))
+        t = _Trace(_p(tr["kind"], _i32), _p(tr["a"], _i32), _p(tr["b"], _i32), _p(tr["u"], _d),
+                   _p(tr["logw"], _d), _p(tr["logz"], _d), _p(tr["acc"], _i8))
+        self.ctx._check(lib().fkgpu_sweep(self._h, nsteps, C.byref(t)))
+        return {k: v.reshape(nsteps, self.B) for k, v in tr.items()}
+
+    def sweep_graph(self, nsteps):
+        self.ctx._check(lib().fkgpu_sweep_graph(self._h, nsteps))
+
+    def last_ms(self):
+        ms = C.c_double()
+        self.ctx._check(lib().fkgpu_chains_last_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def state(self):
+        B, N = self.B, self.N
+        f = np.zeros((B, N), np.int32)
+        nf = np.zeros(B, np.int32)
+        lz = np.zeros(B)
+        sp = np.zeros((B, N))
+        na = np.zeros(B, np.int64)
+        ns = np.zeros(B, np.int64)
+        self.ctx._check(lib().fkgpu_chains_state(self._h, _p(f, _i32), _p(nf, _i32), _p(lz, _d), _p(sp, _d),
+                                                 _p(na, _i64), _p(ns, _i64)))
+        return dict(f=f, nf=nf, logZ=lz, spectrum=sp, naccept=na, nsolve=ns)
+
+
+def rng_draws(seed, mode, count, hi=0, probs=None):
+    """Draws from the library's port of the reference's generator (host side)."""
+    out = np.zeros(count)
+    p = None if probs is None else np.ascontiguousarray(probs, dtype=np.float64)
+    r = lib().fkgpu_rng_draws(seed, mode, hi, 0 if p is None else len(p), _p(p, _d), count, _p(out, _d))
+    if r != OK:
+        raise FkgpuError(r, "fkgpu_rng_draws failed")
+    return out

@@ -1,0 +1,120 @@
+/* fkgpu.h -- C ABI of the MI355X per-move diagonalisation path of fk_mc.
+ *
+ * The reference has no FFI: its operator API for this path is the C++ surface of
+ * fk::configuration_t (include/fk_mc/configuration.hpp:52-87) consumed by the
+ * moves (src/moves.cpp), the Metropolis engine (src/mc_metropolis.cpp) and the
+ * measures (include/fk_mc/measures/).  Each entry point below replaces one piece
+ * of that surface; the C++ host layer in fk_mc_amd/host/ keeps the reference's
+ * class interfaces and calls these functions.
+ *
+ * Conventions: every function returns an int status (FKGPU_OK = 0, negative on
+ * error; fkgpu_last_error() holds the message).  No C++ exception crosses the
+ * ABI.  Pointers named d_* are device pointers; all others are host pointers.
+ * Host-pointer calls are synchronous; device-pointer calls are ordered on the
+ * context's HIP stream (fkgpu_stream()).  One context per host thread.
+ * Occupations f are int32 0/1, one per lattice site, site order as
+ * hypercubic_lattice::index_to_pos (last coordinate fastest).
+ * Matrices are row-major; eigenvectors are stored as columns (evecs[i*N + k] is
+ * component i of eigenvector k), eigenvalues ascending.
+ */
+#ifndef FKGPU_H
+#define FKGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FKGPU_OK 0
+#define FKGPU_EINVAL (-1)       /* invalid argument: null pointer, size, NaN/Inf parameter */
+#define FKGPU_ENOLATTICE (-2)   /* no lattice set on the context */
+#define FKGPU_EUNSUPPORTED (-3) /* size or option outside what the library implements */
+#define FKGPU_EHIP (-4)         /* HIP runtime error */
+#define FKGPU_ENOMEM (-5)       /* device allocation failed */
+
+#define FKGPU_LATTICE_HYPERCUBIC 0 /* fill_nearest_neighbors<D>, src/lattice/hypercubic.cpp:115-131 */
+#define FKGPU_LATTICE_TRIANGULAR 1 /* fill_triangular, src/lattice/hypercubic.cpp:137-155 */
+
+#define FKGPU_MOVE_ADDREMOVE 0 /* move_addremove, src/moves.cpp:52-67 (single-site f flip) */
+#define FKGPU_MOVE_FLIP 1      /* move_flip, src/moves.cpp:5-21 (global exchange)       */
+#define FKGPU_MOVE_HOP 2       /* nearest-neighbour exchange; no reference counterpart   */
+#define FKGPU_MOVE_RANDOMIZE 3 /* move_randomize, src/moves.cpp:35-49                   */
+
+typedef struct fkgpu_ctx fkgpu_ctx;
+typedef struct fkgpu_chains fkgpu_chains;
+
+/* Context on one HIP device.  Replaces nothing in the reference (which has no
+ * device); owns the lattice, the stream and the workspaces. */
+int fkgpu_create(int device, fkgpu_ctx** out);
+void fkgpu_destroy(fkgpu_ctx* ctx);
+const char* fkgpu_last_error(const fkgpu_ctx* ctx);
+void* fkgpu_stream(fkgpu_ctx* ctx); /* hipStream_t */
+
+/* Lattice hopping structure.  Replaces hypercubic_lattice<D>(L) +
+ * fill_nearest_neighbors(l, t) (include/fk_mc/lattice/hypercubic.hpp:136,166;
+ * src/lattice/hypercubic.cpp:115-131) and fill_triangular(l, t, tp) (:137-155):
+ * H_ij = -t for nearest neighbours, -tp on the triangular diagonal. L >= 3. */
+int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, double tp);
+/* Arbitrary symmetric hopping matrix in CSR form (abstract_lattice::set_hopping_m,
+ * include/fk_mc/lattice.hpp:39; graph lattices). */
+int fkgpu_set_lattice_csr(fkgpu_ctx* ctx, int n, const int32_t* row_ptr, const int32_t* col, const double* val);
+int fkgpu_msize(const fkgpu_ctx* ctx);
+
+/* configuration_t::calc_hamiltonian() + calc_ed(bool calc_evecs) for `batch`
+ * configurations (src/configuration.cpp:79-91, 208-246).  Outputs per config:
+ * evals[N] (ed_cache::cached_spectrum), logZ (ed_cache::logZ), cached_exp[N],
+ * cached_fermi[N] (nullable), evecs[N*N] (ed_cache::cached_evecs, only when
+ * want_vectors != 0).  Parameters are per configuration. */
+int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                   const double* mu_c, int want_vectors, double* evals, double* evecs, double* logZ,
+                   double* cached_exp, double* cached_fermi);
+/* Same, device pointers, asynchronous on the context stream (eigenvalues only). */
+int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta, const double* d_U,
+                          const double* d_mu_c, double* d_evals, double* d_logZ);
+
+/* ---- Batched persistent Metropolis chains ------------------------------------
+ * Replaces mc_metropolis::update (src/mc_metropolis.cpp:34-52) with the moves
+ * of src/moves.cpp for many independent chains at once; each chain owns its
+ * std::mt19937 state (seeded like src/mc_metropolis.cpp:25) and consumes random
+ * words in the reference's order, so a chain reproduces the reference chain of
+ * the same seed and initial configuration. */
+typedef struct {
+    int32_t* kind;     /* move kind                                   */
+    int32_t* site_a;   /* toggled / source site (-1 if none)         */
+    int32_t* site_b;   /* destination site (-1 if none)              */
+    double* u;         /* Metropolis uniform                         */
+    double* logw;      /* log of the move weight (-inf: zero weight) */
+    double* logz_new;  /* logZ of the proposal (0 if not solved)     */
+    int8_t* accepted;  /* 1 if accepted                              */
+} fkgpu_trace;         /* host arrays of nsteps*nchains, index [step*nchains + chain] */
+
+int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, const double* beta, const double* U,
+                        const double* mu_c, const double* mu_f, const uint32_t* seeds, fkgpu_chains** out);
+void fkgpu_chains_destroy(fkgpu_chains* ch);
+/* mc_metropolis::add_move in order (include/fk_mc/mc_metropolis.hpp:263-270):
+ * move kinds and their probabilities (up to 4). */
+int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, const double* weights);
+/* f-f interaction W of 1-D lattices (configuration_t::calc_ff_energy, src/configuration.cpp:59-77). */
+int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W);
+/* Run nsteps single moves on every chain (one launch per move step).  trace may be null. */
+int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace);
+/* Replay nsteps move steps from a captured HIP graph (no trace). */
+int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps);
+/* Device time of the last fkgpu_sweep/fkgpu_sweep_graph call (HIP events on the context stream). */
+int fkgpu_chains_last_ms(const fkgpu_chains* ch, double* ms);
+/* Current state of every chain (nullable outputs): f[nchains*N], nf, logZ,
+ * spectrum[nchains*N] (of the current configuration), accepted and solved counts. */
+int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, double* spectrum,
+                       int64_t* naccept, int64_t* nsolve);
+
+/* Diagnostic (host only, no device needed; no reference counterpart): draws
+ * from the library's port of std::mt19937(seed) and the libstdc++
+ * distributions used by the reference -- mode 0 raw words, 1
+ * uniform_int(0, hi), 2 uniform_real(0, 1), 3 discrete_distribution(probs).
+ * Used to pin the port against libstdc++ itself. */
+int fkgpu_rng_draws(uint32_t seed, int mode, int hi, int nprobs, const double* probs, int count, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FKGPU_H */

@@ -1,0 +1,116 @@
+"""GPU Metropolis chains against the oracle chain (src/mc_metropolis.cpp:34-52,
+src/moves.cpp).
+
+For a fixed seed the device chain must draw the same sites and uniforms as the
+reference chain, and take the same accept/reject decisions except at near-ties
+|log u - log w| < 1e-9 (counted).  After a near-tie decided differently the
+oracle is re-run following the device's decision there, so the comparison
+continues over the whole chain.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NEAR_TIE = 1e-9
+
+
+def compare_chain(orc, kind, D, L, beta, U, mu_c, mu_f, seed, f0, moves, tr, t=1.0, tp=1.0):
+    """Compare one device chain trace with the oracle; returns #near-ties."""
+    nsteps = len(tr["u"])
+    forced = -np.ones(nsteps, np.int32)
+    for _attempt in range(8):
+        ref = orc.run_chain(kind, D, L, beta, U, mu_c, mu_f, seed, f0, moves, nsteps, t=t, tp=tp, forced=forced)
+        assert np.array_equal(tr["kind"], ref["kind"]), "move choice differs"
+        assert np.array_equal(tr["a"], ref["a"]), "site draw differs"
+        assert np.array_equal(tr["b"], ref["b"]), "second site differs"
+        assert np.array_equal(tr["u"], ref["u"]), "uniform differs"
+        solved = ref["logz"] != 0.0
+        dl = np.abs(tr["logz"][solved] - ref["logz"][solved])
+        assert dl.max(initial=0.0) <= 1e-10 * np.abs(ref["logz"][solved]).max(initial=1.0), dl.max()
+        mism = np.nonzero(tr["acc"] != ref["acc"])[0]
+        if len(mism) == 0:
+            return int(np.sum(forced >= 0))
+        s = mism[0]
+        gap = abs(np.log(ref["u"][s]) - ref["logw"][s])
+        assert gap < NEAR_TIE, f"accept differs at move {s}: |log u - log w| = {gap}"
+        forced[s] = tr["acc"][s]
+    raise AssertionError("too many near-ties")
+
+
+@pytest.mark.parametrize("D,L,nsteps", [(2, 8, 2000), (1, 30, 500), (2, 16, 150)])
+def test_addremove_chain(fk, orc, D, L, nsteps):
+    print(f"case addremove D={D} L={L}", flush=True)
+    N = L ** D
+    rng = np.random.default_rng(L)
+    B = 4
+    f0 = rng.integers(0, 2, size=(B, N)).astype(np.int32)
+    beta, U = 1.0, 1.0
+    seeds = np.arange(B, dtype=np.uint32) + 1000
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, D, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
+    tr = ch.sweep(nsteps, trace=True)
+    for c in range(B):
+        trc = {k: v[:, c] for k, v in tr.items()}
+        compare_chain(orc, orc.KIND_HYPERCUBIC, D, L, beta, U, U / 2, U / 2, int(seeds[c]), f0[c],
+                      [(orc.MOVE_ADDREMOVE, 1.0)], trc, tp=0.0)
+    st = ch.state()
+    ref = orc.run_chain(orc.KIND_HYPERCUBIC, D, L, beta, U, U / 2, U / 2, int(seeds[0]), f0[0],
+                        [(orc.MOVE_ADDREMOVE, 1.0)], nsteps, tp=0.0, forced=None)
+    if np.array_equal(tr["acc"][:, 0], ref["acc"]):
+        assert np.array_equal(st["f"][0], ref["f_final"])
+        assert np.abs(st["spectrum"][0] - ref["spectrum_final"]).max() < 1e-10 * np.abs(ref["spectrum_final"]).max()
+    ch.close()
+    ctx.close()
+
+
+def test_mixed_moves_chain(fk, orc):
+    """addremove + global exchange + nearest-neighbour exchange + randomize."""
+    print("case mixed moves", flush=True)
+    D, L = 2, 6
+    N = L ** D
+    rng = np.random.default_rng(3)
+    B = 3
+    f0 = rng.integers(0, 2, size=(B, N)).astype(np.int32)
+    f0[2] = 0  # empty configuration: flip returns weight 0 without drawing
+    beta = np.array([0.7, 2.0, 5.0])
+    U = np.array([1.0, 2.0, 4.0])
+    seeds = np.array([11, 12, 13], np.uint32)
+    moves_fk = [(fk.MOVE_ADDREMOVE, 0.4), (fk.MOVE_FLIP, 0.3), (fk.MOVE_HOP, 0.25), (fk.MOVE_RANDOMIZE, 0.05)]
+    moves_orc = [(orc.MOVE_ADDREMOVE, 0.4), (orc.MOVE_FLIP, 0.3), (orc.MOVE_HOP, 0.25), (orc.MOVE_RANDOMIZE, 0.05)]
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, D, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
+    ch.set_moves(moves_fk)
+    tr = ch.sweep(600, trace=True)
+    for c in range(B):
+        trc = {k: v[:, c] for k, v in tr.items()}
+        compare_chain(orc, orc.KIND_HYPERCUBIC, D, L, beta[c], U[c], U[c] / 2, U[c] / 2, int(seeds[c]), f0[c],
+                      moves_orc, trc, tp=0.0)
+    ch.close()
+    ctx.close()
+
+
+def test_graph_replay_matches_eager(fk, orc):
+    """hipGraph-captured move steps give the same chain state as eager launches."""
+    print("case graph replay", flush=True)
+    D, L = 2, 8
+    N = L ** D
+    rng = np.random.default_rng(9)
+    f0 = rng.integers(0, 2, size=(5, N)).astype(np.int32)
+    seeds = np.arange(5, dtype=np.uint32) + 77
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, D, L, 1.0, 0.0)
+    a = fk.Chains(ctx, f0, 2.0, 2.0, 1.0, 1.0, seeds)
+    b = fk.Chains(ctx, f0, 2.0, 2.0, 1.0, 1.0, seeds)
+    for ch in (a, b):
+        ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    for rep in range(3):
+        a.sweep(16)
+        b.sweep_graph(16)
+        sa, sb = a.state(), b.state()
+        assert np.array_equal(sa["f"], sb["f"]), rep
+        assert np.array_equal(sa["logZ"], sb["logZ"]), rep
+        assert np.array_equal(sa["naccept"], sb["naccept"]), rep
+    a.close()
+    b.close()
+    ctx.close()

@@ -33,12 +33,13 @@ struct reg_solver {
     struct smem {
         tri_smem<P, NB> t;
         double ev[NP];
-        double e2[NP];
+        double de[2 * NP];
+        int cnt[NT + 2];
         double scratch[3 * (NT / WAVE) + 8];
     };
     __device__ static const double* dptr(const smem& sm) { return sm.t.d; }
     __device__ static const double* eptr(const smem& sm) { return sm.t.e; }
-    __device__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
+    __device__ __forceinline__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
                                    double beta, double U, double mu_c, double* ws, double* vout, double* tauout,
                                    double* cexp, double* cfermi) {
         const int tid = threadIdx.x;
@@ -48,7 +49,7 @@ struct reg_solver {
         double A[NB * (NB + 1) / 2];
         assemble<P, NB>(A, hop, f_lds, U, mu_c, N, r, c);
         tridiagonalize<P, NB>(A, sm.t, N, vout, tauout);
-        tridiag_eigvals<NT>(sm.t.d, sm.t.e, sm.e2, N, sm.ev, sm.scratch);
+        tridiag_eigvals<NT>(sm.t.d, sm.t.e, sm.de, sm.cnt, N, sm.ev, sm.scratch);
         return logz_block<NT>(sm.ev, N, beta, cexp, cfermi, sm.scratch);
     }
 };
@@ -59,13 +60,14 @@ struct glob_solver {
     static constexpr int NMAX = 2048;
     struct smem {
         double xs[NMAX], vs[NMAX], ps[NMAX], wsv[NMAX];
-        double d[NMAX], e[NMAX], e2[NMAX], ev[NMAX];
+        double d[NMAX], e[NMAX], de[2 * NMAX], ev[NMAX];
+        int cnt[NT + 2];
         double scratch[3 * (NT / WAVE) + 8];
         double red[4];
     };
     __device__ static const double* dptr(const smem& sm) { return sm.d; }
     __device__ static const double* eptr(const smem& sm) { return sm.e; }
-    __device__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
+    __device__ __forceinline__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
                                    double beta, double U, double mu_c, double* __restrict__ W, double* vout,
                                    double* tauout, double* cexp, double* cfermi) {
         const int tid = threadIdx.x;
@@ -142,7 +144,7 @@ struct glob_solver {
             if (tid == 0) { if (N >= 2) tauout[N - 2] = 0.0; tauout[N - 1] = 0.0; }
         }
         __syncthreads();
-        tridiag_eigvals<NT>(sm.d, sm.e, sm.e2, N, sm.ev, sm.scratch);
+        tridiag_eigvals<NT>(sm.d, sm.e, sm.de, sm.cnt, N, sm.ev, sm.scratch);
         return logz_block<NT>(sm.ev, N, beta, cexp, cfermi, sm.scratch);
     }
 };
@@ -156,7 +158,9 @@ __global__ void __launch_bounds__(S::NT) k_eig_logz(eig_args a) {
     __shared__ uint8_t f_lds[S::NMAX];
     const int tid = threadIdx.x;
     const int N = a.N;
-    for (int bidx = blockIdx.x; bidx < a.B; bidx += gridDim.x) {
+    // one workgroup per configuration (a grid-stride loop here costs ~25 spilled VGPRs)
+    {
+        const int bidx = blockIdx.x;
         for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? (uint8_t)(a.f[(size_t)bidx * N + i] != 0) : 0;
         __syncthreads();
         double* ws = a.ws ? a.ws + (size_t)blockIdx.x * N * N : nullptr;
@@ -206,7 +210,8 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     __shared__ int ctl_i[8];
     const int tid = threadIdx.x;
     const int N = a.N;
-    for (int ch = blockIdx.x; ch < a.B; ch += gridDim.x) {
+    {
+        const int ch = blockIdx.x;  // one workgroup per chain
         uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
         for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
         for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;

@@ -21,20 +21,60 @@ namespace fks {
 
 constexpr int WAVE = 64;
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
+// Cross-lane moves on the VALU (no LDS traffic): DPP within 16-lane rows,
+// v_permlane16/32_swap across rows.  A double moves as two dwords.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    int2 w = *reinterpret_cast<int2*>(&v);
+    w.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, 0xF, 0xF, false);
+    w.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, 0xF, 0xF, false);
+    return *reinterpret_cast<double*>(&w);
 }
-__device__ inline double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, WAVE));
-    return v;
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane i <- 7-i within 8
+constexpr int DPP_MIRROR = 0x140;      // lane i <- 15-i within 16
+// (row r, row r^1) pair of values after v_permlane16_swap: .x holds the even
+// row's value in both rows, .y the odd row's (same for the 32-lane halves).
+__device__ __forceinline__ double2 swap_rows16(double v) {
+    int2 w = *reinterpret_cast<int2*>(&v);
+    auto a = __builtin_amdgcn_permlane16_swap(w.x, w.x, false, false);
+    auto b = __builtin_amdgcn_permlane16_swap(w.y, w.y, false, false);
+    int2 lo = make_int2(a[0], b[0]), hi = make_int2(a[1], b[1]);
+    return make_double2(*reinterpret_cast<double*>(&lo), *reinterpret_cast<double*>(&hi));
 }
-__device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
-    return v;
+__device__ __forceinline__ double2 swap_halves32(double v) {
+    int2 w = *reinterpret_cast<int2*>(&v);
+    auto a = __builtin_amdgcn_permlane32_swap(w.x, w.x, false, false);
+    auto b = __builtin_amdgcn_permlane32_swap(w.y, w.y, false, false);
+    int2 lo = make_int2(a[0], b[0]), hi = make_int2(a[1], b[1]);
+    return make_double2(*reinterpret_cast<double*>(&lo), *reinterpret_cast<double*>(&hi));
+}
+// Butterfly all-reduce over the 64 lanes; every lane ends with the same bits.
+template <class Op>
+__device__ __forceinline__ double wave_reduce(double v, Op op) {
+    v = op(v, dpp_f64<DPP_XOR1>(v));
+    v = op(v, dpp_f64<DPP_XOR2>(v));
+    v = op(v, dpp_f64<DPP_HALF_MIRROR>(v));
+    v = op(v, dpp_f64<DPP_MIRROR>(v));
+    double2 t = swap_rows16(v);
+    v = op(t.x, t.y);
+    t = swap_halves32(v);
+    return op(t.x, t.y);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    return wave_reduce(v, [](double a, double b) { return a + b; });
+}
+__device__ __forceinline__ double wave_min(double v) {
+    return wave_reduce(v, [](double a, double b) { return fmin(a, b); });
+}
+__device__ __forceinline__ double wave_max(double v) {
+    return wave_reduce(v, [](double a, double b) { return fmax(a, b); });
+}
+// Sum over aligned groups of 4 lanes (all four lanes get the sum).
+__device__ __forceinline__ double quad_sum(double v) {
+    v += dpp_f64<DPP_XOR1>(v);
+    return v + dpp_f64<DPP_XOR2>(v);
 }
 
 // Knuth TwoSum based pair (s, c) addition: (s1,c1) + (s2,c2).
@@ -62,6 +102,24 @@ __device__ inline void wave_sum_comp(double& s, double& c) {
 // log(1 + e^{-x}) without overflow.
 __device__ inline double log1p_exp_neg(double x) {
     return x > 0.0 ? log1p(exp(-x)) : -x + log1p(exp(x));
+}
+
+// Broadcast lane 0's value of a wave-uniform double into scalar registers.
+__device__ inline double uniform(double v) {
+    int2 w = *reinterpret_cast<int2*>(&v);
+    w.x = __builtin_amdgcn_readfirstlane(w.x);
+    w.y = __builtin_amdgcn_readfirstlane(w.y);
+    return *reinterpret_cast<double*>(&w);
+}
+
+// 1/q from v_rcp_f64 refined by two Newton-Raphson steps (full double precision
+// for normal q; callers keep |q| >= pivmin).
+__device__ inline double rcp_refined(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    double e = fma(-q, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-q, r, 1.0);
+    return fma(r, e, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -93,7 +151,7 @@ struct tri_smem {
 
 // Assemble H(f) = hop + diag(-mu_c + U f) into the thread's slots.
 template <int P, int NB>
-__device__ inline void assemble(double (&A)[NB * (NB + 1) / 2], const double* __restrict__ hop,
+__device__ __forceinline__ void assemble(double (&A)[NB * (NB + 1) / 2], const double* __restrict__ hop,
                                 const uint8_t* f_lds, double U, double mu_c, int N, int r, int c) {
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
@@ -112,7 +170,7 @@ __device__ inline void assemble(double (&A)[NB * (NB + 1) / 2], const double* __
 
 // Write column kk (entries below the diagonal) to xbuf and the diagonal to d.
 template <int P, int NB>
-__device__ inline void emit_column(const double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& sm, double* xbuf,
+__device__ __forceinline__ void emit_column(const double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& sm, double* xbuf,
                                    int kk, int N, int r, int c) {
     if (c != kk % P) return;
     const int bb = kk / P;
@@ -133,7 +191,7 @@ __device__ inline void emit_column(const double (&A)[NB * (NB + 1) / 2], tri_sme
 // If vout != nullptr the Householder vectors are stored row-wise:
 // vout[k*N + i] = v_k[i] (v_k[k+1] = 1, zero above), tauout[k] = tau_k.
 template <int P, int NB>
-__device__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& sm, int N,
+__device__ __forceinline__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& sm, int N,
                                double* __restrict__ vout, double* __restrict__ tauout) {
     using sh = tri_shape<P, NB>;
     const int tid = threadIdx.x;
@@ -158,8 +216,8 @@ __device__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& 
         double tau = 0.0, beta = alpha, scal = 0.0;
         if (xn2 != 0.0) {
             beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-            tau = (beta - alpha) / beta;
-            scal = 1.0 / (alpha - beta);
+            tau = (beta - alpha) * rcp_refined(beta);
+            scal = rcp_refined(alpha - beta);
         }
         if (tid == 0) sm.e[k] = beta;
         if (vout) {
@@ -167,34 +225,45 @@ __device__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& 
                 vout[(size_t)k * N + i] = (i == k + 1) ? 1.0 : ((i > k + 1) ? xk[i] * scal : 0.0);
             if (tid == 0) tauout[k] = tau;
         }
+        // wave-uniform scalars live in SGPRs (scalar branches, fewer VGPRs)
+        tau = uniform(tau);
+        scal = uniform(scal);
         const int a0 = (k + 1) / P;
         auto vfun = [&](int i) -> double {
             return (i == k + 1) ? 1.0 : ((i > k + 1 && i < N) ? xk[i] * scal : 0.0);
         };
-        double vc[NB];
         if (tau != 0.0) {
-            // --- symv partials
-            double pcol[NB];
+            // --- symv partials, two passes so that one NB-vector is live at a time.
+            // rows: part[i][c] = sum_b A[i][c+Pb] v[c+Pb] over the thread's slots
+            {
+                double vc[NB];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) { vc[b] = (b >= a0) ? vfun(c + P * b) : 0.0; pcol[b] = 0.0; }
+                for (int b = 0; b < NB; ++b) vc[b] = (b >= a0) ? vfun(c + P * b) : 0.0;
 #pragma unroll
-            for (int a = 0; a < NB; ++a) {
-                if (a < a0) continue;
-                const double vra = vfun(r + P * a);
-                double s = 0.0;
+                for (int a = 0; a < NB; ++a) {
+                    if (a < a0) continue;
+                    double s = 0.0;
 #pragma unroll
-                for (int b = 0; b <= a; ++b) {
-                    if (b < a0) continue;
-                    const double av = A[slot(a, b)];
-                    s = fma(av, vc[b], s);
-                    if (b < a) pcol[b] = fma(av, vra, pcol[b]);
+                    for (int b = 0; b <= a; ++b) {
+                        if (b < a0) continue;
+                        s = fma(A[slot(a, b)], vc[b], s);
+                    }
+                    sm.part[r + P * a][c] = s;
                 }
-                sm.part[r + P * a][c] = s;
             }
+            // columns (strictly lower blocks only): part[j][P+r] = sum_a A[r+Pa][j] v[r+Pa]
+            {
+                double vr[NB];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (b < a0) continue;
-                sm.part[c + P * b][P + r] = pcol[b];
+                for (int a = 0; a < NB; ++a) vr[a] = (a > a0) ? vfun(r + P * a) : 0.0;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if (b < a0) continue;
+                    double s = 0.0;
+#pragma unroll
+                    for (int a = b + 1; a < NB; ++a) s = fma(A[slot(a, b)], vr[a], s);
+                    sm.part[c + P * b][P + r] = s;
+                }
             }
         }
         __syncthreads();
@@ -206,10 +275,16 @@ __device__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& 
             if (i < N && i > k) {
                 const double* row = sm.part[i];
 #pragma unroll
-                for (int u = 0; u < CH; ++u) s += row[q * CH + u];
+                for (int u = 0; u < CH; ++u) s += row[q + sh::TPR * u];  // interleaved: conflict-free
             }
+            if constexpr (sh::TPR == 4) {
+                s = quad_sum(s);
+            } else if constexpr (sh::TPR == 2) {
+                s += dpp_f64<DPP_XOR1>(s);
+            } else {
 #pragma unroll
-            for (int o = sh::TPR / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o, WAVE);
+                for (int o = sh::TPR / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o, WAVE);
+            }
             double pi = (i < N && i > k) ? tau * s : 0.0;
             if (q == 0 && i < sh::NP) sm.p[i] = pi;
             double pv = (q == 0 && i < N) ? pi * vfun(i) : 0.0;
@@ -222,19 +297,31 @@ __device__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& 
             double pvt = 0.0;  // fixed summation order: bitwise reproducible
 #pragma unroll
             for (int w = 0; w < sh::NW; ++w) pvt += sm.red[8 + w];
-            const double coef = 0.5 * tau * pvt;
-            double wc[NB];
+            const double coef = uniform(0.5 * tau * pvt);
+            // two halves of the block columns keep 2*NB/2 vector values live
+            constexpr int HB = (NB + 1) / 2;
 #pragma unroll
-            for (int b = 0; b < NB; ++b) wc[b] = (b >= a0) ? sm.p[c + P * b] - coef * vc[b] : 0.0;
+            for (int h = 0; h < 2; ++h) {
+                const int b_lo = h * HB, b_hi = (h == 0) ? HB : NB;
+                double vc[HB], wc[HB];
 #pragma unroll
-            for (int a = 0; a < NB; ++a) {
-                if (a < a0) continue;
-                const double vra = vfun(r + P * a);
-                const double wra = sm.p[r + P * a] - coef * vra;
+                for (int bb = 0; bb < HB; ++bb) {
+                    const int b = b_lo + bb;
+                    const bool act = b < b_hi && b >= a0;
+                    vc[bb] = act ? vfun(c + P * b) : 0.0;
+                    wc[bb] = act ? sm.p[c + P * b] - coef * vc[bb] : 0.0;
+                }
 #pragma unroll
-                for (int b = 0; b <= a; ++b) {
-                    if (b < a0) continue;
-                    A[slot(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[slot(a, b)]));
+                for (int a = 0; a < NB; ++a) {
+                    if (a < a0 || a < b_lo) continue;
+                    const double vra = vfun(r + P * a);
+                    const double wra = sm.p[r + P * a] - coef * vra;
+#pragma unroll
+                    for (int bb = 0; bb < HB; ++bb) {
+                        const int b = b_lo + bb;
+                        if (b >= b_hi || b > a || b < a0) continue;
+                        A[slot(a, b)] = fma(-vra, wc[bb], fma(-wra, vc[bb], A[slot(a, b)]));
+                    }
                 }
             }
         }
@@ -266,36 +353,117 @@ __device__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], tri_smem<P, NB>& 
 }
 
 // ---------------------------------------------------------------------------
-// Eigenvalues of the symmetric tridiagonal (d, e) by Sturm-count bisection,
-// one eigenvalue per thread, ascending.  scratch: >= 2*NT/WAVE doubles.
-// e2 (LDS, N entries) receives e^2.
+// Eigenvalues of the symmetric tridiagonal T = (d, e), ascending.
+//
+// Sturm recurrence q_0 = d_0 - s, q_i = (d_i - s) - e_{i-1}^2 / q_{i-1}: the
+// number of negative q_i is the number of eigenvalues below s.  Carrying q'
+// and q'' along gives G = sum_k 1/(s - l_k) = d/ds log|det(T - s)| and
+// H = sum_k 1/(s - l_k)^2 = -dG/ds, the inputs of Laguerre's iteration for the
+// real-rooted polynomial det(T - s) (cubic convergence, monotone towards the
+// adjacent root in the chosen direction).
+//   1. every thread evaluates the count at one of NT equally spaced shifts in
+//      the Gershgorin interval (multisection);
+//   2. thread j takes the cell holding eigenvalue j, bisects until the cell
+//      holds that eigenvalue alone, then takes Laguerre steps towards it
+//      (the Sturm count says which side it is on); a step that leaves the
+//      bracket is replaced by bisection;
+//   3. once a step is tiny, one two-shift Sturm pass checks that l_j lies in
+//      (s - t, s + t]; if not, the bracket is updated and the loop goes on.
+// de: LDS, >= 2*N doubles; cnt: LDS, >= NT+2 ints; scratch >= 3*NT/WAVE.
 // ---------------------------------------------------------------------------
-__device__ inline int sturm_count(const double* __restrict__ d, const double* __restrict__ e2, int N, double sigma,
-                                  double pivmin) {
-    int cnt = 0;
-    double q = d[0] - sigma;
+struct sturm_res {
+    int count;
+    double G, H;
+};
+
+__device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ de, int N, double sigma,
+                                                    double pivmin) {
+    double q = de[0].x - sigma;
     if (fabs(q) < pivmin) q = -pivmin;
-    cnt += (q < 0.0);
-    for (int i = 1; i < N; ++i) {
-        q = (d[i] - sigma) - e2[i - 1] / q;
+    double dq = -1.0, ddq = 0.0, G = 0.0, Hm = 0.0;
+    int cnt = (q < 0.0);
+    int i = 1;
+    auto step = [&](double2 t) {
+        const double r = rcp_refined(q);
+        const double a = dq * r;
+        G += a;
+        Hm = fma(ddq, r, Hm) - a * a;
+        const double u = t.y * r;
+        const double ur = u * r;
+        ddq = ur * fma(-2.0 * dq, a, ddq);
+        dq = fma(u, a, -1.0);
+        q = (t.x - sigma) - u;
         if (fabs(q) < pivmin) q = -pivmin;
         cnt += (q < 0.0);
+    };
+    // loads issued four at a time ahead of the dependent chain
+    for (; i + 3 < N; i += 4) {
+        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
+        step(t0); step(t1); step(t2); step(t3);
     }
-    return cnt;
+    for (; i < N; ++i) step(de[i]);
+    const double r = rcp_refined(q);
+    const double a = dq * r;
+    G += a;
+    Hm = fma(ddq, r, Hm) - a * a;
+    return {cnt, G, -Hm};
+}
+
+// counts at two shifts in one pass (independent chains interleave)
+__device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int N, double s1, double s2,
+                                             double pivmin, int& c1, int& c2) {
+    double q1 = de[0].x - s1, q2 = de[0].x - s2;
+    if (fabs(q1) < pivmin) q1 = -pivmin;
+    if (fabs(q2) < pivmin) q2 = -pivmin;
+    c1 = (q1 < 0.0);
+    c2 = (q2 < 0.0);
+    for (int i = 1; i < N; ++i) {
+        const double2 t = de[i];
+        q1 = (t.x - s1) - t.y * rcp_refined(q1);
+        q2 = (t.x - s2) - t.y * rcp_refined(q2);
+        if (fabs(q1) < pivmin) q1 = -pivmin;
+        if (fabs(q2) < pivmin) q2 = -pivmin;
+        c1 += (q1 < 0.0);
+        c2 += (q2 < 0.0);
+    }
+}
+
+__device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N, double s, double pivmin) {
+    double q = de[0].x - s;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int c = (q < 0.0);
+    int i = 1;
+    for (; i + 3 < N; i += 4) {
+        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
+        q = (t0.x - s) - t0.y * rcp_refined(q); if (fabs(q) < pivmin) q = -pivmin; c += (q < 0.0);
+        q = (t1.x - s) - t1.y * rcp_refined(q); if (fabs(q) < pivmin) q = -pivmin; c += (q < 0.0);
+        q = (t2.x - s) - t2.y * rcp_refined(q); if (fabs(q) < pivmin) q = -pivmin; c += (q < 0.0);
+        q = (t3.x - s) - t3.y * rcp_refined(q); if (fabs(q) < pivmin) q = -pivmin; c += (q < 0.0);
+    }
+    for (; i < N; ++i) {
+        const double2 t = de[i];
+        q = (t.x - s) - t.y * rcp_refined(q);
+        if (fabs(q) < pivmin) q = -pivmin;
+        c += (q < 0.0);
+    }
+    return c;
 }
 
 template <int NT>
-__device__ void tridiag_eigvals(const double* d, const double* e, double* e2, int N, double* ev_out,
-                                double* scratch) {
+__device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e, double* de_buf, int* cnt, int N,
+                                                double* ev_out, double* scratch) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1), wid = tid / WAVE;
+    double2* de = reinterpret_cast<double2*>(de_buf);
     double lo = INFINITY, hi = -INFINITY, em = 0.0;
     for (int i = tid; i < N; i += NT) {
         double el = i > 0 ? fabs(e[i - 1]) : 0.0;
         double er = i + 1 < N ? fabs(e[i]) : 0.0;
         lo = fmin(lo, d[i] - el - er);
         hi = fmax(hi, d[i] + el + er);
-        if (i + 1 < N) { e2[i] = e[i] * e[i]; em = fmax(em, e2[i]); }
+        double e2 = i > 0 ? e[i - 1] * e[i - 1] : 0.0;
+        de[i] = make_double2(d[i], e2);
+        em = fmax(em, e2);
     }
     lo = wave_min(lo);
     hi = wave_max(hi);
@@ -314,16 +482,57 @@ __device__ void tridiag_eigvals(const double* d, const double* e, double* e2, in
     const double pivmin = safemin * fmax(1.0, em);
     lo -= 2.0 * eps * tnorm * N + 2.0 * pivmin;
     hi += 2.0 * eps * tnorm * N + 2.0 * pivmin;
-    const double atol = 2.0 * eps * tnorm;  // absolute width at which bisection stops
+    const double atol = 2.0 * eps * tnorm;
+    const double vtol = 8.0 * eps * fmax(tnorm, 1.0);   // verification half-window
+    const double small = 1e-6 * fmax(tnorm, 1.0);        // Laguerre step below which to verify
+    const double h = (hi - lo) / (double)(NT + 1);
+    // 1. multisection: cnt[m] = count at lo + m*h, m = 0..NT+1
+    cnt[tid + 1] = sturm_count(de, N, lo + (double)(tid + 1) * h, pivmin);
+    if (tid == 0) { cnt[0] = 0; cnt[NT + 1] = N; }
+    __syncthreads();
+    const double nd = (double)N;
     for (int j = tid; j < N; j += NT) {
-        double l = lo, h = hi;
-        for (int it = 0; it < 128; ++it) {
-            double mid = 0.5 * (l + h);
-            if (h - l <= atol + 2.0 * eps * fmax(fabs(l), fabs(h)) || mid == l || mid == h) break;
-            int cnt = sturm_count(d, e2, N, mid, pivmin);
-            if (cnt >= j + 1) h = mid; else l = mid;
+        int ml = 0, mh = NT + 1;  // invariant: cnt[ml] <= j < cnt[mh]
+        while (mh - ml > 1) {
+            int mid = (ml + mh) >> 1;
+            if (cnt[mid] > j) mh = mid; else ml = mid;
         }
-        ev_out[j] = 0.5 * (l + h);
+        double l = lo + (double)ml * h, u = (mh == NT + 1) ? hi : lo + (double)mh * h;
+        int cl = cnt[ml], cu = cnt[mh];
+        double s = 0.5 * (l + u);
+        double res = s;
+        for (int it = 0; it < 200; ++it) {
+            if (u - l <= atol + 2.0 * eps * fmax(fabs(l), fabs(u))) { res = 0.5 * (l + u); break; }
+            const sturm_res sr = sturm_laguerre(de, N, s, pivmin);
+            const bool left = sr.count > j;  // eigenvalue j lies below s
+            if (left) { u = s; cu = sr.count; } else { l = s; cl = sr.count; }
+            double next = 0.5 * (l + u);
+            if (cu - cl == 1) {
+                const double disc = fmax((nd - 1.0) * (nd * sr.H - sr.G * sr.G), 0.0);
+                const double sq = sqrt(disc);
+                // the two Laguerre candidates lie on either side of s; take the one towards eigenvalue j
+                const double den = left ? (sr.G + sq) : (sr.G - sq);
+                if (den != 0.0 && isfinite(den)) {
+                    const double sn = s - nd / den;
+                    if (sn > l && sn < u) {
+                        if (fabs(sn - s) <= small) {
+                            int ca, cb;
+                            sturm_count2(de, N, sn - vtol, sn + vtol, pivmin, ca, cb);
+                            if (ca <= j && cb > j) { res = sn; break; }
+                            if (ca > j) { u = sn - vtol; cu = ca; }
+                            else if (cb <= j) { l = sn + vtol; cl = cb; }
+                            next = 0.5 * (l + u);
+                        } else {
+                            next = sn;
+                        }
+                    }
+                }
+            }
+            if (next == s) { res = s; break; }
+            s = next;
+            res = s;
+        }
+        ev_out[j] = res;
     }
     __syncthreads();
 }
@@ -332,7 +541,7 @@ __device__ void tridiag_eigvals(const double* d, const double* e, double* e2, in
 // also cached_exp[k] = e^{beta eps_k}, cached_fermi[k] = 1/(1+e^{beta eps_k})
 // (src/configuration.cpp:324-334).  Result returned to every thread.
 template <int NT>
-__device__ double logz_block(const double* ev, int N, double beta, double* cexp, double* cfermi, double* scratch) {
+__device__ __noinline__ double logz_block(const double* ev, int N, double beta, double* cexp, double* cfermi, double* scratch) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1), wid = tid / WAVE;
     double s = 0.0, c = 0.0;

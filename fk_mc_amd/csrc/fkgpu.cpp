@@ -95,10 +95,13 @@ bool finite_all(const double* p, int n) {
     return true;
 }
 
+// One workgroup per configuration / chain; the global-memory solver keeps one
+// N x N workspace per workgroup, so its grid is one batch of resident groups
+// and the kernel is launched repeatedly over slices of the batch.
 int grid_for(fkgpu_ctx* ctx, int kind, int B) {
-    int per_cu = fk_solver_threads(kind) <= 256 ? 4 : 1;
-    int g = ctx->num_cu * per_cu;
-    return B < g ? B : g;
+    (void)ctx;
+    (void)kind;
+    return B;
 }
 
 // Site index order of hypercubic_lattice::index_to_pos: last coordinate fastest.

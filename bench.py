@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark: MC moves/s of the batched FP64 per-move diagonalisation path.
+
+Workload (the batched FP64 configuration): cubic2d L=16 (N=256 sites),
+1024 independent chains, beta_c = 0.5 * 40**(c/1023) (log-spaced in [0.5, 20]),
+U_c = {1,2,4,8}[c mod 4], mu_c = mu_f = U/2, f drawn Bernoulli(0.5) per chain,
+moves: single-site flip (move_addremove) and nearest-neighbour exchange at
+probability 0.5 / 0.5.  A step is one Metropolis move of every chain: one launch
+of the fused kernel (proposal, H assembly, FP64 Householder tridiagonalisation,
+tridiagonal eigenvalues, logZ, accept test).  Inputs are resident in HBM.
+
+Printed: one JSON line with the contract fields plus
+  roofline      FP64 rate of the sweep kernel: (4/3) N^3 flop per eigensolve
+                x eigensolves done / kernel time (HIP events on the library
+                stream) against MI355X's 78.6 TFLOP/s FP64 peak;
+  plain         the single-chain drop-in path (calc_hamiltonian + calc_ed
+                through the C ABI with host buffers, one call per move);
+  cpu_baseline  the CPU restatement of the reference path (oracle, C++,
+                one chain per std::thread) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def workload(L=16, B=1024, seed=20240601):
+    N = L * L
+    rng = np.random.default_rng(seed)
+    f = (rng.random((B, N)) < 0.5).astype(np.int32)
+    c = np.arange(B)
+    beta = 0.5 * 40.0 ** (c / (B - 1))
+    U = np.array([1.0, 2.0, 4.0, 8.0])[c % 4]
+    seeds = (seed + c).astype(np.uint32)
+    return N, f, beta, U, seeds
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=160)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--chains", type=int, default=1024)
+    ap.add_argument("--L", type=int, default=16)
+    ap.add_argument("--plain-moves", type=int, default=40)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); chains are independent, "
+              "so several GPUs would each run their own share of the batch.", file=sys.stderr)
+        sys.exit(2)
+
+    import fk_mc_amd as fk
+
+    L, B = args.L, args.chains
+    N, f, beta, U, seeds = workload(L, B)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, f, beta, U, U / 2, U / 2, seeds)
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+
+    # warmup (untimed), then exactly `steps` move steps between synchronisations
+    if args.warmup:
+        ch.sweep(args.warmup)
+    s0 = ch.state()
+    t0 = time.perf_counter()
+    ch.sweep(args.steps)  # synchronous: returns after the last launch completed
+    t1 = time.perf_counter()
+    kern_ms = ch.last_ms()
+    s1 = ch.state()
+    wall_ms = (t1 - t0) * 1e3
+    moves = B * args.steps
+    value = moves / (wall_ms / 1e3)
+    solves = int((s1["nsolve"] - s0["nsolve"]).sum())
+    acc = int((s1["naccept"] - s0["naccept"]).sum())
+    flop = solves * (4.0 / 3.0) * N ** 3
+    achieved = flop / (kern_ms / 1e3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                "kernel": "fkk::k_sweep<reg_solver<32,8>>", "kernel_ms_per_launch": round(kern_ms / args.steps, 4),
+                "flop_per_launch": flop / args.steps, "eigensolves_per_launch": solves / args.steps}
+
+    # plain single-chain drop-in: one calc_hamiltonian+calc_ed per move, host buffers
+    plain = None
+    if args.plain_moves > 0:
+        rng = np.random.default_rng(5)
+        fp = f[:1].copy()
+        ctx.eig_logz(fp, beta[0], U[0], U[0] / 2)
+        tp0 = time.perf_counter()
+        for _ in range(args.plain_moves):
+            i = rng.integers(N)
+            fp[0, i] ^= 1
+            ctx.eig_logz(fp, beta[0], U[0], U[0] / 2)
+        tp1 = time.perf_counter()
+        plain_rate = args.plain_moves / (tp1 - tp0)
+        plain = {"moves_per_s": round(plain_rate, 2), "ms_per_move": round((tp1 - tp0) * 1e3 / args.plain_moves, 4),
+                 "batched_over_plain": round(value / plain_rate, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(L, args.cpu_seconds, ch, ctx, f, beta, U, seeds)
+
+    out = {
+        "metric": "MC moves/sec, cubic2d L=16, 1024 batched chains vs plain; dOmega err vs FP64",
+        "value": round(value, 2),
+        "unit": "moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched FP64 Metropolis, cubic2d L={L} (N={N}), {B} chains, "
+                               "beta log-spaced [0.5,20], U in {1,2,4,8}, mu=U/2, f~Bernoulli(0.5), "
+                               "moves addremove/nn-exchange 0.5/0.5, one move per chain per step",
+                   "lattice": "cubic2d", "L": L, "N": N, "chains": B, "global_batch": B},
+        "accept_rate": round(acc / moves, 4),
+        "roofline": roofline,
+        "plain": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ch.close()
+    ctx.close()
+
+
+def cpu_baseline(L, seconds, ch, ctx, f, beta, U, seeds):
+    """Oracle (C++ restatement of the reference CPU path) on the host cores, and
+    the dOmega error of a short traced device sweep against it."""
+    from oracle import oracle as orc
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    # calibrate one move, then size the sample to about `seconds` of CPU work per thread
+    t = time.perf_counter()
+    orc.bench_chains(orc.KIND_HYPERCUBIC, 2, L, 2.0, 2.0, 1.0, 99, 1, 3, t=1.0, tp=0.0)
+    per_move = (time.perf_counter() - t) / 3
+    nmoves = max(5, int(seconds / per_move))
+    t = time.perf_counter()
+    done = orc.bench_chains(orc.KIND_HYPERCUBIC, 2, L, 2.0, 2.0, 1.0, 99, threads, nmoves, t=1.0, tp=0.0)
+    dt = time.perf_counter() - t
+    # dOmega = -(logZ_new - logZ_old)/beta of proposals: device trace vs oracle chain
+    import fk_mc_amd as fk
+    nchk, msteps = 4, 40
+    sub = fk.Chains(ctx, f[:nchk], beta[:nchk], U[:nchk], U[:nchk] / 2, U[:nchk] / 2, seeds[:nchk])
+    sub.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    tr = sub.sweep(msteps, trace=True)
+    sub.close()
+    errs, abs_errs = [], []
+    for c in range(nchk):
+        ref = orc.run_chain(orc.KIND_HYPERCUBIC, 2, L, beta[c], U[c], U[c] / 2, U[c] / 2, int(seeds[c]), f[c],
+                            [(orc.MOVE_ADDREMOVE, 0.5), (orc.MOVE_HOP, 0.5)], msteps, t=1.0, tp=0.0,
+                            forced=tr["acc"][:, c].astype(np.int32))
+        solved = ref["logz"] != 0.0
+        d_gpu = -(tr["logz"][solved, c] - np.concatenate([[0], ref["logz_cur"][:-1]])[solved]) / beta[c]
+        d_ref = -(ref["logz"][solved] - np.concatenate([[0], ref["logz_cur"][:-1]])[solved]) / beta[c]
+        if solved[0]:
+            d_gpu, d_ref = d_gpu[1:], d_ref[1:]
+        # relative to max(|dOmega|, 1e-6) so that near-zero differences do not blow up
+        scale = np.maximum(np.abs(d_ref), 1e-6)
+        errs.append(np.max(np.abs(d_gpu - d_ref) / scale) if len(d_ref) else 0.0)
+        abs_errs.append(np.max(np.abs(d_gpu - d_ref)) if len(d_ref) else 0.0)
+    return {"value": round(done / dt, 3), "unit": "moves/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} chains x {nmoves} addremove moves, cubic2d L={L}, beta=2, U=2, "
+                      f"FP64 Householder+implicit QR per move ({dt:.1f} s)",
+            "single_core_moves_per_s": round(1.0 / per_move, 3),
+            "dOmega_max_rel_err_vs_fp64_oracle": float(max(errs)),
+            "dOmega_max_abs_err_vs_fp64_oracle": float(max(abs_errs))}
+
+
+if __name__ == "__main__":
+    main()

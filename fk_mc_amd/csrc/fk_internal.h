@@ -65,3 +65,16 @@ int fk_solver_threads(int kind);
 int fk_solver_needs_ws(int kind);
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
+
+hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,
+                           const double* tau, double* Z, double* scratch, hipStream_t st);
+hipError_t fk_launch_ipr(int B, int N, const double* V, double* out, hipStream_t st);
+hipError_t fk_launch_stiffness(int B, int N, int stride0, int L0, double t, const double* V, const double* ev,
+                               const double* fermi, int nw, const double* wgrid, double offset, double* JV,
+                               double* mJ, double* tdiag, double* stiff, double* cond, hipStream_t st);
+hipError_t fk_launch_dos(int S, int N, const double* spec, int nw, const double* wgrid, double eta, double* out,
+                         hipStream_t st);
+hipError_t fk_launch_adjoint(int B, int N, const double* V, const double* fermi, const int32_t* f,
+                             const double* beta, const int32_t* nbr, const int32_t* z, int zmax, int nt_bonds,
+                             double* dens, double* bond_t, double* bond_tp, double* out, double* rho_full,
+                             hipStream_t st);

@@ -1,0 +1,294 @@
+// fk_measures.hip -- eigenvector measures and the adjoint of Omega.
+//
+//   k_ipr            measure_ipr::accumulate (include/fk_mc/measures/ipr.hpp:39-56):
+//                    ipr_k = ||v_k||_4 / ||v_k||_2^2 (lpNorm<4>, not its 4th power).
+//   k_stiff_prep     measure_stiffness (include/fk_mc/measures/stiffness.hpp:69-153):
+//                    (J V) with J_{i,left} = -1, J_{i,right} = +1 along direction 0,
+//                    and the kinetic diagonal (V^T T_x V)_kk.
+//   k_gemm_tn        C = A^T B (N x N, FP64, LDS-tiled), used for V^T (J V).
+//   k_stiff_pairs    the pair sums of stiffness.hpp:156-185: V = sum 2 sigma/(e_k - e_l)
+//                    and the Lorentzian conductivity on a frequency grid.
+//   k_dos            DOS rho(w) = (1/N) sum_k (eta/pi)/((w - e_k)^2 + eta^2)
+//                    (prog/data_save.hxx:264-345, dos0_f).
+//   k_adjoint        dOmega/dH = rho = V f(Lambda) V^T restricted to the diagonal and
+//                    the hopping bonds: local densities, dOmega/dU, dOmega/dt,
+//                    dOmega/dtp, dOmega/dmu_c, and dN_c/dmu_c = beta sum f(1-f).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fk_internal.h"
+#include "fk_solve.h"
+
+namespace fkm {
+
+using namespace fks;
+
+__global__ void k_ipr(int B, int N, const double* __restrict__ V, double* __restrict__ out) {
+    const int b = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || k >= N) return;
+    const double* v = V + (size_t)b * N * N;
+    double s2 = 0.0, s4 = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double x = v[(size_t)i * N + k];
+        const double x2 = x * x;
+        s2 += x2;
+        s4 = fma(x2, x2, s4);
+    }
+    out[(size_t)b * N + k] = sqrt(sqrt(s4)) / s2;
+}
+
+// JV[i][k] = V[right(i)][k] - V[left(i)][k];  tdiag[k] = sum_i V[i][k] * (-t)(V[left][k] + V[right][k])
+__global__ void k_stiff_prep(int B, int N, int stride0, int L0, double t, const double* __restrict__ V,
+                             double* __restrict__ JV, double* __restrict__ tdiag) {
+    const int b = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || k >= N) return;
+    const double* v = V + (size_t)b * N * N;
+    double* jv = JV + (size_t)b * N * N;
+    double td = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const int x = i / stride0, rest = i % stride0;
+        const int il = ((x - 1 + L0) % L0) * stride0 + rest;
+        const int ir = ((x + 1) % L0) * stride0 + rest;
+        const double vl = v[(size_t)il * N + k], vr = v[(size_t)ir * N + k];
+        jv[(size_t)i * N + k] = vr - vl;
+        td = fma(v[(size_t)i * N + k], -t * vl - t * vr, td);
+    }
+    tdiag[(size_t)b * N + k] = td;
+}
+
+// C[b] = A[b]^T * B[b], all N x N row-major.  16x16 output tiles, k-tiles in LDS.
+constexpr int GT = 16;
+__global__ void __launch_bounds__(GT * GT) k_gemm_tn(int B, int N, const double* __restrict__ A,
+                                                     const double* __restrict__ Bm, double* __restrict__ C) {
+    __shared__ double As[GT][GT + 1], Bs[GT][GT + 1];
+    const int b = blockIdx.z;
+    const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
+    const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
+    const double* a = A + (size_t)b * N * N;
+    const double* bb = Bm + (size_t)b * N * N;
+    double acc = 0.0;
+    for (int k0 = 0; k0 < N; k0 += GT) {
+        // As[kk][rr] = A[k0+kk][r0+rr]; Bs[kk][cc] = B[k0+kk][c0+cc]
+        const int kk = ty, cc = tx;
+        As[kk][cc] = (k0 + kk < N && r0 + cc < N) ? a[(size_t)(k0 + kk) * N + r0 + cc] : 0.0;
+        Bs[kk][cc] = (k0 + kk < N && c0 + cc < N) ? bb[(size_t)(k0 + kk) * N + c0 + cc] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < GT; ++u) acc = fma(As[u][ty], Bs[u][tx], acc);
+        __syncthreads();
+    }
+    if (r0 + ty < N && c0 + tx < N) C[(size_t)b * N * N + (size_t)(r0 + ty) * N + c0 + tx] = acc;
+}
+
+// Pair sums, one workgroup per matrix.  Threads stride over rows i; each row's
+// partial V is summed over j < i sequentially; partials reduced in fixed order.
+// cond[w] is accumulated by thread w (loop over all pairs, fixed order).
+__global__ void __launch_bounds__(256) k_stiff_pairs(int B, int N, const double* __restrict__ mJg,
+                                                     const double* __restrict__ evg, const double* __restrict__ fg,
+                                                     const double* __restrict__ tdg, int nw,
+                                                     const double* __restrict__ wgrid, double offset,
+                                                     double* __restrict__ stiff, double* __restrict__ cond) {
+    __shared__ double red[256];
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const int tid = threadIdx.x;
+    const double* mJ = mJg + (size_t)b * N * N;
+    const double* ev = evg + (size_t)b * N;
+    const double* fe = fg + (size_t)b * N;
+    const double PI = 3.14159265358979323846;
+    double vsum = 0.0, tsum = 0.0;
+    for (int i = tid; i < N; i += blockDim.x) {
+        tsum = fma(tdg[(size_t)b * N + i], fe[i], tsum);
+        for (int j = 0; j < i; ++j) {
+            if (!(fabs(ev[i] - ev[j]) > 1e-12)) continue;
+            const double sigma = PI * (fe[j] - fe[i]) * mJ[(size_t)j * N + i] * mJ[(size_t)i * N + j];
+            if (fabs(sigma) > 1e-13) vsum += 2.0 * sigma / (ev[i] - ev[j]);
+        }
+    }
+    red[tid] = vsum;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int u = 0; u < (int)blockDim.x; ++u) s += red[u];
+        red[0] = s;
+    }
+    __syncthreads();
+    const double V = red[0];
+    __syncthreads();
+    red[tid] = tsum;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int u = 0; u < (int)blockDim.x; ++u) s += red[u];
+        stiff[b] = (V - PI * s) / (double)N;
+    }
+    for (int w = tid; w < nw; w += blockDim.x) {
+        const double om = wgrid[w];
+        double acc = 0.0;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < i; ++j) {
+                if (!(fabs(ev[i] - ev[j]) > 1e-12)) continue;
+                const double sigma = PI * (fe[j] - fe[i]) * mJ[(size_t)j * N + i] * mJ[(size_t)i * N + j];
+                if (!(fabs(sigma) > 1e-13)) continue;
+                const double x1 = om + (ev[j] - ev[i]), x2 = om + (ev[i] - ev[j]);
+                acc += offset / PI / (x1 * x1 + offset * offset) * sigma;
+                acc += offset / PI / (x2 * x2 + offset * offset) * (-sigma);
+            }
+        cond[(size_t)b * nw + w] = acc;
+    }
+}
+
+__global__ void k_dos(int S, int N, const double* __restrict__ spec, int nw, const double* __restrict__ wgrid,
+                      double eta, double* __restrict__ out) {
+    const int s = blockIdx.y;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || w >= nw) return;
+    const double PI = 3.14159265358979323846;
+    const double om = wgrid[w];
+    const double* e = spec + (size_t)s * N;
+    double acc = 0.0;
+    for (int k = 0; k < N; ++k) {
+        const double x = om - e[k];
+        acc += (eta / PI) / (x * x + eta * eta);
+    }
+    out[(size_t)s * nw + w] = acc / (double)N;
+}
+
+// One thread per site i: rho_ii and the bond sums of row i.  Per-matrix
+// reductions of the per-site values are done by k_adjoint_reduce.
+__global__ void k_adjoint_sites(int B, int N, const double* __restrict__ V, const double* __restrict__ fermi,
+                                const int32_t* __restrict__ nbr, const int32_t* __restrict__ z, int zmax,
+                                int nt_bonds, double* __restrict__ dens, double* __restrict__ bond_t,
+                                double* __restrict__ bond_tp) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || i >= N) return;
+    const double* v = V + (size_t)b * N * N;
+    const double* f = fermi + (size_t)b * N;
+    const double* vi = v + (size_t)i * N;
+    double rii = 0.0;
+    for (int k = 0; k < N; ++k) rii = fma(vi[k] * vi[k], f[k], rii);
+    double bt = 0.0, btp = 0.0;
+    const int zi = z[i];
+    for (int s = 0; s < zi; ++s) {
+        const double* vj = v + (size_t)nbr[(size_t)i * zmax + s] * N;
+        double rij = 0.0;
+        for (int k = 0; k < N; ++k) rij = fma(vi[k] * vj[k], f[k], rij);
+        if (s < nt_bonds) bt += rij; else btp += rij;
+    }
+    dens[(size_t)b * N + i] = rii;
+    bond_t[(size_t)b * N + i] = bt;
+    bond_tp[(size_t)b * N + i] = btp;
+}
+
+// out[b*8 + ...] = {dOmega/dU, dOmega/dt, dOmega/dtp, dOmega/dmu_c, N_c, dN_c/dmu_c}
+__global__ void __launch_bounds__(256) k_adjoint_reduce(int B, int N, const int32_t* __restrict__ f,
+                                                        const double* __restrict__ fermi,
+                                                        const double* __restrict__ beta,
+                                                        const double* __restrict__ dens,
+                                                        const double* __restrict__ bond_t,
+                                                        const double* __restrict__ bond_tp,
+                                                        double* __restrict__ out) {
+    __shared__ double red[5][256];
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const int tid = threadIdx.x;
+    double su = 0, st = 0, stp = 0, snc = 0, sdn = 0;
+    for (int i = tid; i < N; i += blockDim.x) {
+        su += f[(size_t)b * N + i] ? dens[(size_t)b * N + i] : 0.0;
+        st += bond_t[(size_t)b * N + i];
+        stp += bond_tp[(size_t)b * N + i];
+        const double fk = fermi[(size_t)b * N + i];
+        snc += fk;
+        sdn += fk * (1.0 - fk);
+    }
+    red[0][tid] = su; red[1][tid] = st; red[2][tid] = stp; red[3][tid] = snc; red[4][tid] = sdn;
+    __syncthreads();
+    if (tid < 5) {
+        double s = 0.0;
+        for (int u = 0; u < (int)blockDim.x; ++u) s += red[tid][u];
+        red[tid][0] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double* o = out + (size_t)b * 8;
+        o[0] = red[0][0];                // dOmega/dU = sum_i n_f,i rho_ii
+        o[1] = -red[1][0];               // dOmega/dt: H_ij = -t on bonds
+        o[2] = -red[2][0];               // dOmega/dtp
+        o[3] = -red[3][0];               // dOmega/dmu_c = -sum_k f_k
+        o[4] = red[3][0];                // N_c
+        o[5] = beta[b] * red[4][0];      // dN_c/dmu_c
+        o[6] = 0.0;
+        o[7] = 0.0;
+    }
+}
+
+// rho = V diag(f) V^T (full matrix), 16x16 tiles.
+__global__ void __launch_bounds__(GT * GT) k_rho_full(int B, int N, const double* __restrict__ V,
+                                                      const double* __restrict__ fermi, double* __restrict__ R) {
+    __shared__ double As[GT][GT + 1], Bs[GT][GT + 1];
+    const int b = blockIdx.z;
+    const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
+    const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
+    const double* v = V + (size_t)b * N * N;
+    const double* f = fermi + (size_t)b * N;
+    double acc = 0.0;
+    for (int k0 = 0; k0 < N; k0 += GT) {
+        // As[rr][kk] = V[r0+rr][k0+kk] f[k0+kk], Bs[cc][kk] = V[c0+cc][k0+kk]
+        As[ty][tx] = (r0 + ty < N && k0 + tx < N) ? v[(size_t)(r0 + ty) * N + k0 + tx] * f[k0 + tx] : 0.0;
+        Bs[ty][tx] = (c0 + ty < N && k0 + tx < N) ? v[(size_t)(c0 + ty) * N + k0 + tx] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < GT; ++u) acc = fma(As[ty][u], Bs[tx][u], acc);
+        __syncthreads();
+    }
+    if (r0 + ty < N && c0 + tx < N) R[(size_t)b * N * N + (size_t)(r0 + ty) * N + c0 + tx] = acc;
+}
+
+}  // namespace fkm
+
+using namespace fkm;
+
+hipError_t fk_launch_ipr(int B, int N, const double* V, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ipr, dim3((N + 255) / 256, B), dim3(256), 0, st, B, N, V, out);
+    return hipGetLastError();
+}
+
+hipError_t fk_launch_stiffness(int B, int N, int stride0, int L0, double t, const double* V, const double* ev,
+                               const double* fermi, int nw, const double* wgrid, double offset, double* JV,
+                               double* mJ, double* tdiag, double* stiff, double* cond, hipStream_t st) {
+    hipLaunchKernelGGL(k_stiff_prep, dim3((N + 255) / 256, B), dim3(256), 0, st, B, N, stride0, L0, t, V, JV, tdiag);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_gemm_tn, dim3((N + GT - 1) / GT, (N + GT - 1) / GT, B), dim3(GT * GT), 0, st, B, N, V, JV, mJ);
+    if ((r = hipGetLastError()) != hipSuccess) return r;
+    hipLaunchKernelGGL(k_stiff_pairs, dim3(B), dim3(256), 0, st, B, N, mJ, ev, fermi, tdiag, nw, wgrid, offset, stiff,
+                       cond);
+    return hipGetLastError();
+}
+
+hipError_t fk_launch_dos(int S, int N, const double* spec, int nw, const double* wgrid, double eta, double* out,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_dos, dim3((nw + 255) / 256, S), dim3(256), 0, st, S, N, spec, nw, wgrid, eta, out);
+    return hipGetLastError();
+}
+
+hipError_t fk_launch_adjoint(int B, int N, const double* V, const double* fermi, const int32_t* f,
+                             const double* beta, const int32_t* nbr, const int32_t* z, int zmax, int nt_bonds,
+                             double* dens, double* bond_t, double* bond_tp, double* out, double* rho_full,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_adjoint_sites, dim3((N + 127) / 128, B), dim3(128), 0, st, B, N, V, fermi, nbr, z, zmax,
+                       nt_bonds, dens, bond_t, bond_tp);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_adjoint_reduce, dim3(B), dim3(256), 0, st, B, N, f, fermi, beta, dens, bond_t, bond_tp, out);
+    if ((r = hipGetLastError()) != hipSuccess) return r;
+    if (rho_full) {
+        hipLaunchKernelGGL(k_rho_full, dim3((N + GT - 1) / GT, (N + GT - 1) / GT, B), dim3(GT * GT), 0, st, B, N, V,
+                           fermi, rho_full);
+        r = hipGetLastError();
+    }
+    return r;
+}

@@ -1,0 +1,204 @@
+// fk_vectors.hip -- eigenvectors for calc_ed(true) (src/configuration.cpp:304-311,
+// ed_cache::cached_evecs) on top of the eigenvalue path.
+//
+//   k_tri_evecs     eigenvectors of the tridiagonal T by inverse iteration, one
+//                   thread per eigenvalue (LU of T - lambda with partial
+//                   pivoting fused with the right-hand side, O(N) per solve);
+//                   eigenvalues closer than ortol form clusters whose members
+//                   are computed in rounds and orthogonalised (modified
+//                   Gram-Schmidt) against the earlier members.
+//   k_backtransform V = Q Z with Q = H_0 H_1 ... H_{N-3} the Householder
+//                   reflectors stored by the tridiagonaliser: Z <- H_k Z for
+//                   k = N-3 .. 0, one thread per column of Z (coalesced rows).
+// Layouts: Z and V row-major N x N per matrix (column k = vector k); the
+// per-thread solve scratch is [i][j] so that a step i is one coalesced row.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fk_internal.h"
+#include "fk_solve.h"
+
+namespace fkv {
+
+using namespace fks;
+
+// Solve (T - lam) z = x in place (x overwritten by z).  T = (d, e).  The LU
+// with partial pivoting is formed on the fly; U's three diagonals go to
+// u0/u1/u2 (stride ld, this thread's column).  Zero pivots are replaced by
+// +-tiny so the solve always completes (the usual inverse-iteration rule).
+__device__ void tri_solve(const double* __restrict__ d, const double* __restrict__ e, int N, double lam, double tiny,
+                          double* x, double* u0, double* u1, double* u2, size_t ld) {
+    // forward elimination: rows k, k+1
+    double a = d[0] - lam;                 // current pivot-row diagonal
+    double b = N > 1 ? e[0] : 0.0;         // current pivot-row superdiagonal
+    double xk = x[0];
+    for (int k = 0; k + 1 < N; ++k) {
+        const double c = e[k];             // subdiagonal below the pivot
+        const double dn = d[k + 1] - lam;  // next diagonal
+        const double en = (k + 2 < N) ? e[k + 1] : 0.0;
+        double xn = x[(size_t)(k + 1) * ld];
+        if (fabs(a) >= fabs(c)) {
+            double aa = a == 0.0 ? tiny : a;
+            const double m = c / aa;
+            u0[(size_t)k * ld] = aa; u1[(size_t)k * ld] = b; u2[(size_t)k * ld] = 0.0;
+            x[(size_t)k * ld] = xk;
+            a = dn - m * b;
+            b = en;
+            xk = xn - m * xk;
+        } else {
+            const double m = a / c;
+            u0[(size_t)k * ld] = c; u1[(size_t)k * ld] = dn; u2[(size_t)k * ld] = en;
+            x[(size_t)k * ld] = xn;
+            a = b - m * dn;
+            b = -m * en;
+            xk = xk - m * xn;
+        }
+    }
+    if (a == 0.0) a = tiny;
+    u0[(size_t)(N - 1) * ld] = a;
+    x[(size_t)(N - 1) * ld] = xk;
+    // back substitution with U (diagonals u0, u1, u2)
+    double z1 = 0.0, z2 = 0.0;  // z[k+1], z[k+2]
+    for (int k = N - 1; k >= 0; --k) {
+        double s = x[(size_t)k * ld];
+        if (k + 1 < N) s -= u1[(size_t)k * ld] * z1;
+        if (k + 2 < N) s -= u2[(size_t)k * ld] * z2;
+        const double z = s / u0[(size_t)k * ld];
+        x[(size_t)k * ld] = z;
+        z2 = z1;
+        z1 = z;
+    }
+}
+
+__device__ inline double col_norm2(const double* x, int N, size_t ld) {
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) s = fma(x[(size_t)i * ld], x[(size_t)i * ld], s);
+    return s;
+}
+
+// One workgroup per matrix.  d, e, ev: B x N.  Z: B x N x N output.
+// scratch: B x 3 x N x N.
+__global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* __restrict__ dg,
+                                                    const double* __restrict__ eg, const double* __restrict__ evg,
+                                                    double* __restrict__ Zg, double* __restrict__ scr) {
+    __shared__ double d[2048], e[2048];
+    __shared__ int cstart[2048];
+    __shared__ int maxpos;
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    if (b >= B) return;
+    const double* ev = evg + (size_t)b * N;
+    double* Z = Zg + (size_t)b * N * N;
+    const size_t NN = (size_t)N * N;
+    double* u0 = scr + (size_t)b * 3 * NN;
+    double* u1 = u0 + NN;
+    double* u2 = u1 + NN;
+    double tn = 0.0;
+    for (int i = tid; i < N; i += NT) {
+        d[i] = dg[(size_t)b * N + i];
+        e[i] = eg[(size_t)b * N + i];
+    }
+    if (tid == 0) maxpos = 0;
+    __syncthreads();
+    for (int i = 0; i < N; ++i) tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < N ? fabs(e[i]) : 0.0));
+    const double eps = 2.220446049250313e-16;
+    const double ortol = 1e-3 * tn;            // cluster threshold (as LAPACK dstein)
+    const double tiny = eps * fmax(tn, 1e-300);
+    // cluster starts (sequential scan by one thread; N is small)
+    if (tid == 0) {
+        int cs = 0, mp = 0;
+        for (int j = 0; j < N; ++j) {
+            if (j > 0 && ev[j] - ev[j - 1] > ortol) cs = j;
+            cstart[j] = cs;
+            mp = max(mp, j - cs);
+        }
+        maxpos = mp;
+    }
+    __syncthreads();
+    const int rounds = maxpos + 1;
+    for (int round = 0; round < rounds; ++round) {
+        for (int j = tid; j < N; j += NT) {
+            const int pos = j - cstart[j];
+            if (pos != round) continue;
+            double* x = Z + j;  // column j, stride N
+            // deterministic start vector, different per eigenvalue
+            uint32_t h = 2654435761u * (uint32_t)(j + 1);
+            for (int i = 0; i < N; ++i) {
+                h = h * 1664525u + 1013904223u;
+                x[(size_t)i * N] = 1.0 + 0.5 * ((double)(h >> 8) / 16777216.0 - 0.5);
+            }
+            for (int it = 0; it < 3; ++it) {
+                const double nrm = sqrt(col_norm2(x, N, N));
+                const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
+                for (int i = 0; i < N; ++i) x[(size_t)i * N] *= sc;
+                tri_solve(d, e, N, ev[j], tiny, x, u0 + j, u1 + j, u2 + j, N);
+                // orthogonalise against earlier members of the cluster
+                for (int p = cstart[j]; p < j; ++p) {
+                    const double* y = Z + p;
+                    double dot = 0.0;
+                    for (int i = 0; i < N; ++i) dot = fma(x[(size_t)i * N], y[(size_t)i * N], dot);
+                    for (int i = 0; i < N; ++i) x[(size_t)i * N] = fma(-dot, y[(size_t)i * N], x[(size_t)i * N]);
+                }
+            }
+            const double nrm = sqrt(col_norm2(x, N, N));
+            const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
+            // sign convention: largest-magnitude component positive
+            double big = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const double v = x[(size_t)i * N];
+                if (fabs(v) > fabs(big)) big = v;
+            }
+            const double sg = big < 0.0 ? -sc : sc;
+            for (int i = 0; i < N; ++i) x[(size_t)i * N] *= sg;
+        }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+// Z <- Q Z.  One workgroup per matrix; thread t handles column j = t % N with
+// part q = t / N of the row range for the dot products (parts reduced in LDS).
+__global__ void __launch_bounds__(1024) k_backtransform(int B, int N, const double* __restrict__ vg,
+                                                        const double* __restrict__ taug, double* __restrict__ Zg) {
+    __shared__ double v[2048];
+    __shared__ double part[1024];
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int parts = max(1, NT / N);
+    const int j = tid % N, q = tid / N;
+    const bool active = q < parts && j < N;
+    double* Z = Zg + (size_t)b * N * N;
+    const double* V = vg + (size_t)b * N * N;
+    for (int k = N - 3; k >= 0; --k) {
+        const double tau = taug[(size_t)b * N + k];
+        if (tau == 0.0) continue;  // uniform across the block
+        for (int i = tid; i < N; i += NT) v[i] = V[(size_t)k * N + i];
+        __syncthreads();
+        double s = 0.0;
+        if (active)
+            for (int i = k + 1 + q; i < N; i += parts) s = fma(v[i], Z[(size_t)i * N + j], s);
+        if (active) part[q * N + j] = s;
+        __syncthreads();
+        if (active) {
+            double w = 0.0;
+            for (int p = 0; p < parts; ++p) w += part[p * N + j];
+            w *= tau;
+            for (int i = k + 1 + q; i < N; i += parts) Z[(size_t)i * N + j] = fma(-v[i], w, Z[(size_t)i * N + j]);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace fkv
+
+hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,
+                           const double* tau, double* Z, double* scratch, hipStream_t st) {
+    if (N > 2048) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fkv::k_tri_evecs, dim3(B), dim3(N <= 256 ? 256 : 1024), 0, st, B, N, d, e, ev, Z, scratch);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    const int nt = N <= 1024 ? ((1024 / N) * N > 0 ? (1024 / N) * N : N) : 1024;
+    hipLaunchKernelGGL(fkv::k_backtransform, dim3(B), dim3(nt <= 1024 ? nt : 1024), 0, st, B, N, vh, tau, Z);
+    return hipGetLastError();
+}

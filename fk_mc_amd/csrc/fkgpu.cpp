@@ -33,6 +33,7 @@ struct fkgpu_ctx {
     // grow-on-demand scratch for the host-pointer entry points
     struct buf { void* p = nullptr; size_t n = 0; };
     buf b_f, b_beta, b_U, b_mu, b_ev, b_lz, b_ce, b_cf, b_ws;
+    buf b_vh, b_tau, b_d, b_e, b_Z, b_scr, b_m1, b_m2, b_m3, b_m4, b_m5, b_out, b_w;
 };
 
 struct fkgpu_chains {
@@ -186,7 +187,8 @@ static void ctx_release(fkgpu_ctx* ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto* b : {&ctx->b_f, &ctx->b_beta, &ctx->b_U, &ctx->b_mu, &ctx->b_ev, &ctx->b_lz, &ctx->b_ce, &ctx->b_cf,
-                    &ctx->b_ws})
+                    &ctx->b_ws, &ctx->b_vh, &ctx->b_tau, &ctx->b_d, &ctx->b_e, &ctx->b_Z, &ctx->b_scr, &ctx->b_m1,
+                    &ctx->b_m2, &ctx->b_m3, &ctx->b_m4, &ctx->b_m5, &ctx->b_out, &ctx->b_w})
         if (b->p) hipFree(b->p);
     if (ctx->d_hop) hipFree(ctx->d_hop);
     if (ctx->d_nbr) hipFree(ctx->d_nbr);
@@ -274,6 +276,62 @@ static int check_params(fkgpu_ctx* ctx, int batch, const int32_t* f, const doubl
     return FKGPU_OK;
 }
 
+// Device-resident eigenpairs of `batch` configurations given on the host.
+struct eig_dev {
+    int32_t* f;
+    double *beta, *U, *mu, *ev, *lz, *ce, *cf, *V;
+};
+static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                         const double* mu_c, bool vectors, eig_dev& o) {
+    const int N = ctx->N;
+    const size_t BN = (size_t)batch * N, BNN = BN * N;
+    int r;
+    if ((r = ensure(ctx, ctx->b_f, BN, &o.f)) || (r = ensure(ctx, ctx->b_beta, batch, &o.beta)) ||
+        (r = ensure(ctx, ctx->b_U, batch, &o.U)) || (r = ensure(ctx, ctx->b_mu, batch, &o.mu)) ||
+        (r = ensure(ctx, ctx->b_ev, BN, &o.ev)) || (r = ensure(ctx, ctx->b_lz, batch, &o.lz)) ||
+        (r = ensure(ctx, ctx->b_ce, BN, &o.ce)) || (r = ensure(ctx, ctx->b_cf, BN, &o.cf)))
+        return r;
+    o.V = nullptr;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemcpyAsync(o.f, f, sizeof(int32_t) * BN, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(o.beta, beta, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(o.U, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(o.mu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    const int kind = fk_solver_kind(N);
+    const int grid = grid_for(ctx, kind, batch);
+    eig_args a{};
+    a.B = batch;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = o.f;
+    a.beta = o.beta;
+    a.U = o.U;
+    a.mu_c = o.mu;
+    a.evals = o.ev;
+    a.logz = o.lz;
+    a.cexp = o.ce;
+    a.cfermi = o.cf;
+    if (fk_solver_needs_ws(kind) && (r = ensure(ctx, ctx->b_ws, (size_t)grid * N * N, &a.ws))) return r;
+    double *vh = nullptr, *tau = nullptr, *dd = nullptr, *ee = nullptr, *Z = nullptr, *scr = nullptr;
+    if (vectors) {
+        if (N > 1024) return fail(ctx, FKGPU_EUNSUPPORTED, "eigenvectors are implemented for N <= 1024");
+        if ((r = ensure(ctx, ctx->b_vh, BNN, &vh)) || (r = ensure(ctx, ctx->b_tau, BN, &tau)) ||
+            (r = ensure(ctx, ctx->b_d, BN, &dd)) || (r = ensure(ctx, ctx->b_e, BN, &ee)) ||
+            (r = ensure(ctx, ctx->b_Z, BNN, &Z)) || (r = ensure(ctx, ctx->b_scr, 3 * BNN, &scr)))
+            return r;
+        a.vout = vh;
+        a.tauout = tau;
+        a.d_out = dd;
+        a.e_out = ee;
+    }
+    FK_HIP(ctx, fk_launch_eig(kind, a, grid, st));
+    if (vectors) {
+        FK_HIP(ctx, fk_launch_evecs(batch, N, dd, ee, o.ev, vh, tau, Z, scr, st));
+        o.V = Z;
+    }
+    return FKGPU_OK;
+}
+
 int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta, const double* d_U,
                           const double* d_mu_c, double* d_evals, double* d_logZ) {
     if (!ctx) return FKGPU_EINVAL;
@@ -308,9 +366,21 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     int r = check_params(ctx, batch, f, beta, U, mu_c);
     if (r) return r;
     if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
-    if (want_vectors) return fail(ctx, FKGPU_EUNSUPPORTED, "eigenvectors are not available in this build");
-    (void)evecs;
     hipSetDevice(ctx->device);
+    if (want_vectors) {
+        if (!evecs) return fail(ctx, FKGPU_EINVAL, "null evecs with want_vectors");
+        eig_dev o;
+        if ((r = eig_pairs_dev(ctx, batch, f, beta, U, mu_c, true, o))) return r;
+        const size_t BN = (size_t)batch * ctx->N;
+        hipStream_t st = ctx->stream;
+        FK_HIP(ctx, hipMemcpyAsync(evals, o.ev, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+        FK_HIP(ctx, hipMemcpyAsync(logZ, o.lz, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
+        FK_HIP(ctx, hipMemcpyAsync(evecs, o.V, sizeof(double) * BN * ctx->N, hipMemcpyDeviceToHost, st));
+        if (cached_exp) FK_HIP(ctx, hipMemcpyAsync(cached_exp, o.ce, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+        if (cached_fermi) FK_HIP(ctx, hipMemcpyAsync(cached_fermi, o.cf, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+        FK_HIP(ctx, hipStreamSynchronize(st));
+        return FKGPU_OK;
+    }
     const int N = ctx->N;
     const size_t BN = (size_t)batch * N;
     int32_t* df;
@@ -349,6 +419,111 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     FK_HIP(ctx, hipMemcpyAsync(logZ, dlz, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
     if (cached_exp) FK_HIP(ctx, hipMemcpyAsync(cached_exp, dce, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
     if (cached_fermi) FK_HIP(ctx, hipMemcpyAsync(cached_fermi, dcf, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Measures and adjoint
+// ---------------------------------------------------------------------------
+int fkgpu_measure_ipr(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                      const double* mu_c, double* evals, double* ipr) {
+    int r = check_params(ctx, batch, f, beta, U, mu_c);
+    if (r) return r;
+    if (!ipr) return fail(ctx, FKGPU_EINVAL, "null output");
+    hipSetDevice(ctx->device);
+    eig_dev o;
+    if ((r = eig_pairs_dev(ctx, batch, f, beta, U, mu_c, true, o))) return r;
+    const int N = ctx->N;
+    const size_t BN = (size_t)batch * N;
+    double* dout;
+    if ((r = ensure(ctx, ctx->b_m1, BN, &dout))) return r;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, fk_launch_ipr(batch, N, o.V, dout, st));
+    FK_HIP(ctx, hipMemcpyAsync(ipr, dout, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+    if (evals) FK_HIP(ctx, hipMemcpyAsync(evals, o.ev, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
+int fkgpu_measure_stiffness(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                            const double* mu_c, int nw, const double* wgrid, double offset, double* stiffness,
+                            double* cond) {
+    int r = check_params(ctx, batch, f, beta, U, mu_c);
+    if (r) return r;
+    if (!stiffness || nw < 0 || (nw > 0 && (!wgrid || !cond)) || !std::isfinite(offset))
+        return fail(ctx, FKGPU_EINVAL, "invalid stiffness arguments");
+    if (ctx->lat_kind != FKGPU_LATTICE_HYPERCUBIC || ctx->lat_ndim < 2)
+        return fail(ctx, FKGPU_EUNSUPPORTED, "stiffness needs a hypercubic lattice of dimension >= 2");
+    hipSetDevice(ctx->device);
+    eig_dev o;
+    if ((r = eig_pairs_dev(ctx, batch, f, beta, U, mu_c, true, o))) return r;
+    const int N = ctx->N;
+    const size_t BN = (size_t)batch * N, BNN = BN * N;
+    double *JV, *mJ, *td, *st_d, *cd, *wg;
+    if ((r = ensure(ctx, ctx->b_m1, BNN, &JV)) || (r = ensure(ctx, ctx->b_m2, BNN, &mJ)) ||
+        (r = ensure(ctx, ctx->b_m3, BN, &td)) || (r = ensure(ctx, ctx->b_m4, batch, &st_d)) ||
+        (r = ensure(ctx, ctx->b_m5, (size_t)batch * (nw > 0 ? nw : 1), &cd)) ||
+        (r = ensure(ctx, ctx->b_w, (size_t)(nw > 0 ? nw : 1), &wg)))
+        return r;
+    hipStream_t st = ctx->stream;
+    if (nw > 0) FK_HIP(ctx, hipMemcpyAsync(wg, wgrid, sizeof(double) * nw, hipMemcpyHostToDevice, st));
+    int stride0 = 1;
+    for (int d = 1; d < ctx->lat_ndim; ++d) stride0 *= ctx->lat_L;
+    // T_x uses hopping 1 whatever t is (stiffness.hpp:59, t_ = 1)
+    FK_HIP(ctx, fk_launch_stiffness(batch, N, stride0, ctx->lat_L, 1.0, o.V, o.ev, o.cf, nw, wg, offset, JV, mJ, td,
+                                    st_d, cd, st));
+    FK_HIP(ctx, hipMemcpyAsync(stiffness, st_d, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
+    if (nw > 0) FK_HIP(ctx, hipMemcpyAsync(cond, cd, sizeof(double) * batch * nw, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
+int fkgpu_dos(fkgpu_ctx* ctx, int nspectra, const double* spectra, int nw, const double* wgrid, double eta,
+              double* dos) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
+    if (nspectra <= 0 || nw <= 0 || !spectra || !wgrid || !dos || !(eta > 0.0))
+        return fail(ctx, FKGPU_EINVAL, "invalid DOS arguments");
+    hipSetDevice(ctx->device);
+    const int N = ctx->N;
+    double *sp, *wg, *out;
+    int r;
+    if ((r = ensure(ctx, ctx->b_m1, (size_t)nspectra * N, &sp)) || (r = ensure(ctx, ctx->b_w, nw, &wg)) ||
+        (r = ensure(ctx, ctx->b_m2, (size_t)nspectra * nw, &out)))
+        return r;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemcpyAsync(sp, spectra, sizeof(double) * nspectra * N, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(wg, wgrid, sizeof(double) * nw, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, fk_launch_dos(nspectra, N, sp, nw, wg, eta, out, st));
+    FK_HIP(ctx, hipMemcpyAsync(dos, out, sizeof(double) * nspectra * nw, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
+int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                  const double* mu_c, double* logZ, double* dens, double* grads, double* rho) {
+    int r = check_params(ctx, batch, f, beta, U, mu_c);
+    if (r) return r;
+    if (!logZ || !dens || !grads) return fail(ctx, FKGPU_EINVAL, "null output");
+    hipSetDevice(ctx->device);
+    eig_dev o;
+    if ((r = eig_pairs_dev(ctx, batch, f, beta, U, mu_c, true, o))) return r;
+    const int N = ctx->N;
+    const size_t BN = (size_t)batch * N;
+    double *dd, *bt, *btp, *out, *rf = nullptr;
+    if ((r = ensure(ctx, ctx->b_m1, BN, &dd)) || (r = ensure(ctx, ctx->b_m3, BN, &bt)) ||
+        (r = ensure(ctx, ctx->b_m4, BN, &btp)) || (r = ensure(ctx, ctx->b_out, (size_t)batch * 8, &out)))
+        return r;
+    if (rho && (r = ensure(ctx, ctx->b_m2, BN * N, &rf))) return r;
+    const int nt_bonds = ctx->lat_kind == 2 ? ctx->zmax : 2 * ctx->lat_ndim;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, fk_launch_adjoint(batch, N, o.V, o.cf, o.f, o.beta, ctx->d_nbr, ctx->d_z, ctx->zmax, nt_bonds, dd, bt,
+                                  btp, out, rf, st));
+    FK_HIP(ctx, hipMemcpyAsync(logZ, o.lz, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipMemcpyAsync(dens, dd, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipMemcpyAsync(grads, out, sizeof(double) * batch * 8, hipMemcpyDeviceToHost, st));
+    if (rho) FK_HIP(ctx, hipMemcpyAsync(rho, rf, sizeof(double) * BN * N, hipMemcpyDeviceToHost, st));
     FK_HIP(ctx, hipStreamSynchronize(st));
     return FKGPU_OK;
 }

@@ -82,6 +82,10 @@ def lib():
         L.fkgpu_chains_last_ms.argtypes = [P, _d]
         L.fkgpu_chains_state.argtypes = [P, _i32, _i32, _d, _d, _i64, _i64]
         L.fkgpu_rng_draws.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, _d, C.c_int, _d]
+        L.fkgpu_measure_ipr.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d]
+        L.fkgpu_measure_stiffness.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, _d, C.c_double, _d, _d]
+        L.fkgpu_dos.argtypes = [P, C.c_int, _d, C.c_int, _d, C.c_double, _d]
+        L.fkgpu_adjoint.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _d, _d]
         _LIB = L
     return _LIB
 
@@ -169,6 +173,62 @@ class Context:
             out["cexp"] = ce
             out["cfermi"] = cf
         return out
+
+
+def _batch(f, *params):
+    f = np.ascontiguousarray(np.atleast_2d(f), dtype=np.int32)
+    B = f.shape[0]
+    return (f, B) + tuple(np.ascontiguousarray(np.broadcast_to(np.asarray(p, dtype=np.float64), (B,)))
+                          for p in params)
+
+
+def _measure_ipr(self, f, beta, U, mu_c):
+    """measure_ipr for a batch: returns (evals[B,N], ipr[B,N])."""
+    f, B, b, u, m = _batch(f, beta, U, mu_c)
+    ev = np.zeros((B, self.N))
+    ipr = np.zeros((B, self.N))
+    self._check(lib().fkgpu_measure_ipr(self._h, B, _p(f, _i32), _p(b, _d), _p(u, _d), _p(m, _d), _p(ev, _d),
+                                        _p(ipr, _d)))
+    return ev, ipr
+
+
+def _measure_stiffness(self, f, beta, U, mu_c, wgrid=(0.0,), offset=0.05):
+    f, B, b, u, m = _batch(f, beta, U, mu_c)
+    wg = np.ascontiguousarray(wgrid, dtype=np.float64)
+    st = np.zeros(B)
+    cond = np.zeros((B, max(len(wg), 1)))
+    self._check(lib().fkgpu_measure_stiffness(self._h, B, _p(f, _i32), _p(b, _d), _p(u, _d), _p(m, _d), len(wg),
+                                              _p(wg, _d), offset, _p(st, _d), _p(cond, _d)))
+    return st, cond[:, :len(wg)]
+
+
+def _dos(self, spectra, wgrid, eta=0.05):
+    sp = np.ascontiguousarray(np.atleast_2d(spectra), dtype=np.float64)
+    wg = np.ascontiguousarray(wgrid, dtype=np.float64)
+    out = np.zeros((sp.shape[0], len(wg)))
+    self._check(lib().fkgpu_dos(self._h, sp.shape[0], _p(sp, _d), len(wg), _p(wg, _d), eta, _p(out, _d)))
+    return out
+
+
+def _adjoint(self, f, beta, U, mu_c, full=False):
+    f, B, b, u, m = _batch(f, beta, U, mu_c)
+    lz = np.zeros(B)
+    dens = np.zeros((B, self.N))
+    gr = np.zeros((B, 8))
+    rho = np.zeros((B, self.N, self.N)) if full else None
+    self._check(lib().fkgpu_adjoint(self._h, B, _p(f, _i32), _p(b, _d), _p(u, _d), _p(m, _d), _p(lz, _d),
+                                    _p(dens, _d), _p(gr, _d), _p(rho, _d)))
+    out = dict(logZ=lz, dens=dens, dOmega_dU=gr[:, 0], dOmega_dt=gr[:, 1], dOmega_dtp=gr[:, 2],
+               dOmega_dmu=gr[:, 3], Nc=gr[:, 4], dNc_dmu=gr[:, 5])
+    if full:
+        out["rho"] = rho
+    return out
+
+
+Context.measure_ipr = _measure_ipr
+Context.measure_stiffness = _measure_stiffness
+Context.dos = _dos
+Context.adjoint = _adjoint
 
 
 class Chains:

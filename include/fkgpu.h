@@ -72,6 +72,33 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
 int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta, const double* d_U,
                           const double* d_mu_c, double* d_evals, double* d_logZ);
 
+/* ---- Eigenvector measures and the adjoint (eigenvectors stay on the device) ----
+ * Each call runs calc_hamiltonian + calc_ed(true) for `batch` configurations and
+ * then the measure, without copying the N x N eigenvector matrices to the host. */
+
+/* measure_ipr::accumulate (include/fk_mc/measures/ipr.hpp:39-56):
+ * ipr[b*N + k] = ||v_k||_4 / ||v_k||_2^2 (the reference's lpNorm<4> form). */
+int fkgpu_measure_ipr(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                      const double* mu_c, double* evals, double* ipr);
+/* measure_stiffness::accumulate (include/fk_mc/measures/stiffness.hpp:129-187), hypercubic
+ * lattices, direction 0, T_x hopping 1 as in the reference: stiffness[b] = (T + V)/N and
+ * cond[b*nw + w] = the Lorentzian conductivity sum at wgrid[w] with width `offset`. */
+int fkgpu_measure_stiffness(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                            const double* mu_c, int nw, const double* wgrid, double offset, double* stiffness,
+                            double* cond);
+/* DOS of spectra (prog/data_save.hxx:264-345, dos0_f): dos[s*nw + w] =
+ * (1/N) sum_k (eta/pi) / ((wgrid[w] - spectra[s*N + k])^2 + eta^2); N = fkgpu_msize. */
+int fkgpu_dos(fkgpu_ctx* ctx, int nspectra, const double* spectra, int nw, const double* wgrid, double eta,
+              double* dos);
+/* Adjoint of Omega = -logZ/beta (no reference counterpart; the closest reference data are the
+ * eigenvector densities sum_k f_k V_ik^2 from ed_cache::cached_evecs / cached_fermi,
+ * include/fk_mc/configuration.hpp:28-31).  dOmega/dH = rho = V f(Lambda) V^T.
+ * Outputs: logZ[b]; dens[b*N + i] = rho_ii = dOmega/d eps_i (local conduction density);
+ * grads[b*8 + {0..5}] = {dOmega/dU, dOmega/dt, dOmega/dtp, dOmega/dmu_c, N_c, dN_c/dmu_c};
+ * rho[b*N*N] (nullable) = the full matrix V f(Lambda) V^T. */
+int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                  const double* mu_c, double* logZ, double* dens, double* grads, double* rho);
+
 /* ---- Batched persistent Metropolis chains ------------------------------------
  * Replaces mc_metropolis::update (src/mc_metropolis.cpp:34-52) with the moves
  * of src/moves.cpp for many independent chains at once; each chain owns its

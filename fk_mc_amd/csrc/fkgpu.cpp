@@ -6,6 +6,7 @@
 // for timing and HIP graph capture of move steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -25,15 +26,17 @@ struct fkgpu_ctx {
     int N = 0;
     int lat_kind = -1, lat_ndim = 0, lat_L = 0;
     double* d_hop = nullptr;
+    double* d_hopv = nullptr;  // hopping value of each neighbour slot (N x zmax)
     int32_t* d_nbr = nullptr;
     int32_t* d_z = nullptr;
     int zmax = 0;
     std::vector<double> hop;
     std::vector<int32_t> nbr, z;
+    std::vector<double> hopv;
     // grow-on-demand scratch for the host-pointer entry points
     struct buf { void* p = nullptr; size_t n = 0; };
     buf b_f, b_beta, b_U, b_mu, b_ev, b_lz, b_ce, b_cf, b_ws;
-    buf b_vh, b_tau, b_d, b_e, b_Z, b_scr, b_m1, b_m2, b_m3, b_m4, b_m5, b_out, b_w;
+    buf b_vh, b_tau, b_d, b_e, b_Z, b_scr, b_m1, b_m2, b_m3, b_m4, b_m5, b_out, b_w, b_c1, b_c2, b_c3, b_c4;
 };
 
 struct fkgpu_chains {
@@ -140,11 +143,24 @@ void build_hypercubic(fkgpu_ctx* ctx, int kind, int D, int L, double t, double t
 
 int upload_lattice(fkgpu_ctx* ctx) {
     if (ctx->d_hop) hipFree(ctx->d_hop);
+    if (ctx->d_hopv) hipFree(ctx->d_hopv);
     if (ctx->d_nbr) hipFree(ctx->d_nbr);
     if (ctx->d_z) hipFree(ctx->d_z);
-    ctx->d_hop = nullptr;
+    ctx->d_hop = ctx->d_hopv = nullptr;
     ctx->d_nbr = ctx->d_z = nullptr;
     size_t n = ctx->N;
+    ctx->hopv.assign(ctx->nbr.size(), 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (int q = 0; q < ctx->z[i]; ++q) {
+            const int j = ctx->nbr[i * ctx->zmax + q];
+            // the dense template sums all slots pointing at j; split the value evenly so that
+            // the slots of one row add up to the template entry
+            int mult = 0;
+            for (int q2 = 0; q2 < ctx->z[i]; ++q2) mult += ctx->nbr[i * ctx->zmax + q2] == j;
+            ctx->hopv[i * ctx->zmax + q] = ctx->hop[i * n + j] / mult;
+        }
+    FK_HIP(ctx, hipMalloc(&ctx->d_hopv, sizeof(double) * ctx->hopv.size()));
+    FK_HIP(ctx, hipMemcpy(ctx->d_hopv, ctx->hopv.data(), sizeof(double) * ctx->hopv.size(), hipMemcpyHostToDevice));
     FK_HIP(ctx, hipMalloc(&ctx->d_hop, sizeof(double) * n * n));
     FK_HIP(ctx, hipMalloc(&ctx->d_nbr, sizeof(int32_t) * ctx->nbr.size()));
     FK_HIP(ctx, hipMalloc(&ctx->d_z, sizeof(int32_t) * n));
@@ -188,9 +204,11 @@ static void ctx_release(fkgpu_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto* b : {&ctx->b_f, &ctx->b_beta, &ctx->b_U, &ctx->b_mu, &ctx->b_ev, &ctx->b_lz, &ctx->b_ce, &ctx->b_cf,
                     &ctx->b_ws, &ctx->b_vh, &ctx->b_tau, &ctx->b_d, &ctx->b_e, &ctx->b_Z, &ctx->b_scr, &ctx->b_m1,
-                    &ctx->b_m2, &ctx->b_m3, &ctx->b_m4, &ctx->b_m5, &ctx->b_out, &ctx->b_w})
+                    &ctx->b_m2, &ctx->b_m3, &ctx->b_m4, &ctx->b_m5, &ctx->b_out, &ctx->b_w, &ctx->b_c1, &ctx->b_c2,
+                    &ctx->b_c3, &ctx->b_c4})
         if (b->p) hipFree(b->p);
     if (ctx->d_hop) hipFree(ctx->d_hop);
+    if (ctx->d_hopv) hipFree(ctx->d_hopv);
     if (ctx->d_nbr) hipFree(ctx->d_nbr);
     if (ctx->d_z) hipFree(ctx->d_z);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -525,6 +543,159 @@ int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* bet
     FK_HIP(ctx, hipMemcpyAsync(grads, out, sizeof(double) * batch * 8, hipMemcpyDeviceToHost, st));
     if (rho) FK_HIP(ctx, hipMemcpyAsync(rho, rf, sizeof(double) * BN * N, hipMemcpyDeviceToHost, st));
     FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev paths
+// ---------------------------------------------------------------------------
+namespace {
+// chebyshev_eval (include/fk_mc/chebyshev.hpp:15-63): angle grid linspace(0,1,G),
+// x_i = -cos(pi theta_i), T_k(x_i) = cos(k acos x_i), trapezoid in theta.
+struct cheb_quad {
+    int M, G;
+    std::vector<double> th, x;
+    cheb_quad(int max_moment, int grid) : G(grid) {
+        M = max_moment + max_moment % 2;
+        th.resize(G);
+        x.resize(G);
+        for (int i = 0; i < G; ++i) {
+            th[i] = G > 1 ? (double)i / (double)(G - 1) : 0.0;
+            x[i] = -std::cos(M_PI * th[i]);
+        }
+    }
+    double moment(const std::vector<double>& v, int k) const {
+        double s = 0.0;
+        for (int i = 0; i + 1 < G; ++i)
+            s += (v[i + 1] * std::cos(k * std::acos(x[i + 1])) + v[i] * std::cos(k * std::acos(x[i]))) * (th[i + 1] - th[i]);
+        return s * 0.5;
+    }
+};
+}  // namespace
+
+int fkgpu_cheb_logz(fkgpu_ctx* ctx, const int32_t* f, double beta, double U, double mu_c, int cheb_size,
+                    int grid_size, double* moments, double* logZ, double* ab) {
+    int r = check_params(ctx, 1, f, &beta, &U, &mu_c);
+    if (r) return r;
+    if (cheb_size < 2 || grid_size < 2 || !moments || !logZ || !ab) return fail(ctx, FKGPU_EINVAL, "invalid Chebyshev arguments");
+    if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "exact-trace Chebyshev needs N <= 2048");
+    const int N = ctx->N;
+    cheb_quad cq(cheb_size, grid_size);
+    const int M = cq.M;
+    // extreme eigenvalues from the dense solver
+    std::vector<double> ev(N);
+    double lz;
+    if ((r = fkgpu_eig_logz(ctx, 1, f, &beta, &U, &mu_c, 0, ev.data(), nullptr, &lz, nullptr, nullptr))) return r;
+    const double e_min = ev[0], e_max = ev[N - 1];
+    const double a = (e_max - e_min) / 2.0, b = (e_max + e_min) / 2.0;
+    if (!(a > 0.0)) return fail(ctx, FKGPU_EINVAL, "degenerate spectrum: e_max == e_min");
+    const size_t NN = (size_t)N * N;
+    double *T0, *T1, *parts, *traces, *diag;
+    if ((r = ensure(ctx, ctx->b_c1, NN, &T0)) || (r = ensure(ctx, ctx->b_c2, NN, &T1)) ||
+        (r = ensure(ctx, ctx->b_c3, 512, &parts)) || (r = ensure(ctx, ctx->b_c4, (size_t)3 * M + N, &traces)))
+        return r;
+    diag = traces + 3 * M;
+    std::vector<double> dg(N);
+    double trX = 0.0;
+    for (int i = 0; i < N; ++i) {
+        dg[i] = -mu_c + U * f[i];
+        trX += (dg[i] - b) / a;
+    }
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemsetAsync(traces, 0, sizeof(double) * 3 * M, st));
+    FK_HIP(ctx, hipMemcpyAsync(diag, dg.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, fk_cheb_exact_moments(N, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, diag, a, b, M, T0, T1, parts,
+                                      nullptr, traces, st));
+    std::vector<double> tr(3 * M);
+    FK_HIP(ctx, hipMemcpyAsync(tr.data(), traces, sizeof(double) * 3 * M, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    // moment bookkeeping of src/configuration.cpp:217-288
+    std::vector<double> mom(M, 0.0);
+    std::vector<char> is_set(M, 0);
+    mom[0] = 1.0; is_set[0] = 1;
+    mom[1] = tr[1] / N; is_set[1] = 1;
+    for (int m = 2; m <= M / 2; ++m) {
+        if (!is_set[m]) { mom[m] = tr[m] / N; is_set[m] = 1; }
+        int k = 2 * m - 1;
+        if (k < M && k >= M / 2) {
+            mom[k] = (tr[M + m] * 2.0 - trX) / N;
+            is_set[k] = 1;
+            if (k != M - 1) {
+                ++k;
+                mom[k] = tr[2 * M + m] / N * 2.0 - 1.0;
+                is_set[k] = 1;
+            }
+        }
+    }
+    std::vector<double> fv(cq.G);
+    for (int i = 0; i < cq.G; ++i) fv[i] = N * std::log(1.0 + std::exp(-beta * (a * cq.x[i] + b)));
+    double s = cq.moment(fv, 0);
+    for (int m = 1; m < M; ++m) s += 2.0 * cq.moment(fv, m) * mom[m];
+    for (int m = 0; m < M; ++m) moments[m] = mom[m];
+    *logZ = s;
+    ab[0] = a; ab[1] = b; ab[2] = e_min; ab[3] = e_max;
+    return FKGPU_OK;
+}
+
+int fkgpu_kpm_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                   const double* mu_c, int R, int M, uint64_t seed, int jackson, double* logZ, double* moments,
+                   double* ab) {
+    int r = check_params(ctx, batch, f, beta, U, mu_c);
+    if (r) return r;
+    const int N = ctx->N;
+    if (!logZ || R < 1 || M < 2 || (M & 1) || M > 512) return fail(ctx, FKGPU_EINVAL, "invalid KPM arguments");
+    if (N > 4096 || ctx->zmax > 8) return fail(ctx, FKGPU_EUNSUPPORTED, "KPM path needs N <= 4096 and <= 8 neighbours");
+    hipSetDevice(ctx->device);
+    // Gershgorin bounds and Chebyshev-Gauss coefficients of log(1+e^{-beta(a x + b)})
+    std::vector<double> abv(2 * (size_t)batch), coef((size_t)batch * M), abo(2 * (size_t)batch);
+    const int K = std::max(2 * M, 64);
+    for (int c = 0; c < batch; ++c) {
+        double lo = 1e300, hi = -1e300;
+        for (int i = 0; i < N; ++i) {
+            double rad = 0.0;
+            for (int q = 0; q < ctx->z[i]; ++q) rad += std::fabs(ctx->hopv[(size_t)i * ctx->zmax + q]);
+            const double dgi = -mu_c[c] + U[c] * f[(size_t)c * N + i];
+            lo = std::min(lo, dgi - rad);
+            hi = std::max(hi, dgi + rad);
+        }
+        double a = 0.5 * (hi - lo) * 1.01, b = 0.5 * (hi + lo);
+        if (!(a > 0.0)) a = 1.0;
+        abv[2 * c] = 1.0 / a;
+        abv[2 * c + 1] = b;
+        abo[2 * c] = a;
+        abo[2 * c + 1] = b;
+        for (int m = 0; m < M; ++m) {
+            double sacc = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const double th = M_PI * (k + 0.5) / K;
+                const double x = std::cos(th);
+                const double y = beta[c] * (a * x + b);
+                const double F = y > 0 ? std::log1p(std::exp(-y)) : -y + std::log1p(std::exp(y));
+                sacc += F * std::cos(m * th);
+            }
+            coef[(size_t)c * M + m] = (m == 0 ? 1.0 : 2.0) * sacc / K;
+        }
+    }
+    int32_t* df;
+    double *dU, *dmu, *dab, *dcoef, *dmom, *dmuo, *dlz;
+    if ((r = ensure(ctx, ctx->b_f, (size_t)batch * N, &df)) || (r = ensure(ctx, ctx->b_U, batch, &dU)) ||
+        (r = ensure(ctx, ctx->b_mu, batch, &dmu)) || (r = ensure(ctx, ctx->b_c1, 2 * (size_t)batch, &dab)) ||
+        (r = ensure(ctx, ctx->b_c2, (size_t)batch * M, &dcoef)) ||
+        (r = ensure(ctx, ctx->b_c3, (size_t)batch * R * M, &dmom)) ||
+        (r = ensure(ctx, ctx->b_c4, (size_t)batch * M, &dmuo)) || (r = ensure(ctx, ctx->b_lz, batch, &dlz)))
+        return r;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemcpyAsync(df, f, sizeof(int32_t) * batch * N, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dU, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dmu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dab, abv.data(), sizeof(double) * 2 * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * batch * M, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, fk_launch_kpm(batch, N, R, M, df, dU, dmu, dab, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, seed,
+                              dcoef, jackson, dmom, dmuo, dlz, st));
+    FK_HIP(ctx, hipMemcpyAsync(logZ, dlz, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
+    if (moments) FK_HIP(ctx, hipMemcpyAsync(moments, dmuo, sizeof(double) * batch * M, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    if (ab) for (int i = 0; i < 2 * batch; ++i) ab[i] = abo[i];
     return FKGPU_OK;
 }
 

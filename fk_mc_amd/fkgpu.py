@@ -86,6 +86,8 @@ def lib():
         L.fkgpu_measure_stiffness.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, _d, C.c_double, _d, _d]
         L.fkgpu_dos.argtypes = [P, C.c_int, _d, C.c_int, _d, C.c_double, _d]
         L.fkgpu_adjoint.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _d, _d]
+        L.fkgpu_cheb_logz.argtypes = [P, _i32, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _d, _d, _d]
+        L.fkgpu_kpm_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, C.c_int, C.c_uint64, C.c_int, _d, _d, _d]
         _LIB = L
     return _LIB
 
@@ -225,6 +227,30 @@ def _adjoint(self, f, beta, U, mu_c, full=False):
     return out
 
 
+def _cheb_logz(self, f, beta, U, mu_c, cheb_size, grid_size):
+    """calc_chebyshev (exact trace) of one configuration."""
+    f = np.ascontiguousarray(np.ravel(f), dtype=np.int32)
+    M = cheb_size + cheb_size % 2
+    mom = np.zeros(M)
+    lz = np.zeros(1)
+    ab = np.zeros(4)
+    self._check(lib().fkgpu_cheb_logz(self._h, _p(f, _i32), beta, U, mu_c, cheb_size, grid_size, _p(mom, _d),
+                                      _p(lz, _d), _p(ab, _d)))
+    return dict(moments=mom, logZ=float(lz[0]), a=ab[0], b=ab[1], e_min=ab[2], e_max=ab[3])
+
+
+def _kpm_logz(self, f, beta, U, mu_c, R=64, M=256, seed=1, jackson=True):
+    f, B, b, u, m = _batch(f, beta, U, mu_c)
+    lz = np.zeros(B)
+    mom = np.zeros((B, M))
+    ab = np.zeros((B, 2))
+    self._check(lib().fkgpu_kpm_logz(self._h, B, _p(f, _i32), _p(b, _d), _p(u, _d), _p(m, _d), R, M, seed,
+                                     int(jackson), _p(lz, _d), _p(mom, _d), _p(ab, _d)))
+    return dict(logZ=lz, moments=mom, a=ab[:, 0], b=ab[:, 1])
+
+
+Context.cheb_logz = _cheb_logz
+Context.kpm_logz = _kpm_logz
 Context.measure_ipr = _measure_ipr
 Context.measure_stiffness = _measure_stiffness
 Context.dos = _dos

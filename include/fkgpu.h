@@ -99,6 +99,28 @@ int fkgpu_dos(fkgpu_ctx* ctx, int nspectra, const double* spectra, int nw, const
 int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
                   const double* mu_c, double* logZ, double* dens, double* grads, double* rho);
 
+/* ---- Chebyshev paths ----------------------------------------------------------
+ * configuration_t::calc_chebyshev (src/configuration.cpp:94-206) with
+ * chebyshev_eval(cheb_size, grid_size) (include/fk_mc/chebyshev.hpp:15-63): exact
+ * trace moments mu_m = Tr T_m(X)/N of X = (H - b)/a, a = (e_max - e_min)/2,
+ * b = (e_max + e_min)/2 from the extreme eigenvalues (ARPACK in the reference, the
+ * dense solver here, N <= 2048), the reference's doubling identities for the upper
+ * half of the moments, and logZ = g_0 + 2 sum_m g_m mu_m with g_m the trapezoid
+ * quadrature of N log(1 + e^{-beta(a w + b)}) on the Lobatto grid.  No kernel damping.
+ * cheb_size is rounded up to even as in the reference.  Outputs: moments[cheb_size],
+ * logZ, ab[4] = {a, b, e_min, e_max}. */
+int fkgpu_cheb_logz(fkgpu_ctx* ctx, const int32_t* f, double beta, double U, double mu_c, int cheb_size,
+                    int grid_size, double* moments, double* logZ, double* ab);
+/* Stochastic-trace Chebyshev (kernel polynomial method; no reference counterpart, the
+ * reference's path is the exact trace above): R Rademacher vectors from `seed` (the same
+ * vectors for every configuration), M moments (even, <= 512), FP32 recursion with FP64
+ * moment accumulation, Jackson damping when jackson != 0, spectral bounds from the
+ * Gershgorin discs widened by 1%; N <= 4096, at most 8 neighbours per site.
+ * Outputs logZ[b], moments[b*M + m] (nullable), ab[b*2 + {0,1}] = {a, b} (nullable). */
+int fkgpu_kpm_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                   const double* mu_c, int R, int M, uint64_t seed, int jackson, double* logZ, double* moments,
+                   double* ab);
+
 /* ---- Batched persistent Metropolis chains ------------------------------------
  * Replaces mc_metropolis::update (src/mc_metropolis.cpp:34-52) with the moves
  * of src/moves.cpp for many independent chains at once; each chain owns its

@@ -1,0 +1,69 @@
+"""Chebyshev paths on the GPU.
+
+Exact trace (configuration_t::calc_chebyshev, src/configuration.cpp:94-206)
+against the oracle's restatement on the same configuration (moments 1e-10,
+logZ 1e-10 relative) and, as the reference's own test does
+(fast_update_test.cpp:77), against ED within 5e-2.  The stochastic trace
+(KPM) against the exact spectral moments within the estimator's error.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("D,L,U,beta", [(2, 8, 1.0, 1.0), (2, 24, 8.0, 1 / 0.16), (1, 50, 2.0, 3.0)])
+def test_exact_trace_vs_oracle(fk, orc, D, L, U, beta):
+    print(f"case cheb exact D={D} L={L}", flush=True)
+    n = L ** D
+    f = orc.randomize_f(n, 32167, n // 2)
+    # reference defaults: cheb_size = int(2.2 ln N) rounded to even, grid = max(2 M, 10) (fk_mc.hxx:61-64)
+    M = int(np.log(n) * 2.2)
+    M += M % 2
+    G = max(2 * M, 10)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, D, L, 1.0, 0.0)
+    g = ctx.cheb_logz(f, beta, U, U / 2, M, G)
+    r = orc.calc_chebyshev(0, D, L, f, beta, U, U / 2, M, G, tp=0.0)
+    assert abs(g["a"] - r["a"]) < 1e-12 * abs(r["a"]) and abs(g["b"] - r["b"]) < 1e-11
+    assert np.abs(g["moments"] - r["moments"]).max() < 1e-10
+    assert abs(g["logZ"] - r["logZ"]) < 1e-10 * abs(r["logZ"])
+    ed = orc.calc_ed(0, D, L, f, beta, U, U / 2)
+    if L == 24:
+        assert abs((g["logZ"] - ed["logZ"]) / g["logZ"]) < 5e-2
+    ctx.close()
+
+
+def test_kpm_moments_and_logz(fk, orc):
+    print("case kpm", flush=True)
+    L, U, beta = 32, 2.0, 10.0
+    n = L * L
+    rng = np.random.default_rng(3)
+    f = rng.integers(0, 2, (2, n)).astype(np.int32)
+    R, M = 64, 256
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    out = ctx.kpm_logz(f, beta, U, U / 2, R=R, M=M, seed=7, jackson=True)
+    again = ctx.kpm_logz(f, beta, U, U / 2, R=R, M=M, seed=7, jackson=True)
+    assert np.array_equal(out["logZ"], again["logZ"])  # deterministic
+    for b in range(2):
+        ed = ctx.eig_logz(f[b:b + 1], beta, U, U / 2)
+        x = (ed["evals"][0] - out["b"][b]) / out["a"][b]
+        assert np.abs(x).max() < 1.0
+        exact = np.array([np.mean(np.cos(m * np.arccos(x))) for m in range(M)])
+        # Rademacher trace estimator: sd <= sqrt(2/(N R)) per moment
+        tol = 6 * np.sqrt(2.0 / (n * R))
+        assert np.abs(out["moments"][b] - exact).max() < tol
+        assert abs(out["moments"][b][0] - 1.0) < 1e-15
+        # logZ: Jackson-damped expansion + stochastic error; the estimator's sd for
+        # Tr F(X) is sqrt(2 sum_{i!=j} F_ij^2 / R), about 1% of logZ here
+        print("kpm logZ", out["logZ"][b], "ed", ed["logZ"][0], flush=True)
+        assert abs(out["logZ"][b] - ed["logZ"][0]) / ed["logZ"][0] < 3e-2
+    # a single-site flip with the same vectors: the noise largely cancels in the difference
+    g = f[0:1].copy()
+    g[0, 17] ^= 1
+    pair = np.concatenate([f[0:1], g])
+    kp = ctx.kpm_logz(pair, beta, U, U / 2, R=R, M=M, seed=7, jackson=True)
+    d_kpm = kp["logZ"][1] - kp["logZ"][0]
+    d_ed = ctx.eig_logz(g, beta, U, U / 2)["logZ"][0] - ctx.eig_logz(f[0:1], beta, U, U / 2)["logZ"][0]
+    print("kpm dlogZ", d_kpm, "ed", d_ed, flush=True)
+    assert abs(d_kpm - d_ed) < 0.1 * abs(d_ed) + 0.2
+    ctx.close()

@@ -97,11 +97,14 @@ __global__ void __launch_bounds__(KPM_NT) k_kpm(int N, int R, int M, const int32
                                                 const double* __restrict__ U, const double* __restrict__ mu_c,
                                                 const double* __restrict__ ab, const int32_t* __restrict__ nbr,
                                                 const int32_t* __restrict__ z, int zmax,
-                                                const double* __restrict__ hopv, uint64_t seed,
+                                                const double* __restrict__ hopv, uint64_t seed0,
+                                                int per_item_seed, const int32_t* __restrict__ skip,
                                                 double* __restrict__ mom_out) {
     __shared__ float vbuf[3][KPM_NMAX];
     __shared__ double dots[2][KPM_MMAX / 2 + 2][KPM_NT / WAVE];
     const int r = blockIdx.x, b = blockIdx.y;
+    if (skip && skip[b]) return;  // uniform over the workgroup
+    const uint64_t seed = per_item_seed ? seed0 + 0x9E3779B97F4A7C15ull * (uint64_t)(b + 1) : seed0;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const double inv_a = ab[2 * b], shift = ab[2 * b + 1];
     constexpr int PER = KPM_NMAX / KPM_NT;
@@ -180,9 +183,11 @@ __global__ void __launch_bounds__(KPM_NT) k_kpm(int N, int R, int M, const int32
 // Average over the R vectors (fixed order), Jackson damping, logZ = sum_m g_m c_m mu_m.
 // coef[b*M + m] = c_m (Chebyshev coefficient of log(1+e^{-beta(a x + b)})) computed on the host.
 __global__ void k_kpm_finish(int B, int N, int R, int M, const double* __restrict__ mom, const double* __restrict__ coef,
-                             int jackson, double* __restrict__ mu_out, double* __restrict__ logz) {
+                             int jackson, const int32_t* __restrict__ skip, double* __restrict__ mu_out,
+                             double* __restrict__ logz) {
     const int b = blockIdx.x;
     if (b >= B || threadIdx.x != 0) return;
+    if (skip && skip[b]) return;
     const double PI = 3.14159265358979323846;
     double s = 0.0;
     for (int m = 0; m < M; ++m) {
@@ -240,12 +245,13 @@ hipError_t fk_cheb_exact_moments(int N, const int32_t* nbr, const int32_t* z, in
 
 hipError_t fk_launch_kpm(int B, int N, int R, int M, const int32_t* f, const double* U, const double* mu_c,
                          const double* ab, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
-                         uint64_t seed, const double* coef, int jackson, double* mom, double* mu_out, double* logz,
-                         hipStream_t st) {
+                         uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,
+                         double* mom, double* mu_out, double* logz, hipStream_t st) {
     if (N > KPM_NMAX || M > KPM_MMAX || M < 2 || (M & 1) || zmax > 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_kpm, dim3(R, B), dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed, mom);
+    hipLaunchKernelGGL(k_kpm, dim3(R, B), dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
+                       per_item_seed, skip, mom);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(k_kpm_finish, dim3(B), dim3(64), 0, st, B, N, R, M, mom, coef, jackson, mu_out, logz);
+    hipLaunchKernelGGL(k_kpm_finish, dim3(B), dim3(64), 0, st, B, N, R, M, mom, coef, jackson, skip, mu_out, logz);
     return hipGetLastError();
 }

@@ -60,6 +60,16 @@ struct sweep_args {
     int8_t* tr_acc;
 };
 
+// One proposed move (kept between the launches of a stochastic-trace move step).
+struct move_prop {
+    int kind, sa, sb, solve, mode, nf_new;  // mode: 0 weighed, 1 zero weight, 2 unit weight
+    double u, ffdiff;
+};
+hipError_t fk_launch_kpm_propose(const sweep_args& a, move_prop* props, int32_t* fprop, int32_t* skip,
+                                 hipStream_t st);
+hipError_t fk_launch_kpm_accept(const sweep_args& a, const move_prop* props, const int32_t* fprop,
+                                const double* lz_new, hipStream_t st);
+
 int fk_solver_kind(int N);          // -1: unsupported size
 int fk_solver_threads(int kind);
 int fk_solver_needs_ws(int kind);
@@ -83,5 +93,5 @@ hipError_t fk_cheb_exact_moments(int N, const int32_t* nbr, const int32_t* z, in
                                  double* parts, double* scal, double* traces, hipStream_t st);
 hipError_t fk_launch_kpm(int B, int N, int R, int M, const int32_t* f, const double* U, const double* mu_c,
                          const double* ab, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
-                         uint64_t seed, const double* coef, int jackson, double* mom, double* mu_out, double* logz,
-                         hipStream_t st);
+                         uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,
+                         double* mom, double* mu_out, double* logz, hipStream_t st);

@@ -201,147 +201,206 @@ __device__ inline double ff_energy_dev(const uint8_t* f, int V, int nW, const do
     return e;
 }
 
+// Proposal of one move (thread 0 of the chain's workgroup): the draws of
+// mc_metropolis::update (src/mc_metropolis.cpp:41) and of the move's attempt()
+// (src/moves.cpp) in the reference's order, then the Metropolis uniform
+// (src/mc_metropolis.cpp:43; the eigensolve between them draws nothing).
+// f is modified in place into the proposed configuration.
+__device__ __noinline__ move_prop propose_move(uint32_t* mt, uint8_t* f, int N, int nf, const sweep_args& a) {
+    move_prop p;
+    p.kind = a.kinds[fkrng::discrete(mt, a.cp, a.ncp)];
+    p.sa = p.sb = -1;
+    p.solve = 0;
+    p.mode = 0;
+    p.nf_new = nf;
+    double ff_old = 0.0, ff_new = 0.0;
+    if (a.nW > 0) ff_old = ff_energy_dev(f, N, a.nW, a.W);
+    if (p.kind == FKGPU_MOVE_ADDREMOVE) {
+        p.sa = fkrng::uniform_int(mt, 0, N - 1);
+        f[p.sa] ^= 1;
+        p.nf_new = nf + (f[p.sa] ? 1 : -1);
+        p.solve = 1;
+    } else if (p.kind == FKGPU_MOVE_FLIP) {
+        if (nf == 0 || nf == N) {
+            p.mode = 1;  // src/moves.cpp:8: weight 0, nothing drawn
+        } else {
+            int from = fkrng::uniform_int(mt, 0, N - 1);
+            while (f[from] == 0) from = fkrng::uniform_int(mt, 0, N - 1);
+            int to = fkrng::uniform_int(mt, 0, N - 1);
+            while (f[to] == 1) to = fkrng::uniform_int(mt, 0, N - 1);
+            f[from] = 0;
+            f[to] = 1;
+            p.sa = from;
+            p.sb = to;
+            p.solve = 1;
+        }
+    } else if (p.kind == FKGPU_MOVE_HOP) {
+        const int i = fkrng::uniform_int(mt, 0, N - 1);
+        const int j = a.nbr[(size_t)i * a.zmax + fkrng::uniform_int(mt, 0, a.z[i] - 1)];
+        p.sa = i;
+        p.sb = j;
+        if (f[i] == f[j]) {
+            p.mode = 2;  // null move, weight 1
+        } else {
+            const uint8_t t = f[i];
+            f[i] = f[j];
+            f[j] = t;
+            p.solve = 1;
+        }
+    } else {  // FKGPU_MOVE_RANDOMIZE: randomize_f(rnd) with nf = 0 (src/configuration.cpp:47-56)
+        const int nfs = fkrng::uniform_int(mt, 0, N - 1);
+        for (int i = 0; i < N; ++i) f[i] = 0;
+        for (int s2 = 0; s2 < nfs; ++s2) {
+            int ind = fkrng::uniform_int(mt, 0, N - 1);
+            while (f[ind] == 1) ind = fkrng::uniform_int(mt, 0, N - 1);
+            f[ind] = 1;
+        }
+        p.nf_new = nfs;
+        p.solve = 1;
+    }
+    if (a.nW > 0 && p.solve) ff_new = ff_energy_dev(f, N, a.nW, a.W);
+    p.u = fkrng::canonical(mt);
+    p.ffdiff = ff_new - ff_old;
+    return p;
+}
+
+// Weight of the proposal and the accept test |w| > u (src/mc_metropolis.cpp:43).
+// f_new_sa: the proposed occupation of site sa.  Returns accept; logw via pointer.
+__device__ __noinline__ int decide_move(const move_prop& p, int f_new_sa, double lz_new, double lz_old, double beta,
+                                        double mu_f, int nf_old, double* logw_out) {
+    double w = 0.0, logw = -INFINITY;
+    if (p.mode == 1) {
+        w = 0.0;
+    } else if (p.mode == 2) {
+        w = 1.0;
+        logw = 0.0;
+    } else if (p.kind == FKGPU_MOVE_ADDREMOVE) {
+        // src/moves.cpp:63-66
+        const double ebmf = exp(beta * mu_f);
+        const double ratio = exp(lz_new - lz_old);
+        w = (f_new_sa ? ratio * ebmf : ratio / ebmf) * exp(-beta * p.ffdiff);
+        logw = (lz_new - lz_old) + (f_new_sa ? 1.0 : -1.0) * beta * mu_f - beta * p.ffdiff;
+    } else if (p.kind == FKGPU_MOVE_FLIP) {
+        // src/moves.cpp:19-20
+        w = exp(lz_new - lz_old);
+        logw = lz_new - lz_old;
+    } else if (p.kind == FKGPU_MOVE_HOP) {
+        w = exp(lz_new - lz_old) * exp(-beta * p.ffdiff);
+        logw = lz_new - lz_old - beta * p.ffdiff;
+    } else {
+        // src/moves.cpp:43-48 including the clamp
+        const double log_ratio = lz_new - lz_old;
+        const double dn = (double)(p.nf_new - nf_old);
+        const double g = beta * mu_f * dn - p.ffdiff;
+        logw = log_ratio + g;
+        if (g > 2.7182818 - log_ratio) w = 1.0;
+        else if (g + log_ratio < 0) w = 0.0;
+        else w = exp(log_ratio) * exp(beta * (mu_f * dn - p.ffdiff));
+    }
+    *logw_out = logw;
+    return fabs(w) > p.u;
+}
+
+// Bookkeeping of one decided move (thread 0): counters, trace, chain scalars.
+__device__ __noinline__ void record_move(const sweep_args& a, int ch, const move_prop& p, int acc, double logw,
+                                         double lz_new) {
+    if (a.trace) {
+        const size_t t = (size_t)a.step * a.B + ch;
+        a.tr_kind[t] = p.kind;
+        a.tr_a[t] = p.sa;
+        a.tr_b[t] = p.sb;
+        a.tr_u[t] = p.u;
+        a.tr_logw[t] = logw;
+        a.tr_logz[t] = p.solve ? lz_new : 0.0;
+        a.tr_acc[t] = acc;
+    }
+    if (acc) {
+        a.naccept[ch] += 1;
+        a.nf[ch] = p.nf_new;
+        if (p.solve) a.logz[ch] = lz_new;
+    }
+    if (p.solve) a.nsolve[ch] += 1;
+}
+
 template <class S>
 __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     __shared__ typename S::smem sm;
     __shared__ uint8_t f_lds[S::NMAX];
     __shared__ uint32_t mt[fkrng::MT_WORDS];
-    __shared__ double ctl_d[8];
-    __shared__ int ctl_i[8];
+    __shared__ move_prop prop;
+    __shared__ int acc_s;
     const int tid = threadIdx.x;
     const int N = a.N;
-    {
-        const int ch = blockIdx.x;  // one workgroup per chain
-        uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
-        for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
-        for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;
-        __syncthreads();
-        const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch], mu_f = a.mu_f[ch];
-        if (tid == 0) {
-            const int nf = a.nf[ch];
-            const int mi = fkrng::discrete(mt, a.cp, a.ncp);
-            const int kind = a.kinds[mi];
-            int sa = -1, sb = -1, solve = 0, mode = 0;  // mode: 0 normal, 1 zero weight, 2 unit weight
-            int nf_new = nf;
-            double ff_old = 0.0, ff_new = 0.0;
-            if (a.nW > 0) ff_old = ff_energy_dev(f_lds, N, a.nW, a.W);
-            if (kind == FKGPU_MOVE_ADDREMOVE) {
-                sa = fkrng::uniform_int(mt, 0, N - 1);
-                f_lds[sa] ^= 1;
-                nf_new = nf + (f_lds[sa] ? 1 : -1);
-                solve = 1;
-            } else if (kind == FKGPU_MOVE_FLIP) {
-                if (nf == 0 || nf == N) {
-                    mode = 1;
-                } else {
-                    int from = fkrng::uniform_int(mt, 0, N - 1);
-                    while (f_lds[from] == 0) from = fkrng::uniform_int(mt, 0, N - 1);
-                    int to = fkrng::uniform_int(mt, 0, N - 1);
-                    while (f_lds[to] == 1) to = fkrng::uniform_int(mt, 0, N - 1);
-                    f_lds[from] = 0;
-                    f_lds[to] = 1;
-                    sa = from;
-                    sb = to;
-                    solve = 1;
-                }
-            } else if (kind == FKGPU_MOVE_HOP) {
-                int i = fkrng::uniform_int(mt, 0, N - 1);
-                int zi = a.z[i];
-                int j = a.nbr[(size_t)i * a.zmax + fkrng::uniform_int(mt, 0, zi - 1)];
-                sa = i;
-                sb = j;
-                if (f_lds[i] == f_lds[j]) {
-                    mode = 2;
-                } else {
-                    uint8_t t = f_lds[i];
-                    f_lds[i] = f_lds[j];
-                    f_lds[j] = t;
-                    solve = 1;
-                }
-            } else {  // FKGPU_MOVE_RANDOMIZE: randomize_f(rnd) with nf = 0 (src/configuration.cpp:47-56)
-                int nfs = fkrng::uniform_int(mt, 0, N - 1);
-                for (int i = 0; i < N; ++i) f_lds[i] = 0;
-                for (int s = 0; s < nfs; ++s) {
-                    int ind = fkrng::uniform_int(mt, 0, N - 1);
-                    while (f_lds[ind] == 1) ind = fkrng::uniform_int(mt, 0, N - 1);
-                    f_lds[ind] = 1;
-                }
-                nf_new = nfs;
-                solve = 1;
-            }
-            if (a.nW > 0 && solve) ff_new = ff_energy_dev(f_lds, N, a.nW, a.W);
-            const double u = fkrng::canonical(mt);
-            ctl_i[0] = kind; ctl_i[1] = sa; ctl_i[2] = sb; ctl_i[3] = solve; ctl_i[4] = mode; ctl_i[5] = nf_new;
-            ctl_d[0] = u; ctl_d[1] = ff_new - ff_old;
-        }
-        __syncthreads();
-        const int solve = ctl_i[3];
-        double lz_new = 0.0;
-        if (solve) {
-            double* ws = a.ws ? a.ws + (size_t)blockIdx.x * N * N : nullptr;
-            lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
-        }
-        if (tid == 0) {
-            const int kind = ctl_i[0], mode = ctl_i[4], sa = ctl_i[1];
-            const double u = ctl_d[0], ffdiff = ctl_d[1];
-            const double lz_old = a.logz[ch];
-            double w = 0.0, logw = -INFINITY;
-            if (mode == 1) {
-                w = 0.0;
-            } else if (mode == 2) {
-                w = 1.0;
-                logw = 0.0;
-            } else if (kind == FKGPU_MOVE_ADDREMOVE) {
-                // src/moves.cpp:63-66
-                double ebmf = exp(beta * mu_f);
-                double ratio = exp(lz_new - lz_old);
-                w = (f_lds[sa] ? ratio * ebmf : ratio / ebmf) * exp(-beta * ffdiff);
-                logw = (lz_new - lz_old) + (f_lds[sa] ? 1.0 : -1.0) * beta * mu_f - beta * ffdiff;
-            } else if (kind == FKGPU_MOVE_FLIP) {
-                // src/moves.cpp:19-20
-                w = exp(lz_new - lz_old);
-                logw = lz_new - lz_old;
-            } else if (kind == FKGPU_MOVE_HOP) {
-                w = exp(lz_new - lz_old) * exp(-beta * ffdiff);
-                logw = lz_new - lz_old - beta * ffdiff;
-            } else {
-                // src/moves.cpp:43-48 including the clamp
-                double log_ratio = lz_new - lz_old;
-                double dn = (double)(ctl_i[5] - a.nf[ch]);
-                double g = beta * mu_f * dn - ffdiff;
-                logw = log_ratio + g;
-                if (g > 2.7182818 - log_ratio) w = 1.0;
-                else if (g + log_ratio < 0) w = 0.0;
-                else w = exp(log_ratio) * exp(beta * (mu_f * dn - ffdiff));
-            }
-            const int acc = fabs(w) > u;
-            ctl_i[6] = acc;
-            if (a.trace) {
-                const size_t t = (size_t)a.step * a.B + ch;
-                a.tr_kind[t] = kind;
-                a.tr_a[t] = ctl_i[1];
-                a.tr_b[t] = ctl_i[2];
-                a.tr_u[t] = u;
-                a.tr_logw[t] = logw;
-                a.tr_logz[t] = ctl_i[3] ? lz_new : 0.0;
-                a.tr_acc[t] = acc;
-            }
-            if (acc) {
-                a.naccept[ch] += 1;
-                a.nf[ch] = ctl_i[5];
-                if (ctl_i[3]) a.logz[ch] = lz_new;
-            }
-            if (ctl_i[3]) a.nsolve[ch] += 1;
-        }
-        __syncthreads();
-        if (ctl_i[6]) {
-            for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * N + i] = f_lds[i];
-            if (ctl_i[3] && a.spectrum)
-                for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = sm.ev[i];
-        }
-        for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mtg[i] = mt[i];
-        __syncthreads();
+    const int ch = blockIdx.x;  // one workgroup per chain
+    uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
+    for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
+    for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;
+    __syncthreads();
+    const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch], mu_f = a.mu_f[ch];
+    if (tid == 0) prop = propose_move(mt, f_lds, N, a.nf[ch], a);
+    __syncthreads();
+    double lz_new = 0.0;
+    if (prop.solve) {
+        double* ws = a.ws ? a.ws + (size_t)blockIdx.x * N * N : nullptr;
+        lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
     }
+    if (tid == 0) {
+        double logw;
+        const int acc = decide_move(prop, prop.sa >= 0 ? f_lds[prop.sa] : 0, lz_new, a.logz[ch], beta, mu_f,
+                                    a.nf[ch], &logw);
+        record_move(a, ch, prop, acc, logw, lz_new);
+        acc_s = acc;
+    }
+    __syncthreads();
+    if (acc_s) {
+        for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * N + i] = f_lds[i];
+        if (prop.solve && a.spectrum)
+            for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = sm.ev[i];
+    }
+    for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mtg[i] = mt[i];
+}
+
+// ---- Metropolis moves with the stochastic-trace logZ (three launches per step) ----
+constexpr int KC_NT = 64;
+constexpr int KC_NMAX = 4096;
+
+// proposals: f' (int32) for the KPM kernel, proposal record, skip flag
+__global__ void __launch_bounds__(KC_NT) k_kpm_propose(sweep_args a, move_prop* __restrict__ props,
+                                                       int32_t* __restrict__ fprop, int32_t* __restrict__ skip) {
+    __shared__ uint8_t f_lds[KC_NMAX];
+    __shared__ uint32_t mt[fkrng::MT_WORDS];
+    const int ch = blockIdx.x, tid = threadIdx.x, N = a.N;
+    uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
+    for (int i = tid; i < fkrng::MT_WORDS; i += KC_NT) mt[i] = mtg[i];
+    for (int i = tid; i < N; i += KC_NT) f_lds[i] = a.f[(size_t)ch * N + i];
+    __syncthreads();
+    if (tid == 0) {
+        const move_prop p = propose_move(mt, f_lds, N, a.nf[ch], a);
+        props[ch] = p;
+        skip[ch] = !p.solve;
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += KC_NT) fprop[(size_t)ch * N + i] = f_lds[i];
+    for (int i = tid; i < fkrng::MT_WORDS; i += KC_NT) mtg[i] = mt[i];
+}
+
+__global__ void __launch_bounds__(KC_NT) k_kpm_accept(sweep_args a, const move_prop* __restrict__ props,
+                                                      const int32_t* __restrict__ fprop,
+                                                      const double* __restrict__ lz_new_g) {
+    __shared__ int acc_s;
+    const int ch = blockIdx.x, tid = threadIdx.x, N = a.N;
+    const move_prop p = props[ch];
+    const double lz_new = p.solve ? lz_new_g[ch] : 0.0;
+    if (tid == 0) {
+        double logw;
+        const int acc = decide_move(p, p.sa >= 0 ? fprop[(size_t)ch * N + p.sa] : 0, lz_new, a.logz[ch], a.beta[ch],
+                                    a.mu_f[ch], a.nf[ch], &logw);
+        record_move(a, ch, p, acc, logw, lz_new);
+        acc_s = acc;
+    }
+    __syncthreads();
+    if (acc_s)
+        for (int i = tid; i < N; i += KC_NT) a.f[(size_t)ch * N + i] = (uint8_t)fprop[(size_t)ch * N + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -388,6 +447,17 @@ hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) 
         case 3: return launch_eig<glob_solver>(a, grid, st);
     }
     return hipErrorInvalidValue;
+}
+hipError_t fk_launch_kpm_propose(const sweep_args& a, move_prop* props, int32_t* fprop, int32_t* skip,
+                                 hipStream_t st) {
+    if (a.N > KC_NMAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_kpm_propose, dim3(a.B), dim3(KC_NT), 0, st, a, props, fprop, skip);
+    return hipGetLastError();
+}
+hipError_t fk_launch_kpm_accept(const sweep_args& a, const move_prop* props, const int32_t* fprop,
+                                const double* lz_new, hipStream_t st) {
+    hipLaunchKernelGGL(k_kpm_accept, dim3(a.B), dim3(KC_NT), 0, st, a, props, fprop, lz_new);
+    return hipGetLastError();
 }
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st) {
     switch (kind) {

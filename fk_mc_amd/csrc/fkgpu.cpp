@@ -60,6 +60,13 @@ struct fkgpu_chains {
     int32_t *t_kind = nullptr, *t_a = nullptr, *t_b = nullptr;
     double *t_u = nullptr, *t_logw = nullptr, *t_logz = nullptr;
     int8_t* t_acc = nullptr;
+    // stochastic-trace mode
+    int kpm = 0, kR = 0, kM = 0, kjack = 1;
+    uint64_t kseed = 0;
+    move_prop* k_props = nullptr;
+    int32_t *k_fprop = nullptr, *k_skip = nullptr;
+    double *k_ab = nullptr, *k_coef = nullptr, *k_mom = nullptr, *k_lz = nullptr;
+    bool dense_ok = true;
 };
 
 namespace {
@@ -233,7 +240,7 @@ int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, doubl
     if (!std::isfinite(t) || !std::isfinite(tp)) return fail(ctx, FKGPU_EINVAL, "non-finite hopping");
     long n = 1;
     for (int i = 0; i < ndim; ++i) n *= L;
-    if (fk_solver_kind((int)n) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "lattice volume beyond the dense solver range");
+    if (n > 4096) return fail(ctx, FKGPU_EUNSUPPORTED, "lattice volume above 4096 sites");
     hipSetDevice(ctx->device);
     build_hypercubic(ctx, kind, ndim, L, t, tp);
     ctx->lat_kind = kind;
@@ -245,7 +252,7 @@ int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, doubl
 int fkgpu_set_lattice_csr(fkgpu_ctx* ctx, int n, const int32_t* row_ptr, const int32_t* col, const double* val) {
     if (!ctx) return FKGPU_EINVAL;
     if (n <= 0 || !row_ptr || !col || !val) return fail(ctx, FKGPU_EINVAL, "invalid CSR arguments");
-    if (fk_solver_kind(n) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "matrix size beyond the dense solver range");
+    if (n > 4096) return fail(ctx, FKGPU_EUNSUPPORTED, "matrix size above 4096");
     std::vector<double> h((size_t)n * n, 0.0);
     std::vector<int32_t> z(n, 0);
     int zmax = 0;
@@ -302,6 +309,7 @@ struct eig_dev {
 static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
                          const double* mu_c, bool vectors, eig_dev& o) {
     const int N = ctx->N;
+    if (fk_solver_kind(N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "dense eigensolver implemented for N <= 2048");
     const size_t BN = (size_t)batch * N, BNN = BN * N;
     int r;
     if ((r = ensure(ctx, ctx->b_f, BN, &o.f)) || (r = ensure(ctx, ctx->b_beta, batch, &o.beta)) ||
@@ -356,6 +364,7 @@ int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const d
     if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
     if (batch <= 0 || !d_f || !d_beta || !d_U || !d_mu_c || !d_evals || !d_logZ)
         return fail(ctx, FKGPU_EINVAL, "null argument or empty batch");
+    if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "dense eigensolver implemented for N <= 2048");
     const int N = ctx->N;
     const int kind = fk_solver_kind(N);
     const int grid = grid_for(ctx, kind, batch);
@@ -384,6 +393,7 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     int r = check_params(ctx, batch, f, beta, U, mu_c);
     if (r) return r;
     if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
+    if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "dense eigensolver implemented for N <= 2048");
     hipSetDevice(ctx->device);
     if (want_vectors) {
         if (!evecs) return fail(ctx, FKGPU_EINVAL, "null evecs with want_vectors");
@@ -690,8 +700,8 @@ int fkgpu_kpm_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     FK_HIP(ctx, hipMemcpyAsync(dmu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dab, abv.data(), sizeof(double) * 2 * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * batch * M, hipMemcpyHostToDevice, st));
-    FK_HIP(ctx, fk_launch_kpm(batch, N, R, M, df, dU, dmu, dab, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, seed,
-                              dcoef, jackson, dmom, dmuo, dlz, st));
+    FK_HIP(ctx, fk_launch_kpm(batch, N, R, M, df, dU, dmu, dab, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, seed, 0,
+                              nullptr, dcoef, jackson, dmom, dmuo, dlz, st));
     FK_HIP(ctx, hipMemcpyAsync(logZ, dlz, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
     if (moments) FK_HIP(ctx, hipMemcpyAsync(moments, dmuo, sizeof(double) * batch * M, hipMemcpyDeviceToHost, st));
     FK_HIP(ctx, hipStreamSynchronize(st));
@@ -712,7 +722,8 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     if (ch->graph) hipGraphDestroy(ch->graph);
     void* ptrs[] = {ch->f, ch->nf, ch->logz, ch->spectrum, ch->mt, ch->beta, ch->U, ch->mu_c, ch->mu_f,
                     ch->naccept, ch->nsolve, ch->ws, ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw,
-                    ch->t_logz, ch->t_acc};
+                    ch->t_logz, ch->t_acc, ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef,
+                    ch->k_mom, ch->k_lz};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ch->ev0) hipEventDestroy(ch->ev0);
@@ -737,6 +748,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     ch->B = B;
     ch->N = N;
     ch->kind = fk_solver_kind(N);
+    ch->dense_ok = ch->kind >= 0;
     ch->grid = grid_for(ctx, ch->kind, B);
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
@@ -751,7 +763,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     chk(dalloc(&ch->mu_f, B));
     chk(dalloc(&ch->naccept, B));
     chk(dalloc(&ch->nsolve, B));
-    if (fk_solver_needs_ws(ch->kind)) chk(dalloc(&ch->ws, (size_t)ch->grid * N * N));
+    if (ch->dense_ok && fk_solver_needs_ws(ch->kind)) chk(dalloc(&ch->ws, (size_t)ch->grid * N * N));
     chk(hipEventCreate(&ch->ev0));
     chk(hipEventCreate(&ch->ev1));
     if (e != hipSuccess) {
@@ -797,7 +809,12 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     a.evals = ch->spectrum;
     a.logz = ch->logz;
     a.ws = ch->ws;
-    chk(fk_launch_eig(ch->kind, a, ch->grid, st));
+    if (ch->dense_ok) {
+        chk(fk_launch_eig(ch->kind, a, ch->grid, st));
+    } else {  // logZ comes from fkgpu_chains_set_kpm
+        chk(hipMemsetAsync(ch->logz, 0, sizeof(double) * B, st));
+        chk(hipMemsetAsync(ch->spectrum, 0, sizeof(double) * B * N, st));
+    }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
         fkgpu_chains_destroy(ch);
@@ -871,6 +888,95 @@ int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W) {
     return FKGPU_OK;
 }
 
+// One move step of every chain: the fused dense kernel, or propose / stochastic
+// trace / accept for the Chebyshev (KPM) mode.
+static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t st) {
+    if (!ch->kpm) return fk_launch_sweep(ch->kind, a, ch->grid, st);
+    fkgpu_ctx* ctx = ch->ctx;
+    hipError_t e = fk_launch_kpm_propose(a, ch->k_props, ch->k_fprop, ch->k_skip, st);
+    if (e != hipSuccess) return e;
+    e = fk_launch_kpm(ch->B, ch->N, ch->kR, ch->kM, ch->k_fprop, ch->U, ch->mu_c, ch->k_ab, ctx->d_nbr, ctx->d_z,
+                      ctx->zmax, ctx->d_hopv, ch->kseed, 1, ch->k_skip, ch->k_coef, ch->kjack, ch->k_mom, nullptr,
+                      ch->k_lz, st);
+    if (e != hipSuccess) return e;
+    return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->k_lz, st);
+}
+
+int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jackson) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    const int B = ch->B, N = ch->N;
+    if (R < 1 || M < 2 || (M & 1) || M > 512) return fail(ctx, FKGPU_EINVAL, "invalid KPM arguments");
+    if (N > 4096 || ctx->zmax > 8) return fail(ctx, FKGPU_EUNSUPPORTED, "KPM path needs N <= 4096 and <= 8 neighbours");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    std::vector<double> beta(B), U(B), mu(B);
+    FK_HIP(ctx, hipMemcpy(beta.data(), ch->beta, sizeof(double) * B, hipMemcpyDeviceToHost));
+    FK_HIP(ctx, hipMemcpy(U.data(), ch->U, sizeof(double) * B, hipMemcpyDeviceToHost));
+    FK_HIP(ctx, hipMemcpy(mu.data(), ch->mu_c, sizeof(double) * B, hipMemcpyDeviceToHost));
+    // bounds valid for every configuration of a chain: diag in {-mu_c, -mu_c + U}
+    double rmax = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double rad = 0.0;
+        for (int q = 0; q < ctx->z[i]; ++q) rad += std::fabs(ctx->hopv[(size_t)i * ctx->zmax + q]);
+        rmax = std::max(rmax, rad);
+    }
+    std::vector<double> ab(2 * (size_t)B), coef((size_t)B * M);
+    const int K = std::max(2 * M, 64);
+    for (int c = 0; c < B; ++c) {
+        const double lo = std::min(-mu[c], -mu[c] + U[c]) - rmax, hi = std::max(-mu[c], -mu[c] + U[c]) + rmax;
+        double a = 0.5 * (hi - lo) * 1.01;
+        if (!(a > 0.0)) a = 1.0;
+        const double b = 0.5 * (hi + lo);
+        ab[2 * c] = 1.0 / a;
+        ab[2 * c + 1] = b;
+        for (int m = 0; m < M; ++m) {
+            double sacc = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const double th = M_PI * (k + 0.5) / K;
+                const double y = beta[c] * (a * std::cos(th) + b);
+                const double F = y > 0 ? std::log1p(std::exp(-y)) : -y + std::log1p(std::exp(y));
+                sacc += F * std::cos(m * th);
+            }
+            coef[(size_t)c * M + m] = (m == 0 ? 1.0 : 2.0) * sacc / K;
+        }
+    }
+    void* old[] = {ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef, ch->k_mom, ch->k_lz};
+    for (void* p : old)
+        if (p) hipFree(p);
+    ch->k_props = nullptr; ch->k_fprop = ch->k_skip = nullptr;
+    ch->k_ab = ch->k_coef = ch->k_mom = ch->k_lz = nullptr;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    chk(dalloc(&ch->k_props, B));
+    chk(dalloc(&ch->k_fprop, (size_t)B * N));
+    chk(dalloc(&ch->k_skip, B));
+    chk(dalloc(&ch->k_ab, 2 * (size_t)B));
+    chk(dalloc(&ch->k_coef, (size_t)B * M));
+    chk(dalloc(&ch->k_mom, (size_t)B * R * M));
+    chk(dalloc(&ch->k_lz, B));
+    if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "KPM buffers");
+    FK_HIP(ctx, hipMemcpyAsync(ch->k_ab, ab.data(), sizeof(double) * 2 * B, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(ch->k_coef, coef.data(), sizeof(double) * B * M, hipMemcpyHostToDevice, st));
+    // logZ of the current configurations with the chain's own probe vectors
+    std::vector<uint8_t> f8((size_t)B * N);
+    FK_HIP(ctx, hipMemcpy(f8.data(), ch->f, f8.size(), hipMemcpyDeviceToHost));
+    std::vector<int32_t> f32(f8.begin(), f8.end());
+    FK_HIP(ctx, hipMemcpyAsync(ch->k_fprop, f32.data(), sizeof(int32_t) * f32.size(), hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, fk_launch_kpm(B, N, R, M, ch->k_fprop, ch->U, ch->mu_c, ch->k_ab, ctx->d_nbr, ctx->d_z, ctx->zmax,
+                              ctx->d_hopv, seed, 1, nullptr, ch->k_coef, jackson, ch->k_mom, nullptr, ch->logz, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    ch->kpm = 1;
+    ch->kR = R;
+    ch->kM = M;
+    ch->kseed = seed;
+    ch->kjack = jackson;
+    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+    ch->graph_steps = 0;
+    return FKGPU_OK;
+}
+
 int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
@@ -906,10 +1012,12 @@ int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
         a.tr_logz = ch->t_logz;
         a.tr_acc = ch->t_acc;
     }
+    if (!ch->kpm && !ch->dense_ok)
+        return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
     FK_HIP(ctx, hipEventRecord(ch->ev0, st));
     for (int s = 0; s < nsteps; ++s) {
         a.step = s;
-        FK_HIP(ctx, fk_launch_sweep(ch->kind, a, ch->grid, st));
+        FK_HIP(ctx, launch_step(ch, a, st));
     }
     FK_HIP(ctx, hipEventRecord(ch->ev1, st));
     FK_HIP(ctx, hipEventSynchronize(ch->ev1));
@@ -943,9 +1051,11 @@ int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
         sweep_args a = ch->args;
         a.trace = 0;
         FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        if (!ch->kpm && !ch->dense_ok)
+            return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
         for (int s = 0; s < nsteps; ++s) {
             a.step = s;
-            hipError_t e = fk_launch_sweep(ch->kind, a, ch->grid, st);
+            hipError_t e = launch_step(ch, a, st);
             if (e != hipSuccess) {
                 hipGraph_t g;
                 hipStreamEndCapture(st, &g);

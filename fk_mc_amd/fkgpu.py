@@ -78,6 +78,7 @@ def lib():
         L.fkgpu_chains_set_moves.argtypes = [P, C.c_int, _i32, _d]
         L.fkgpu_chains_set_ff.argtypes = [P, C.c_int, _d]
         L.fkgpu_sweep.argtypes = [P, C.c_int, C.POINTER(_Trace)]
+        L.fkgpu_chains_set_kpm.argtypes = [P, C.c_int, C.c_int, C.c_uint64, C.c_int]
         L.fkgpu_sweep_graph.argtypes = [P, C.c_int]
         L.fkgpu_chains_last_ms.argtypes = [P, _d]
         L.fkgpu_chains_state.argtypes = [P, _i32, _i32, _d, _d, _i64, _i64]
@@ -294,6 +295,9 @@ class Chains:
         kinds = np.array([m[0] for m in moves], dtype=np.int32)
         w = np.array([m[1] for m in moves], dtype=np.float64)
         self.ctx._check(lib().fkgpu_chains_set_moves(self._h, len(kinds), _p(kinds, _i32), _p(w, _d)))
+
+    def set_kpm(self, R=64, M=256, seed=1, jackson=True):
+        self.ctx._check(lib().fkgpu_chains_set_kpm(self._h, R, M, seed, int(jackson)))
 
     def set_ff(self, W):
         W = np.ascontiguousarray(W, dtype=np.float64)

@@ -145,6 +145,14 @@ void fkgpu_chains_destroy(fkgpu_chains* ch);
 int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, const double* weights);
 /* f-f interaction W of 1-D lattices (configuration_t::calc_ff_energy, src/configuration.cpp:59-77). */
 int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W);
+/* Switch the chain set to the stochastic-trace logZ (the Chebyshev moves of
+ * src/moves_chebyshev.cpp with the kernel-polynomial estimate instead of the exact
+ * trace): R probe vectors per chain (fixed for the whole run, so old and new
+ * configurations see the same vectors), M moments, Jackson damping if jackson != 0,
+ * spectral bounds valid for every configuration of the chain.  The chains' logZ is
+ * re-estimated; each move step then takes three launches (propose, trace, accept).
+ * Required for lattices above the dense solver's size (N > 2048). */
+int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jackson);
 /* Run nsteps single moves on every chain (one launch per move step).  trace may be null. */
 int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace);
 /* Replay nsteps move steps from a captured HIP graph (no trace). */

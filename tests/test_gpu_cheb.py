@@ -67,3 +67,75 @@ def test_kpm_moments_and_logz(fk, orc):
     print("kpm dlogZ", d_kpm, "ed", d_ed, flush=True)
     assert abs(d_kpm - d_ed) < 0.1 * abs(d_ed) + 0.2
     ctx.close()
+
+
+def test_kpm_chains(fk, orc):
+    """Chebyshev-move chains (src/moves_chebyshev.cpp) with the stochastic trace:
+    draws follow the reference order, dlogZ of each solved proposal agrees with
+    dense ED within the estimator's noise, graph replay equals eager launches."""
+    print("case kpm chains", flush=True)
+    L, U, beta = 32, 2.0, 10.0
+    n = L * L
+    rng = np.random.default_rng(11)
+    B = 4
+    f0 = rng.integers(0, 2, (B, n)).astype(np.int32)
+    seeds = np.arange(B, dtype=np.uint32) + 500
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
+    ch.set_kpm(R=64, M=256, seed=3, jackson=True)
+    lz0 = ch.state()["logZ"].copy()
+    tr = ch.sweep(12, trace=True)
+    # first move of every chain: same site and uniform as the reference chain
+    for c in range(B):
+        ref = orc.run_chain(0, 2, L, beta, U, U / 2, U / 2, int(seeds[c]), f0[c], [(orc.MOVE_ADDREMOVE, 1.0)], 1,
+                            tp=0.0)
+        assert tr["a"][0, c] == ref["a"][0] and tr["u"][0, c] == ref["u"][0]
+        # the chain's estimates are the one-shot estimate with the chain's probe vectors
+        # (seed + golden*(c+1)) and bounds (those of any configuration holding both occupations)
+        g = f0[c:c + 1].copy()
+        g[0, tr["a"][0, c]] ^= 1
+        seed_c = (3 + 0x9E3779B97F4A7C15 * (c + 1)) % 2 ** 64
+        one = ctx.kpm_logz(np.concatenate([f0[c:c + 1], g]), beta, U, U / 2, R=64, M=256, seed=seed_c, jackson=True)
+        assert abs(one["logZ"][0] - lz0[c]) <= 1e-12 * lz0[c]
+        assert abs(one["logZ"][1] - tr["logz"][0, c]) <= 1e-12 * lz0[c]
+        # against dense ED: the estimator's noise on a difference is a few units at beta=10, R=64
+        d_ed = ctx.eig_logz(g, beta, U, U / 2)["logZ"][0] - ctx.eig_logz(f0[c:c + 1], beta, U, U / 2)["logZ"][0]
+        d_kpm = tr["logz"][0, c] - lz0[c]
+        print("chain", c, "dlogZ kpm", d_kpm, "ed", d_ed, flush=True)
+        assert abs(d_kpm - d_ed) < 10.0
+    # graph replay vs eager from identical starting points
+    a = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
+    b = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
+    for x in (a, b):
+        x.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+        x.set_kpm(R=16, M=64, seed=9, jackson=True)
+    for rep in range(2):
+        a.sweep(8)
+        b.sweep_graph(8)
+        sa, sb = a.state(), b.state()
+        assert np.array_equal(sa["f"], sb["f"]) and np.array_equal(sa["logZ"], sb["logZ"]), rep
+    for x in (ch, a, b):
+        x.close()
+    ctx.close()
+
+
+def test_kpm_chain_large_lattice(fk):
+    """cubic2d L=64 (N=4096) runs on the stochastic trace only (above the dense range)."""
+    print("case kpm L=64", flush=True)
+    L = 64
+    n = L * L
+    rng = np.random.default_rng(2)
+    f0 = rng.integers(0, 2, (2, n)).astype(np.int32)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, f0, 10.0, 2.0, 1.0, 1.0, [1, 2])
+    with pytest.raises(fk.FkgpuError):
+        ch.sweep(1)  # no dense solver at this size
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
+    ch.set_kpm(R=64, M=256, seed=5, jackson=True)
+    tr = ch.sweep(4, trace=True)
+    st = ch.state()
+    assert np.all(np.isfinite(st["logZ"])) and np.all(st["logZ"] > 0)
+    assert np.all(tr["logz"][tr["kind"] == 0] != 0.0)
+    ch.close()
+    ctx.close()

@@ -140,7 +140,7 @@ typedef struct {
 int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, const double* beta, const double* U,
                         const double* mu_c, const double* mu_f, const uint32_t* seeds, fkgpu_chains** out);
 void fkgpu_chains_destroy(fkgpu_chains* ch);
-/* mc_metropolis::add_move in order (include/fk_mc/mc_metropolis.hpp:263-270):
+/* mc_metropolis::add_move in order (include/fk_mc/mc_metropolis.hpp:176-185):
  * move kinds and their probabilities (up to 4). */
 int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, const double* weights);
 /* f-f interaction W of 1-D lattices (configuration_t::calc_ff_energy, src/configuration.cpp:59-77). */

@@ -18,7 +18,9 @@ struct eig_args {
     double* logz;          // B
     double* cexp;          // B x N or null
     double* cfermi;        // B x N or null
-    double* ws;            // grid x N x N workspace (global-memory solver) or null
+    double* ws;            // grid x ws_stride workspace (HBM-resident solvers) or null
+    size_t ws_stride;      // doubles per workgroup
+    const int32_t* only;   // B flags: solve only where nonzero (null: all)
     double* vout;          // B x N x N Householder vectors or null
     double* tauout;        // B x N or null
     double* d_out;         // B x N tridiagonal diagonal or null
@@ -43,7 +45,8 @@ struct sweep_args {
     const double* mu_f;
     long long* naccept;    // B
     long long* nsolve;     // B
-    double* ws;            // grid x N x N workspace or null
+    double* ws;            // grid x ws_stride workspace or null
+    size_t ws_stride;      // doubles per workgroup
     int ncp;               // number of registered moves
     double cp[4];          // cumulative move probabilities (libstdc++ discrete_distribution)
     int kinds[4];          // move kind of each registered move
@@ -73,6 +76,8 @@ hipError_t fk_launch_kpm_accept(const sweep_args& a, const move_prop* props, con
 int fk_solver_kind(int N);          // -1: unsupported size
 int fk_solver_threads(int kind);
 int fk_solver_needs_ws(int kind);
+int fk_solver_nmax(int kind);
+size_t fk_solver_ws_doubles(int kind, int N);
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fk_blocked.h"
 #include "fk_internal.h"
 #include "fk_rng.h"
 #include "fk_solve.h"
@@ -18,6 +19,7 @@
 namespace fkk {
 
 using namespace fks;
+using fkb::blocked_solver;
 
 // ---------------------------------------------------------------------------
 // Solver policies.  Each provides:
@@ -39,6 +41,7 @@ struct reg_solver {
     };
     __device__ static const double* dptr(const smem& sm) { return sm.t.d; }
     __device__ static const double* eptr(const smem& sm) { return sm.t.e; }
+    __device__ static double* evptr(smem& sm) { return sm.ev; }
     __device__ __forceinline__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
                                    double beta, double U, double mu_c, double* ws, double* vout, double* tauout,
                                    double* cexp, double* cfermi) {
@@ -67,6 +70,7 @@ struct glob_solver {
     };
     __device__ static const double* dptr(const smem& sm) { return sm.d; }
     __device__ static const double* eptr(const smem& sm) { return sm.e; }
+    __device__ static double* evptr(smem& sm) { return sm.ev; }
     __device__ __forceinline__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds, int N,
                                    double beta, double U, double mu_c, double* __restrict__ W, double* vout,
                                    double* tauout, double* cexp, double* cfermi) {
@@ -161,15 +165,17 @@ __global__ void __launch_bounds__(S::NT) k_eig_logz(eig_args a) {
     // one workgroup per configuration (a grid-stride loop here costs ~25 spilled VGPRs)
     {
         const int bidx = blockIdx.x;
+        if (a.only && !a.only[bidx]) return;  // uniform over the workgroup
         for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? (uint8_t)(a.f[(size_t)bidx * N + i] != 0) : 0;
         __syncthreads();
-        double* ws = a.ws ? a.ws + (size_t)blockIdx.x * N * N : nullptr;
+        double* ws = a.ws ? a.ws + (size_t)blockIdx.x * a.ws_stride : nullptr;
         double* vout = a.vout ? a.vout + (size_t)bidx * N * N : nullptr;
         double* tauout = a.tauout ? a.tauout + (size_t)bidx * N : nullptr;
         double lz = S::solve(sm, a.hop, f_lds, N, a.beta[bidx], a.U[bidx], a.mu_c[bidx], ws, vout, tauout,
                              a.cexp ? a.cexp + (size_t)bidx * N : nullptr,
                              a.cfermi ? a.cfermi + (size_t)bidx * N : nullptr);
-        for (int i = tid; i < N; i += S::NT) a.evals[(size_t)bidx * N + i] = sm.ev[i];
+        const double* ev = S::evptr(sm);
+        for (int i = tid; i < N; i += S::NT) a.evals[(size_t)bidx * N + i] = ev[i];
         if (tid == 0) a.logz[bidx] = lz;
         if (a.d_out) {
             // tridiagonal (d, e) for the eigenvector stage
@@ -341,7 +347,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     __syncthreads();
     double lz_new = 0.0;
     if (prop.solve) {
-        double* ws = a.ws ? a.ws + (size_t)blockIdx.x * N * N : nullptr;
+        double* ws = a.ws ? a.ws + (size_t)blockIdx.x * a.ws_stride : nullptr;
         lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
     }
     if (tid == 0) {
@@ -355,7 +361,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     if (acc_s) {
         for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * N + i] = f_lds[i];
         if (prop.solve && a.spectrum)
-            for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = sm.ev[i];
+            for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = S::evptr(sm)[i];
     }
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mtg[i] = mt[i];
 }
@@ -421,12 +427,30 @@ static hipError_t launch_sweep(const sweep_args& a, int grid, hipStream_t st) {
 
 using namespace fkk;
 
+// Solver kinds: 0-2 register-resident one-stage (N <= 64 / 128 / 256), 3 unblocked
+// HBM workspace (N <= 2048), 4 blocked FP64 with MFMA trailing updates (N <= 1024),
+// 5 blocked FP32 (first stage of the mixed-precision path, N <= 1024).
+typedef blocked_solver<double, 512> blk64;
+typedef blocked_solver<float, 512> blk32;
+
 int fk_solver_kind(int N) {
     if (N <= 64) return 0;
     if (N <= 128) return 1;
     if (N <= 256) return 2;
+    if (N <= blk64::NMAX) return 4;
     if (N <= glob_solver::NMAX) return 3;
     return -1;
+}
+int fk_solver_nmax(int kind) {
+    switch (kind) {
+        case 0: return reg_solver<16, 4>::NMAX;
+        case 1: return reg_solver<16, 8>::NMAX;
+        case 2: return reg_solver<32, 8>::NMAX;
+        case 3: return glob_solver::NMAX;
+        case 4: return blk64::NMAX;
+        case 5: return blk32::NMAX;
+    }
+    return 0;
 }
 int fk_solver_threads(int kind) {
     switch (kind) {
@@ -434,10 +458,20 @@ int fk_solver_threads(int kind) {
         case 1: return reg_solver<16, 8>::NT;
         case 2: return reg_solver<32, 8>::NT;
         case 3: return glob_solver::NT;
+        case 4: return blk64::NT;
+        case 5: return blk32::NT;
     }
     return 0;
 }
-int fk_solver_needs_ws(int kind) { return kind == 3; }
+int fk_solver_needs_ws(int kind) { return kind >= 3; }
+size_t fk_solver_ws_doubles(int kind, int N) {
+    switch (kind) {
+        case 3: return (size_t)N * N;
+        case 4: return blk64::ws_doubles(N);
+        case 5: return blk32::ws_doubles(N);
+    }
+    return 0;
+}
 
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) {
     switch (kind) {
@@ -445,6 +479,8 @@ hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) 
         case 1: return launch_eig<reg_solver<16, 8>>(a, grid, st);
         case 2: return launch_eig<reg_solver<32, 8>>(a, grid, st);
         case 3: return launch_eig<glob_solver>(a, grid, st);
+        case 4: return launch_eig<blk64>(a, grid, st);
+        case 5: return launch_eig<blk32>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }
@@ -465,6 +501,7 @@ hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t 
         case 1: return launch_sweep<reg_solver<16, 8>>(a, grid, st);
         case 2: return launch_sweep<reg_solver<32, 8>>(a, grid, st);
         case 3: return launch_sweep<glob_solver>(a, grid, st);
+        case 4: return launch_sweep<blk64>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }

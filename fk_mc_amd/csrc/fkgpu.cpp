@@ -25,6 +25,7 @@ struct fkgpu_ctx {
     int num_cu = 0;
     int N = 0;
     int lat_kind = -1, lat_ndim = 0, lat_L = 0;
+    int solver = -1;  // forced solver kind (fkgpu_set_solver / fkgpu_autotune), -1: by size
     double* d_hop = nullptr;
     double* d_hopv = nullptr;  // hopping value of each neighbour slot (N x zmax)
     int32_t* d_nbr = nullptr;
@@ -50,6 +51,7 @@ struct fkgpu_chains {
     double *beta = nullptr, *U = nullptr, *mu_c = nullptr, *mu_f = nullptr;
     long long *naccept = nullptr, *nsolve = nullptr;
     double* ws = nullptr;
+    size_t ws_stride = 0;
     sweep_args args{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -109,6 +111,12 @@ bool finite_all(const double* p, int n) {
 // One workgroup per configuration / chain; the global-memory solver keeps one
 // N x N workspace per workgroup, so its grid is one batch of resident groups
 // and the kernel is launched repeatedly over slices of the batch.
+// Solver kind for the context's lattice: the forced / autotuned one when it fits N.
+int pick_kind(const fkgpu_ctx* ctx) {
+    if (ctx->solver >= 0 && ctx->solver != 5 && ctx->N <= fk_solver_nmax(ctx->solver)) return ctx->solver;
+    return fk_solver_kind(ctx->N);
+}
+
 int grid_for(fkgpu_ctx* ctx, int kind, int B) {
     (void)ctx;
     (void)kind;
@@ -323,7 +331,7 @@ static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
     FK_HIP(ctx, hipMemcpyAsync(o.beta, beta, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(o.U, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(o.mu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
-    const int kind = fk_solver_kind(N);
+    const int kind = pick_kind(ctx);
     const int grid = grid_for(ctx, kind, batch);
     eig_args a{};
     a.B = batch;
@@ -337,7 +345,8 @@ static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
     a.logz = o.lz;
     a.cexp = o.ce;
     a.cfermi = o.cf;
-    if (fk_solver_needs_ws(kind) && (r = ensure(ctx, ctx->b_ws, (size_t)grid * N * N, &a.ws))) return r;
+    a.ws_stride = fk_solver_ws_doubles(kind, N);
+    if (fk_solver_needs_ws(kind) && (r = ensure(ctx, ctx->b_ws, (size_t)grid * a.ws_stride, &a.ws))) return r;
     double *vh = nullptr, *tau = nullptr, *dd = nullptr, *ee = nullptr, *Z = nullptr, *scr = nullptr;
     if (vectors) {
         if (N > 1024) return fail(ctx, FKGPU_EUNSUPPORTED, "eigenvectors are implemented for N <= 1024");
@@ -358,6 +367,94 @@ static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
     return FKGPU_OK;
 }
 
+int fkgpu_set_solver(fkgpu_ctx* ctx, int variant) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (variant < FKGPU_SOLVER_AUTO || variant >= FKGPU_SOLVER_VARIANTS)
+        return fail(ctx, FKGPU_EINVAL, "unknown solver variant");
+    if (variant >= 0 && ctx->N > 0 && ctx->N > fk_solver_nmax(variant))
+        return fail(ctx, FKGPU_EUNSUPPORTED, "solver variant does not fit this lattice size");
+    ctx->solver = variant;
+    return FKGPU_OK;
+}
+
+int fkgpu_get_solver(const fkgpu_ctx* ctx) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (ctx->N == 0) return FKGPU_ENOLATTICE;
+    return pick_kind(ctx);
+}
+
+int fkgpu_autotune(fkgpu_ctx* ctx, int batch, double* ms) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
+    if (batch <= 0) return fail(ctx, FKGPU_EINVAL, "empty batch");
+    hipSetDevice(ctx->device);
+    const int N = ctx->N;
+    const size_t BN = (size_t)batch * N;
+    // synthetic half-filled configurations (a fixed LCG), beta = 1, U = 2, mu_c = 1
+    std::vector<int32_t> f(BN);
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (size_t i = 0; i < BN; ++i) {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        f[i] = (int32_t)(x >> 63);
+    }
+    std::vector<double> one(batch, 1.0), two(batch, 2.0);
+    int32_t* df;
+    double *dbeta, *dU, *dmu, *dev, *dlz;
+    int r;
+    if ((r = ensure(ctx, ctx->b_f, BN, &df)) || (r = ensure(ctx, ctx->b_beta, batch, &dbeta)) ||
+        (r = ensure(ctx, ctx->b_U, batch, &dU)) || (r = ensure(ctx, ctx->b_mu, batch, &dmu)) ||
+        (r = ensure(ctx, ctx->b_ev, BN, &dev)) || (r = ensure(ctx, ctx->b_lz, batch, &dlz)))
+        return r;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemcpyAsync(df, f.data(), sizeof(int32_t) * BN, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dbeta, one.data(), sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dU, two.data(), sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dmu, one.data(), sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    hipEvent_t e0, e1;
+    FK_HIP(ctx, hipEventCreate(&e0));
+    FK_HIP(ctx, hipEventCreate(&e1));
+    int best = -1;
+    double best_ms = 0.0;
+    for (int v = 0; v < FKGPU_SOLVER_VARIANTS; ++v) {
+        if (ms) ms[v] = -1.0;
+        if (N > fk_solver_nmax(v)) continue;
+        const int grid = grid_for(ctx, v, batch);
+        eig_args a{};
+        a.B = batch;
+        a.N = N;
+        a.hop = ctx->d_hop;
+        a.f = df;
+        a.beta = dbeta;
+        a.U = dU;
+        a.mu_c = dmu;
+        a.evals = dev;
+        a.logz = dlz;
+        a.ws_stride = fk_solver_ws_doubles(v, N);
+        if (fk_solver_needs_ws(v) && (r = ensure(ctx, ctx->b_ws, (size_t)grid * a.ws_stride, &a.ws))) break;
+        hipError_t he = fk_launch_eig(v, a, grid, st);
+        if (he == hipSuccess) he = hipEventRecord(e0, st);
+        for (int it = 0; it < 3 && he == hipSuccess; ++it) he = fk_launch_eig(v, a, grid, st);
+        if (he == hipSuccess) he = hipEventRecord(e1, st);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
+        float t = 0.f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
+        if (he != hipSuccess) {
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            return hipfail(ctx, he, "autotune launch");
+        }
+        const double tv = t / 3.0;
+        if (ms) ms[v] = tv;
+        if (best < 0 || tv < best_ms) { best = v; best_ms = tv; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (r) return r;
+    if (best < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "no solver variant fits this lattice size");
+    ctx->solver = best;
+    return best;
+}
+
 int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta, const double* d_U,
                           const double* d_mu_c, double* d_evals, double* d_logZ) {
     if (!ctx) return FKGPU_EINVAL;
@@ -366,7 +463,7 @@ int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const d
         return fail(ctx, FKGPU_EINVAL, "null argument or empty batch");
     if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "dense eigensolver implemented for N <= 2048");
     const int N = ctx->N;
-    const int kind = fk_solver_kind(N);
+    const int kind = pick_kind(ctx);
     const int grid = grid_for(ctx, kind, batch);
     eig_args a{};
     a.B = batch;
@@ -378,8 +475,9 @@ int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const d
     a.mu_c = d_mu_c;
     a.evals = d_evals;
     a.logz = d_logZ;
+    a.ws_stride = fk_solver_ws_doubles(kind, N);
     if (fk_solver_needs_ws(kind)) {
-        int r = ensure(ctx, ctx->b_ws, (size_t)grid * N * N, &a.ws);
+        int r = ensure(ctx, ctx->b_ws, (size_t)grid * a.ws_stride, &a.ws);
         if (r) return r;
     }
     hipSetDevice(ctx->device);
@@ -425,7 +523,7 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     FK_HIP(ctx, hipMemcpyAsync(dbeta, beta, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dU, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dmu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
-    const int kind = fk_solver_kind(N);
+    const int kind = pick_kind(ctx);
     const int grid = grid_for(ctx, kind, batch);
     eig_args a{};
     a.B = batch;
@@ -439,8 +537,9 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     a.logz = dlz;
     a.cexp = dce;
     a.cfermi = dcf;
+    a.ws_stride = fk_solver_ws_doubles(kind, N);
     if (fk_solver_needs_ws(kind)) {
-        if ((r = ensure(ctx, ctx->b_ws, (size_t)grid * N * N, &a.ws))) return r;
+        if ((r = ensure(ctx, ctx->b_ws, (size_t)grid * a.ws_stride, &a.ws))) return r;
     }
     FK_HIP(ctx, fk_launch_eig(kind, a, grid, st));
     FK_HIP(ctx, hipMemcpyAsync(evals, dev, sizeof(double) * BN, hipMemcpyDeviceToHost, st));
@@ -747,7 +846,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     ctx->refs += 1;
     ch->B = B;
     ch->N = N;
-    ch->kind = fk_solver_kind(N);
+    ch->kind = pick_kind(ctx);
     ch->dense_ok = ch->kind >= 0;
     ch->grid = grid_for(ctx, ch->kind, B);
     hipError_t e = hipSuccess;
@@ -763,7 +862,8 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     chk(dalloc(&ch->mu_f, B));
     chk(dalloc(&ch->naccept, B));
     chk(dalloc(&ch->nsolve, B));
-    if (ch->dense_ok && fk_solver_needs_ws(ch->kind)) chk(dalloc(&ch->ws, (size_t)ch->grid * N * N));
+    ch->ws_stride = ch->dense_ok ? fk_solver_ws_doubles(ch->kind, N) : 0;
+    if (ch->dense_ok && fk_solver_needs_ws(ch->kind)) chk(dalloc(&ch->ws, (size_t)ch->grid * ch->ws_stride));
     chk(hipEventCreate(&ch->ev0));
     chk(hipEventCreate(&ch->ev1));
     if (e != hipSuccess) {
@@ -809,6 +909,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     a.evals = ch->spectrum;
     a.logz = ch->logz;
     a.ws = ch->ws;
+    a.ws_stride = ch->ws_stride;
     if (ch->dense_ok) {
         chk(fk_launch_eig(ch->kind, a, ch->grid, st));
     } else {  // logZ comes from fkgpu_chains_set_kpm
@@ -839,6 +940,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     s.naccept = ch->naccept;
     s.nsolve = ch->nsolve;
     s.ws = ch->ws;
+    s.ws_stride = ch->ws_stride;
     s.ncp = 1;
     s.cp[0] = 1.0;
     s.kinds[0] = FKGPU_MOVE_ADDREMOVE;

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libfkgpu.so")
 OK, EINVAL, ENOLATTICE, EUNSUPPORTED, EHIP, ENOMEM = 0, -1, -2, -3, -4, -5
 LATTICE_HYPERCUBIC, LATTICE_TRIANGULAR = 0, 1
 MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
+SOLVER_AUTO, SOLVER_VARIANTS = -1, 5
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -88,6 +89,9 @@ def lib():
         L.fkgpu_dos.argtypes = [P, C.c_int, _d, C.c_int, _d, C.c_double, _d]
         L.fkgpu_adjoint.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _d, _d]
         L.fkgpu_cheb_logz.argtypes = [P, _i32, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _d, _d, _d]
+        L.fkgpu_set_solver.argtypes = [P, C.c_int]
+        L.fkgpu_get_solver.argtypes = [P]
+        L.fkgpu_autotune.argtypes = [P, C.c_int, _d]
         L.fkgpu_kpm_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, C.c_int, C.c_uint64, C.c_int, _d, _d, _d]
         _LIB = L
     return _LIB
@@ -250,6 +254,28 @@ def _kpm_logz(self, f, beta, U, mu_c, R=64, M=256, seed=1, jackson=True):
     return dict(logZ=lz, moments=mom, a=ab[:, 0], b=ab[:, 1])
 
 
+def _set_solver(self, variant):
+    """Force a dense eigensolver variant (-1: by size)."""
+    self._check(lib().fkgpu_set_solver(self._h, variant))
+    return self
+
+
+def _get_solver(self):
+    return lib().fkgpu_get_solver(self._h)
+
+
+def _autotune(self, batch=256):
+    """Time every solver variant that fits N; keeps and returns the fastest and the times."""
+    ms = np.zeros(SOLVER_VARIANTS)
+    r = lib().fkgpu_autotune(self._h, batch, _p(ms, _d))
+    if r < 0:
+        self._check(r)
+    return r, ms
+
+
+Context.set_solver = _set_solver
+Context.get_solver = _get_solver
+Context.autotune = _autotune
 Context.cheb_logz = _cheb_logz
 Context.kpm_logz = _kpm_logz
 Context.measure_ipr = _measure_ipr

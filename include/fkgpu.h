@@ -60,6 +60,24 @@ int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, doubl
 int fkgpu_set_lattice_csr(fkgpu_ctx* ctx, int n, const int32_t* row_ptr, const int32_t* col, const double* val);
 int fkgpu_msize(const fkgpu_ctx* ctx);
 
+/* Dense eigensolver variants (no reference counterpart; the reference always uses
+ * Eigen's SelfAdjointEigenSolver).  All are FP64 and give the same eigenvalues to
+ * rounding; they differ in how the Householder tridiagonalisation is mapped on the GPU:
+ *   FKGPU_SOLVER_AUTO       by matrix size (or the fkgpu_autotune choice)
+ *   0, 1, 2                 register-resident one-stage, P x P threads holding NB x NB
+ *                           blocks: (P, NB) = (16, 4) N <= 64, (16, 8) N <= 128, (32, 8) N <= 256
+ *   3                       unblocked, matrix in an HBM workspace, N <= 2048
+ *   4                       blocked (panel + MFMA rank-2nb trailing update), HBM workspace, N <= 1024 */
+#define FKGPU_SOLVER_AUTO (-1)
+#define FKGPU_SOLVER_VARIANTS 5
+int fkgpu_set_solver(fkgpu_ctx* ctx, int variant);
+int fkgpu_get_solver(const fkgpu_ctx* ctx); /* the variant used for the current lattice */
+/* Autotuner: times every variant that fits the lattice's N on `batch` synthetic
+ * configurations (one warm-up and three timed launches each), keeps the fastest for
+ * this context and returns it; ms[v] (FKGPU_SOLVER_VARIANTS entries, nullable) gets
+ * the time per launch, or -1 where the variant does not fit. */
+int fkgpu_autotune(fkgpu_ctx* ctx, int batch, double* ms);
+
 /* configuration_t::calc_hamiltonian() + calc_ed(bool calc_evecs) for `batch`
  * configurations (src/configuration.cpp:79-91, 208-246).  Outputs per config:
  * evals[N] (ed_cache::cached_spectrum), logZ (ed_cache::logZ), cached_exp[N],

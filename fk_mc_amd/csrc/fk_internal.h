@@ -20,7 +20,7 @@ struct eig_args {
     double* cfermi;        // B x N or null
     double* ws;            // grid x ws_stride workspace (HBM-resident solvers) or null
     size_t ws_stride;      // doubles per workgroup
-    const int32_t* only;   // B flags: solve only where nonzero (null: all)
+    const int32_t* skip;   // B flags: no solve where nonzero (null: solve all)
     double* vout;          // B x N x N Householder vectors or null
     double* tauout;        // B x N or null
     double* d_out;         // B x N tridiagonal diagonal or null
@@ -71,7 +71,13 @@ struct move_prop {
 hipError_t fk_launch_kpm_propose(const sweep_args& a, move_prop* props, int32_t* fprop, int32_t* skip,
                                  hipStream_t st);
 hipError_t fk_launch_kpm_accept(const sweep_args& a, const move_prop* props, const int32_t* fprop,
-                                const double* lz_new, hipStream_t st);
+                                const double* lz_new, hipStream_t st, const double* spec_new = nullptr);
+hipError_t fk_launch_mixed_refine(int B, int N, const double* X, const int32_t* f, const double* U,
+                                  const double* mu_c, const int32_t* nbr, const int32_t* z, int zmax,
+                                  const double* hopv, double* Y, double* S, double* T, double* Q,
+                                  const double* beta, double* evals, double* logz, int32_t* fb_skip,
+                                  const int32_t* skip, unsigned long long* nflag, hipStream_t st);
+int fk_mixed_cluster_max();
 
 int fk_solver_kind(int N);          // -1: unsupported size
 int fk_solver_threads(int kind);
@@ -82,7 +88,8 @@ hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
 
 hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,
-                           const double* tau, double* Z, double* scratch, hipStream_t st);
+                           const double* tau, double* Z, double* scratch, hipStream_t st,
+                           const int32_t* skip = nullptr);
 hipError_t fk_launch_ipr(int B, int N, const double* V, double* out, hipStream_t st);
 hipError_t fk_launch_stiffness(int B, int N, int stride0, int L0, double t, const double* V, const double* ev,
                                const double* fermi, int nw, const double* wgrid, double offset, double* JV,

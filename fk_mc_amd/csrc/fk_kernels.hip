@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(S::NT) k_eig_logz(eig_args a) {
     // one workgroup per configuration (a grid-stride loop here costs ~25 spilled VGPRs)
     {
         const int bidx = blockIdx.x;
-        if (a.only && !a.only[bidx]) return;  // uniform over the workgroup
+        if (a.skip && a.skip[bidx]) return;  // uniform over the workgroup
         for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? (uint8_t)(a.f[(size_t)bidx * N + i] != 0) : 0;
         __syncthreads();
         double* ws = a.ws ? a.ws + (size_t)blockIdx.x * a.ws_stride : nullptr;
@@ -392,7 +392,8 @@ __global__ void __launch_bounds__(KC_NT) k_kpm_propose(sweep_args a, move_prop* 
 
 __global__ void __launch_bounds__(KC_NT) k_kpm_accept(sweep_args a, const move_prop* __restrict__ props,
                                                       const int32_t* __restrict__ fprop,
-                                                      const double* __restrict__ lz_new_g) {
+                                                      const double* __restrict__ lz_new_g,
+                                                      const double* __restrict__ spec_new) {
     __shared__ int acc_s;
     const int ch = blockIdx.x, tid = threadIdx.x, N = a.N;
     const move_prop p = props[ch];
@@ -405,8 +406,11 @@ __global__ void __launch_bounds__(KC_NT) k_kpm_accept(sweep_args a, const move_p
         acc_s = acc;
     }
     __syncthreads();
-    if (acc_s)
+    if (acc_s) {
         for (int i = tid; i < N; i += KC_NT) a.f[(size_t)ch * N + i] = (uint8_t)fprop[(size_t)ch * N + i];
+        if (spec_new && p.solve && a.spectrum)
+            for (int i = tid; i < N; i += KC_NT) a.spectrum[(size_t)ch * N + i] = spec_new[(size_t)ch * N + i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -491,8 +495,8 @@ hipError_t fk_launch_kpm_propose(const sweep_args& a, move_prop* props, int32_t*
     return hipGetLastError();
 }
 hipError_t fk_launch_kpm_accept(const sweep_args& a, const move_prop* props, const int32_t* fprop,
-                                const double* lz_new, hipStream_t st) {
-    hipLaunchKernelGGL(k_kpm_accept, dim3(a.B), dim3(KC_NT), 0, st, a, props, fprop, lz_new);
+                                const double* lz_new, hipStream_t st, const double* spec_new) {
+    hipLaunchKernelGGL(k_kpm_accept, dim3(a.B), dim3(KC_NT), 0, st, a, props, fprop, lz_new, spec_new);
     return hipGetLastError();
 }
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st) {

@@ -80,13 +80,14 @@ __device__ inline double col_norm2(const double* x, int N, size_t ld) {
 // scratch: B x 3 x N x N.
 __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* __restrict__ dg,
                                                     const double* __restrict__ eg, const double* __restrict__ evg,
-                                                    double* __restrict__ Zg, double* __restrict__ scr) {
+                                                    double* __restrict__ Zg, double* __restrict__ scr,
+                                                    const int32_t* __restrict__ skip) {
     __shared__ double d[2048], e[2048];
     __shared__ int cstart[2048];
     __shared__ int maxpos;
     const int b = blockIdx.x;
     const int tid = threadIdx.x, NT = blockDim.x;
-    if (b >= B) return;
+    if (b >= B || (skip && skip[b])) return;
     const double* ev = evg + (size_t)b * N;
     double* Z = Zg + (size_t)b * N * N;
     const size_t NN = (size_t)N * N;
@@ -159,11 +160,12 @@ __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* 
 // Z <- Q Z.  One workgroup per matrix; thread t handles column j = t % N with
 // part q = t / N of the row range for the dot products (parts reduced in LDS).
 __global__ void __launch_bounds__(1024) k_backtransform(int B, int N, const double* __restrict__ vg,
-                                                        const double* __restrict__ taug, double* __restrict__ Zg) {
+                                                        const double* __restrict__ taug, double* __restrict__ Zg,
+                                                        const int32_t* __restrict__ skip) {
     __shared__ double v[2048];
     __shared__ double part[1024];
     const int b = blockIdx.x;
-    if (b >= B) return;
+    if (b >= B || (skip && skip[b])) return;
     const int tid = threadIdx.x, NT = blockDim.x;
     const int parts = max(1, NT / N);
     const int j = tid % N, q = tid / N;
@@ -193,12 +195,13 @@ __global__ void __launch_bounds__(1024) k_backtransform(int B, int N, const doub
 }  // namespace fkv
 
 hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,
-                           const double* tau, double* Z, double* scratch, hipStream_t st) {
+                           const double* tau, double* Z, double* scratch, hipStream_t st, const int32_t* skip) {
     if (N > 2048) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fkv::k_tri_evecs, dim3(B), dim3(N <= 256 ? 256 : 1024), 0, st, B, N, d, e, ev, Z, scratch);
+    hipLaunchKernelGGL(fkv::k_tri_evecs, dim3(B), dim3(N <= 256 ? 256 : 1024), 0, st, B, N, d, e, ev, Z, scratch,
+                       skip);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     const int nt = N <= 1024 ? ((1024 / N) * N > 0 ? (1024 / N) * N : N) : 1024;
-    hipLaunchKernelGGL(fkv::k_backtransform, dim3(B), dim3(nt <= 1024 ? nt : 1024), 0, st, B, N, vh, tau, Z);
+    hipLaunchKernelGGL(fkv::k_backtransform, dim3(B), dim3(nt <= 1024 ? nt : 1024), 0, st, B, N, vh, tau, Z, skip);
     return hipGetLastError();
 }

@@ -26,6 +26,7 @@ struct fkgpu_ctx {
     int N = 0;
     int lat_kind = -1, lat_ndim = 0, lat_L = 0;
     int solver = -1;  // forced solver kind (fkgpu_set_solver / fkgpu_autotune), -1: by size
+    struct mixed_bufs* mx = nullptr;  // buffers of fkgpu_eig_logz_mixed (kept between calls)
     double* d_hop = nullptr;
     double* d_hopv = nullptr;  // hopping value of each neighbour slot (N x zmax)
     int32_t* d_nbr = nullptr;
@@ -69,6 +70,9 @@ struct fkgpu_chains {
     int32_t *k_fprop = nullptr, *k_skip = nullptr;
     double *k_ab = nullptr, *k_coef = nullptr, *k_mom = nullptr, *k_lz = nullptr;
     bool dense_ok = true;
+    // mixed-precision mode (FP32 reduction + FP64 refinement)
+    int mixed = 0;
+    struct mixed_bufs* mx = nullptr;
 };
 
 namespace {
@@ -213,10 +217,13 @@ int fkgpu_create(int device, fkgpu_ctx** out) {
     return FKGPU_OK;
 }
 
+static void mixed_free(mixed_bufs* m);
+
 static void ctx_release(fkgpu_ctx* ctx) {
     if (!ctx || --ctx->refs > 0) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    mixed_free(ctx->mx);
     for (auto* b : {&ctx->b_f, &ctx->b_beta, &ctx->b_U, &ctx->b_mu, &ctx->b_ev, &ctx->b_lz, &ctx->b_ce, &ctx->b_cf,
                     &ctx->b_ws, &ctx->b_vh, &ctx->b_tau, &ctx->b_d, &ctx->b_e, &ctx->b_Z, &ctx->b_scr, &ctx->b_m1,
                     &ctx->b_m2, &ctx->b_m3, &ctx->b_m4, &ctx->b_m5, &ctx->b_out, &ctx->b_w, &ctx->b_c1, &ctx->b_c2,
@@ -364,6 +371,149 @@ static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
         FK_HIP(ctx, fk_launch_evecs(batch, N, dd, ee, o.ev, vh, tau, Z, scr, st));
         o.V = Z;
     }
+    return FKGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 blocked reduction (solver kind 5), FP64 tridiagonal
+// eigenvalues and eigenvectors, FP64 Rayleigh-Ritz refinement (fk_mixed.hip),
+// FP64 solve of the configurations the refinement flags.
+// ---------------------------------------------------------------------------
+struct mixed_bufs {
+    int B = 0, N = 0;
+    double *ws = nullptr, *vh = nullptr, *tau = nullptr, *d = nullptr, *e = nullptr, *evh = nullptr, *lzh = nullptr;
+    double *X = nullptr, *scr = nullptr, *Q = nullptr, *ev = nullptr, *lz = nullptr, *ws64 = nullptr;
+    int32_t* fb = nullptr;
+    unsigned long long* nflag = nullptr;
+    size_t ws_stride = 0, ws64_stride = 0;
+    int kind64 = -1;
+};
+
+static void mixed_free(mixed_bufs* m) {
+    if (!m) return;
+    void* ptrs[] = {m->ws, m->vh, m->tau, m->d, m->e, m->evh, m->lzh, m->X, m->scr, m->Q, m->ev, m->lz, m->ws64,
+                    m->fb, m->nflag};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete m;
+}
+
+static hipError_t mixed_alloc(fkgpu_ctx* ctx, int B, int N, mixed_bufs** out) {
+    mixed_bufs* m = new mixed_bufs();
+    m->B = B;
+    m->N = N;
+    m->ws_stride = fk_solver_ws_doubles(5, N);
+    m->kind64 = pick_kind(ctx);
+    m->ws64_stride = fk_solver_needs_ws(m->kind64) ? fk_solver_ws_doubles(m->kind64, N) : 0;
+    const size_t BN = (size_t)B * N, BNN = BN * N;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    chk(dalloc(&m->ws, (size_t)B * m->ws_stride));
+    chk(dalloc(&m->vh, BNN));
+    chk(dalloc(&m->tau, BN));
+    chk(dalloc(&m->d, BN));
+    chk(dalloc(&m->e, BN));
+    chk(dalloc(&m->evh, BN));
+    chk(dalloc(&m->lzh, B));
+    chk(dalloc(&m->X, BNN));
+    chk(dalloc(&m->scr, 3 * BNN));
+    chk(dalloc(&m->Q, BN * fk_mixed_cluster_max()));
+    chk(dalloc(&m->ev, BN));
+    chk(dalloc(&m->lz, B));
+    if (m->ws64_stride) chk(dalloc(&m->ws64, (size_t)B * m->ws64_stride));
+    chk(dalloc(&m->fb, B));
+    chk(dalloc(&m->nflag, 1));
+    if (e == hipSuccess) e = hipMemset(m->nflag, 0, sizeof(unsigned long long));
+    // eigenvector-stage inputs of skipped items must hold finite numbers
+    if (e == hipSuccess) e = hipMemset(m->d, 0, sizeof(double) * BN);
+    if (e == hipSuccess) e = hipMemset(m->e, 0, sizeof(double) * BN);
+    if (e == hipSuccess) e = hipMemset(m->evh, 0, sizeof(double) * BN);
+    if (e != hipSuccess) { mixed_free(m); return e; }
+    *out = m;
+    return hipSuccess;
+}
+
+// Mixed-precision eigenvalues + logZ of B configurations (device f, int32) into m->ev, m->lz.
+static hipError_t mixed_eig_launch(fkgpu_ctx* ctx, mixed_bufs* m, int B, const int32_t* df, const double* beta,
+                                   const double* U, const double* mu, const int32_t* skip, hipStream_t st) {
+    const int N = m->N;
+    const size_t BNN = (size_t)B * N * N;
+    eig_args a{};
+    a.B = B;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = df;
+    a.beta = beta;
+    a.U = U;
+    a.mu_c = mu;
+    a.evals = m->evh;
+    a.logz = m->lzh;
+    a.ws = m->ws;
+    a.ws_stride = m->ws_stride;
+    a.skip = skip;
+    a.vout = m->vh;
+    a.tauout = m->tau;
+    a.d_out = m->d;
+    a.e_out = m->e;
+    hipError_t e = fk_launch_eig(5, a, B, st);
+    if (e != hipSuccess) return e;
+    e = fk_launch_evecs(B, N, m->d, m->e, m->evh, m->vh, m->tau, m->X, m->scr, st, skip);
+    if (e != hipSuccess) return e;
+    e = fk_launch_mixed_refine(B, N, m->X, df, U, mu, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, m->scr,
+                               m->scr + BNN, m->scr + 2 * BNN, m->Q, beta, m->ev, m->lz, m->fb, skip, m->nflag, st);
+    if (e != hipSuccess) return e;
+    // FP64 solve where the refinement could not certify its result
+    eig_args f{};
+    f.B = B;
+    f.N = N;
+    f.hop = ctx->d_hop;
+    f.f = df;
+    f.beta = beta;
+    f.U = U;
+    f.mu_c = mu;
+    f.evals = m->ev;
+    f.logz = m->lz;
+    f.ws = m->ws64;
+    f.ws_stride = m->ws64_stride;
+    f.skip = m->fb;
+    return fk_launch_eig(m->kind64, f, B, st);
+}
+
+int fkgpu_eig_logz_mixed(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                         const double* mu_c, double* evals, double* logZ, int64_t* nfallback) {
+    int r = check_params(ctx, batch, f, beta, U, mu_c);
+    if (r) return r;
+    if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
+    const int N = ctx->N;
+    if (N > fk_solver_nmax(5)) return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
+    hipSetDevice(ctx->device);
+    const size_t BN = (size_t)batch * N;
+    int32_t* df;
+    double *dbeta, *dU, *dmu;
+    if ((r = ensure(ctx, ctx->b_f, BN, &df)) || (r = ensure(ctx, ctx->b_beta, batch, &dbeta)) ||
+        (r = ensure(ctx, ctx->b_U, batch, &dU)) || (r = ensure(ctx, ctx->b_mu, batch, &dmu)))
+        return r;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemcpyAsync(df, f, sizeof(int32_t) * BN, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dbeta, beta, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dU, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(dmu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    if (ctx->mx && (ctx->mx->N != N || ctx->mx->B < batch || ctx->mx->kind64 != pick_kind(ctx))) {
+        mixed_free(ctx->mx);
+        ctx->mx = nullptr;
+    }
+    if (!ctx->mx) FK_HIP(ctx, mixed_alloc(ctx, batch, N, &ctx->mx));
+    mixed_bufs* m = ctx->mx;
+    FK_HIP(ctx, hipMemsetAsync(m->nflag, 0, sizeof(unsigned long long), st));
+    hipError_t e = mixed_eig_launch(ctx, m, batch, df, dbeta, dU, dmu, nullptr, st);
+    unsigned long long nf = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(evals, m->ev, sizeof(double) * BN, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(logZ, m->lz, sizeof(double) * batch, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&nf, m->nflag, sizeof(nf), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hipfail(ctx, e, "mixed-precision eigensolve");
+    if (nfallback) *nfallback = (int64_t)nf;
     return FKGPU_OK;
 }
 
@@ -827,6 +977,7 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
         if (p) hipFree(p);
     if (ch->ev0) hipEventDestroy(ch->ev0);
     if (ch->ev1) hipEventDestroy(ch->ev1);
+    mixed_free(ch->mx);
     fkgpu_ctx* ctx = ch->ctx;
     delete ch;
     ctx_release(ctx);
@@ -993,8 +1144,15 @@ int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W) {
 // One move step of every chain: the fused dense kernel, or propose / stochastic
 // trace / accept for the Chebyshev (KPM) mode.
 static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t st) {
-    if (!ch->kpm) return fk_launch_sweep(ch->kind, a, ch->grid, st);
     fkgpu_ctx* ctx = ch->ctx;
+    if (ch->mixed) {
+        hipError_t e = fk_launch_kpm_propose(a, ch->k_props, ch->k_fprop, ch->k_skip, st);
+        if (e != hipSuccess) return e;
+        e = mixed_eig_launch(ctx, ch->mx, ch->B, ch->k_fprop, ch->beta, ch->U, ch->mu_c, ch->k_skip, st);
+        if (e != hipSuccess) return e;
+        return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->mx->lz, st, ch->mx->ev);
+    }
+    if (!ch->kpm) return fk_launch_sweep(ch->kind, a, ch->grid, st);
     hipError_t e = fk_launch_kpm_propose(a, ch->k_props, ch->k_fprop, ch->k_skip, st);
     if (e != hipSuccess) return e;
     e = fk_launch_kpm(ch->B, ch->N, ch->kR, ch->kM, ch->k_fprop, ch->U, ch->mu_c, ch->k_ab, ctx->d_nbr, ctx->d_z,
@@ -1002,6 +1160,72 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
                       ch->k_lz, st);
     if (e != hipSuccess) return e;
     return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->k_lz, st);
+}
+
+int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    const int B = ch->B, N = ch->N;
+    if (precision != FKGPU_PRECISION_FP64 && precision != FKGPU_PRECISION_MIXED)
+        return fail(ctx, FKGPU_EINVAL, "unknown precision");
+    if (ch->kpm) return fail(ctx, FKGPU_EUNSUPPORTED, "precision applies to the dense eigensolver, not the stochastic trace");
+    if (precision == FKGPU_PRECISION_MIXED && N > fk_solver_nmax(5))
+        return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+    ch->graph_steps = 0;
+    if (!ch->k_props) {
+        hipError_t e = hipSuccess;
+        auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+        chk(dalloc(&ch->k_props, B));
+        chk(dalloc(&ch->k_fprop, (size_t)B * N));
+        chk(dalloc(&ch->k_skip, B));
+        if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "proposal buffers");
+    }
+    if (precision == FKGPU_PRECISION_MIXED && !ch->mx) FK_HIP(ctx, mixed_alloc(ctx, B, N, &ch->mx));
+    // logZ and spectrum of the current configurations at the requested precision
+    std::vector<uint8_t> f8((size_t)B * N);
+    FK_HIP(ctx, hipMemcpy(f8.data(), ch->f, f8.size(), hipMemcpyDeviceToHost));
+    std::vector<int32_t> f32(f8.begin(), f8.end());
+    FK_HIP(ctx, hipMemcpyAsync(ch->k_fprop, f32.data(), sizeof(int32_t) * f32.size(), hipMemcpyHostToDevice, st));
+    if (precision == FKGPU_PRECISION_MIXED) {
+        FK_HIP(ctx, mixed_eig_launch(ctx, ch->mx, B, ch->k_fprop, ch->beta, ch->U, ch->mu_c, nullptr, st));
+        FK_HIP(ctx, hipMemcpyAsync(ch->logz, ch->mx->lz, sizeof(double) * B, hipMemcpyDeviceToDevice, st));
+        FK_HIP(ctx, hipMemcpyAsync(ch->spectrum, ch->mx->ev, sizeof(double) * B * N, hipMemcpyDeviceToDevice, st));
+    } else {
+        eig_args a{};
+        a.B = B;
+        a.N = N;
+        a.hop = ctx->d_hop;
+        a.f = ch->k_fprop;
+        a.beta = ch->beta;
+        a.U = ch->U;
+        a.mu_c = ch->mu_c;
+        a.evals = ch->spectrum;
+        a.logz = ch->logz;
+        a.ws = ch->ws;
+        a.ws_stride = ch->ws_stride;
+        FK_HIP(ctx, fk_launch_eig(ch->kind, a, ch->grid, st));
+    }
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    ch->mixed = precision == FKGPU_PRECISION_MIXED;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_fallbacks(const fkgpu_chains* ch, int64_t* n) {
+    if (!ch || !n) return FKGPU_EINVAL;
+    unsigned long long v = 0;
+    if (ch->mx) {
+        hipSetDevice(ch->ctx->device);
+        if (hipStreamSynchronize(ch->ctx->stream) != hipSuccess ||
+            hipMemcpy(&v, ch->mx->nflag, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
+            return FKGPU_EHIP;
+    }
+    *n = (int64_t)v;
+    return FKGPU_OK;
 }
 
 int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jackson) {
@@ -1069,6 +1293,7 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
                               ctx->d_hopv, seed, 1, nullptr, ch->k_coef, jackson, ch->k_mom, nullptr, ch->logz, st));
     FK_HIP(ctx, hipStreamSynchronize(st));
     ch->kpm = 1;
+    ch->mixed = 0;
     ch->kR = R;
     ch->kM = M;
     ch->kseed = seed;

@@ -17,6 +17,7 @@ OK, EINVAL, ENOLATTICE, EUNSUPPORTED, EHIP, ENOMEM = 0, -1, -2, -3, -4, -5
 LATTICE_HYPERCUBIC, LATTICE_TRIANGULAR = 0, 1
 MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
 SOLVER_AUTO, SOLVER_VARIANTS = -1, 5
+PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -89,6 +90,9 @@ def lib():
         L.fkgpu_dos.argtypes = [P, C.c_int, _d, C.c_int, _d, C.c_double, _d]
         L.fkgpu_adjoint.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _d, _d]
         L.fkgpu_cheb_logz.argtypes = [P, _i32, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _d, _d, _d]
+        L.fkgpu_eig_logz_mixed.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _i64]
+        L.fkgpu_chains_set_precision.argtypes = [P, C.c_int]
+        L.fkgpu_chains_fallbacks.argtypes = [P, _i64]
         L.fkgpu_set_solver.argtypes = [P, C.c_int]
         L.fkgpu_get_solver.argtypes = [P]
         L.fkgpu_autotune.argtypes = [P, C.c_int, _d]
@@ -273,6 +277,18 @@ def _autotune(self, batch=256):
     return r, ms
 
 
+def _eig_logz_mixed(self, f, beta, U, mu_c):
+    """Mixed-precision calc_ed (eigenvalues): returns dict(evals, logZ, nfallback)."""
+    f, B, b, u, m = _batch(f, beta, U, mu_c)
+    ev = np.zeros((B, self.N))
+    lz = np.zeros(B)
+    nfb = np.zeros(1, np.int64)
+    self._check(lib().fkgpu_eig_logz_mixed(self._h, B, _p(f, _i32), _p(b, _d), _p(u, _d), _p(m, _d), _p(ev, _d),
+                                           _p(lz, _d), _p(nfb, _i64)))
+    return dict(evals=ev, logZ=lz, nfallback=int(nfb[0]))
+
+
+Context.eig_logz_mixed = _eig_logz_mixed
 Context.set_solver = _set_solver
 Context.get_solver = _get_solver
 Context.autotune = _autotune
@@ -340,6 +356,14 @@ class Chains:
                    _p(tr["logw"], _d), _p(tr["logz"], _d), _p(tr["acc"], _i8))
         self.ctx._check(lib().fkgpu_sweep(self._h, nsteps, C.byref(t)))
         return {k: v.reshape(nsteps, self.B) for k, v in tr.items()}
+
+    def set_precision(self, precision):
+        self.ctx._check(lib().fkgpu_chains_set_precision(self._h, precision))
+
+    def fallbacks(self):
+        n = np.zeros(1, np.int64)
+        self.ctx._check(lib().fkgpu_chains_fallbacks(self._h, _p(n, _i64)))
+        return int(n[0])
 
     def sweep_graph(self, nsteps):
         self.ctx._check(lib().fkgpu_sweep_graph(self._h, nsteps))

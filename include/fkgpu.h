@@ -78,6 +78,15 @@ int fkgpu_get_solver(const fkgpu_ctx* ctx); /* the variant used for the current 
  * the time per launch, or -1 where the variant does not fit. */
 int fkgpu_autotune(fkgpu_ctx* ctx, int batch, double* ms);
 
+/* Mixed-precision calc_ed (eigenvalues only; no reference counterpart, the reference
+ * is FP64 throughout): FP32 Householder reduction with MFMA trailing updates, FP64
+ * tridiagonal eigenvalues and eigenvectors, FP64 Rayleigh-Ritz refinement of the
+ * eigenvalues (second-order perturbation with near-degenerate groups diagonalised
+ * exactly), compensated FP64 log-sum.  Configurations the refinement cannot certify
+ * are re-solved in FP64; *nfallback (nullable) counts them.  N <= 1024. */
+int fkgpu_eig_logz_mixed(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
+                         const double* mu_c, double* evals, double* logZ, int64_t* nfallback);
+
 /* configuration_t::calc_hamiltonian() + calc_ed(bool calc_evecs) for `batch`
  * configurations (src/configuration.cpp:79-91, 208-246).  Outputs per config:
  * evals[N] (ed_cache::cached_spectrum), logZ (ed_cache::logZ), cached_exp[N],
@@ -163,6 +172,14 @@ void fkgpu_chains_destroy(fkgpu_chains* ch);
 int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, const double* weights);
 /* f-f interaction W of 1-D lattices (configuration_t::calc_ff_energy, src/configuration.cpp:59-77). */
 int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W);
+/* Precision of the chains' dense eigensolve: FKGPU_PRECISION_FP64 (default) or
+ * FKGPU_PRECISION_MIXED (the fkgpu_eig_logz_mixed pipeline in three launches per move
+ * step: proposal, solve, decision).  Recomputes logZ of the current configurations. */
+#define FKGPU_PRECISION_FP64 0
+#define FKGPU_PRECISION_MIXED 1
+int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision);
+/* Configurations the mixed-precision refinement handed to the FP64 solver so far. */
+int fkgpu_chains_fallbacks(const fkgpu_chains* ch, int64_t* n);
 /* Switch the chain set to the stochastic-trace logZ (the Chebyshev moves of
  * src/moves_chebyshev.cpp with the kernel-polynomial estimate instead of the exact
  * trace): R probe vectors per chain (fixed for the whole run, so old and new

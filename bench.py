@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--plain-moves", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=8, help="mixed-precision timing steps (0: skip)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); chains are independent, "
@@ -104,6 +106,13 @@ def main():
         plain = {"moves_per_s": round(plain_rate, 2), "ms_per_move": round((tp1 - tp0) * 1e3 / args.plain_moves, 4),
                  "batched_over_plain": round(value / plain_rate, 1)}
 
+    mixed = None
+    if args.mixed_steps > 0:
+        mixed = mixed_block(fk, ctx, f, beta, U, seeds, args.mixed_steps, plain)
+    c3 = None
+    if args.mixed_steps > 0 and not args.no_c3:
+        c3 = c3_block(fk, max(2, args.mixed_steps // 2))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, args.cpu_seconds, ch, ctx, f, beta, U, seeds)
@@ -128,11 +137,89 @@ def main():
         "accept_rate": round(acc / moves, 4),
         "roofline": roofline,
         "plain": plain,
+        "mixed": mixed,
+        "mixed_cubic3d_L8": c3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     ch.close()
     ctx.close()
+
+
+def compare_traces(t64, tmx, beta):
+    """Per-move |dOmega_mixed - dOmega_FP64| while the two chains agree, and the accept
+    mismatches (a chain is compared up to its first differing decision)."""
+    steps, B = t64["acc"].shape
+    diffs, mism, ties = [], 0, 0
+    for c in range(B):
+        for s in range(steps):
+            if t64["acc"][s, c] != tmx["acc"][s, c]:
+                mism += 1
+                if abs(np.log(t64["u"][s, c]) - t64["logw"][s, c]) < 1e-9:
+                    ties += 1
+                break
+            if np.isfinite(t64["logw"][s, c]) and t64["logz"][s, c] != 0.0:
+                diffs.append(abs(tmx["logw"][s, c] - t64["logw"][s, c]) / beta[c])
+    d = np.array(diffs) if diffs else np.zeros(1)
+    return {"moves_compared": len(diffs), "dOmega_max_abs_diff_vs_fp64": float(d.max()),
+            "dOmega_mean_abs_diff_vs_fp64": float(d.mean()), "accept_mismatches": mism,
+            "accept_mismatches_at_near_ties": ties}
+
+
+def run_chains(fk, ctx, f, beta, U, seeds, precision, steps, trace=False, warmup=2):
+    ch = fk.Chains(ctx, f, beta, U, U / 2, U / 2, seeds)
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    ch.set_precision(precision)
+    if trace:
+        tr = ch.sweep(steps, trace=True)
+        ch.close()
+        return tr
+    if warmup:
+        ch.sweep_graph(warmup)
+    ch.sweep_graph(steps)
+    ms = ch.last_ms() / steps
+    nfb = ch.fallbacks()
+    ch.close()
+    return ms, nfb
+
+
+def mixed_block(fk, ctx, f, beta, U, seeds, steps, plain):
+    """1024 mixed-precision chains (FP32 MFMA reduction + FP64 refinement) on the same
+    workload: moves/s, and dOmega / decisions against FP64 chains on the same seeds."""
+    B = len(beta)
+    ms, nfb = run_chains(fk, ctx, f, beta, U, seeds, fk.PRECISION_MIXED, steps)
+    t64 = run_chains(fk, ctx, f, beta, U, seeds, fk.PRECISION_FP64, steps, trace=True)
+    tmx = run_chains(fk, ctx, f, beta, U, seeds, fk.PRECISION_MIXED, steps, trace=True)
+    out = {"moves_per_s": round(B / ms * 1e3, 2), "ms_per_step": round(ms, 4), "steps": steps,
+           "fp64_fallbacks": nfb}
+    if plain:
+        out["mixed_over_plain"] = round(out["moves_per_s"] / plain["moves_per_s"], 1)
+    out.update(compare_traces(t64, tmx, beta))
+    return out
+
+
+def c3_block(fk, steps):
+    """cubic3d L=8 (N=512), 256 chains, U in {2,5} x beta in {2,10}, filling ~ U[0.3,0.7]:
+    FP64 (blocked MFMA solver) and mixed-precision chains."""
+    L, B = 8, 256
+    N = L ** 3
+    rng = np.random.default_rng(31)
+    f = (rng.random((B, N)) < rng.uniform(0.3, 0.7, (B, 1))).astype(np.int32)
+    c = np.arange(B)
+    U = np.where((c // 64) % 2 == 0, 2.0, 5.0)
+    beta = np.where(c // 128 == 0, 2.0, 10.0)
+    seeds = (777 + c).astype(np.uint32)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 3, L, 1.0, 0.0)
+    ms64, _ = run_chains(fk, ctx, f, beta, U, seeds, fk.PRECISION_FP64, steps)
+    msmx, nfb = run_chains(fk, ctx, f, beta, U, seeds, fk.PRECISION_MIXED, steps)
+    t64 = run_chains(fk, ctx, f, beta, U, seeds, fk.PRECISION_FP64, steps, trace=True)
+    tmx = run_chains(fk, ctx, f, beta, U, seeds, fk.PRECISION_MIXED, steps, trace=True)
+    out = {"N": N, "chains": B, "steps": steps, "fp64_moves_per_s": round(B / ms64 * 1e3, 2),
+           "mixed_moves_per_s": round(B / msmx * 1e3, 2), "fp64_fallbacks": nfb,
+           "fp64_solver_variant": ctx.get_solver()}
+    out.update(compare_traces(t64, tmx, beta))
+    ctx.close()
+    return out
 
 
 def cpu_baseline(L, seconds, ch, ctx, f, beta, U, seeds):

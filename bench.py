@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+KERNEL_NAMES = {0: "fkk::k_sweep<reg_solver<16,4>>", 1: "fkk::k_sweep<reg_solver<16,8>>",
+                2: "fkk::k_sweep<reg_solver<32,8>>", 3: "fkk::k_sweep<glob_solver>",
+                4: "fkk::k_sweep<blocked_solver<double,512>>", 5: "fkk::k_sweep<wrow_solver>"}
 
 
 def workload(L=16, B=1024, seed=20240601):
@@ -87,7 +90,7 @@ def main():
     achieved = flop / (kern_ms / 1e3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
-                "kernel": "fkk::k_sweep<reg_solver<32,8>>", "kernel_ms_per_launch": round(kern_ms / args.steps, 4),
+                "kernel": KERNEL_NAMES.get(ctx.get_solver(), "k_sweep"), "kernel_ms_per_launch": round(kern_ms / args.steps, 4),
                 "flop_per_launch": flop / args.steps, "eigensolves_per_launch": solves / args.steps}
 
     # plain single-chain drop-in: one calc_hamiltonian+calc_ed per move, host buffers

@@ -15,6 +15,7 @@
 #include "fk_internal.h"
 #include "fk_rng.h"
 #include "fk_solve.h"
+#include "fk_wrow.h"
 
 namespace fkk {
 
@@ -431,16 +432,18 @@ static hipError_t launch_sweep(const sweep_args& a, int grid, hipStream_t st) {
 
 using namespace fkk;
 
-// Solver kinds: 0-2 register-resident one-stage (N <= 64 / 128 / 256), 3 unblocked
-// HBM workspace (N <= 2048), 4 blocked FP64 with MFMA trailing updates (N <= 1024),
-// 5 blocked FP32 (first stage of the mixed-precision path, N <= 1024).
+// Solver kinds: 0-2 register-resident one-stage, block-cyclic P x P grid (N <= 64 /
+// 128 / 256), 3 unblocked HBM workspace (N <= 2048), 4 blocked FP64 with MFMA
+// trailing updates (N <= 1024), 5 register-resident one-stage, wave-row layout
+// (N <= 256); 9 blocked FP32 (first stage of the mixed-precision path, N <= 1024).
+using fkw::wrow_solver;
 typedef blocked_solver<double, 512> blk64;
 typedef blocked_solver<float, 512> blk32;
 
 int fk_solver_kind(int N) {
     if (N <= 64) return 0;
     if (N <= 128) return 1;
-    if (N <= 256) return 2;
+    if (N <= 256) return 5;
     if (N <= blk64::NMAX) return 4;
     if (N <= glob_solver::NMAX) return 3;
     return -1;
@@ -452,7 +455,8 @@ int fk_solver_nmax(int kind) {
         case 2: return reg_solver<32, 8>::NMAX;
         case 3: return glob_solver::NMAX;
         case 4: return blk64::NMAX;
-        case 5: return blk32::NMAX;
+        case 5: return wrow_solver::NMAX;
+        case 9: return blk32::NMAX;
     }
     return 0;
 }
@@ -463,16 +467,17 @@ int fk_solver_threads(int kind) {
         case 2: return reg_solver<32, 8>::NT;
         case 3: return glob_solver::NT;
         case 4: return blk64::NT;
-        case 5: return blk32::NT;
+        case 5: return wrow_solver::NT;
+        case 9: return blk32::NT;
     }
     return 0;
 }
-int fk_solver_needs_ws(int kind) { return kind >= 3; }
+int fk_solver_needs_ws(int kind) { return kind == 3 || kind == 4 || kind == 9; }
 size_t fk_solver_ws_doubles(int kind, int N) {
     switch (kind) {
         case 3: return (size_t)N * N;
         case 4: return blk64::ws_doubles(N);
-        case 5: return blk32::ws_doubles(N);
+        case 9: return blk32::ws_doubles(N);
     }
     return 0;
 }
@@ -484,7 +489,8 @@ hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) 
         case 2: return launch_eig<reg_solver<32, 8>>(a, grid, st);
         case 3: return launch_eig<glob_solver>(a, grid, st);
         case 4: return launch_eig<blk64>(a, grid, st);
-        case 5: return launch_eig<blk32>(a, grid, st);
+        case 5: return launch_eig<wrow_solver>(a, grid, st);
+        case 9: return launch_eig<blk32>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }
@@ -506,6 +512,7 @@ hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t 
         case 2: return launch_sweep<reg_solver<32, 8>>(a, grid, st);
         case 3: return launch_sweep<glob_solver>(a, grid, st);
         case 4: return launch_sweep<blk64>(a, grid, st);
+        case 5: return launch_sweep<wrow_solver>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }

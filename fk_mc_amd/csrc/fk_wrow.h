@@ -1,0 +1,286 @@
+// fk_wrow.h -- register-resident Householder tridiagonalisation, "wave-row" layout,
+// for N <= 256: one workgroup of 16 waves per matrix.
+//
+// Wave w holds the rows i = w + 16a (a = 0..15), lane l the columns j = l + 64b
+// (b = 0..3).  Thread (w, l) keeps A[w + 16a][l + 64b] for b <= a/4 in VGPRs
+// (40 doubles): every 64 x 64 diagonal block is stored in full, the blocks below
+// it once.  For a Householder step (Golub & Van Loan 8.3.1):
+//   * a lane's column values v_j, w_j (j = l + 64b) are its own registers, the
+//     row values v_i, w_i (i = w + 16a, uniform over the wave) are read from the
+//     lane that holds them with v_readlane into scalar registers;
+//   * the row sums of p = A v are reduced across the 64 lanes by a
+//     reduce-scatter on v_permlane32_swap / v_permlane16_swap / DPP (no LDS); the
+//     column sums of the blocks below the diagonal go through LDS (16 partials
+//     per column);
+//   * LDS traffic per step is the column x_k, p and the column partials
+//     (~100 KB), against ~0.5 MB for a block-cyclic grid that broadcasts v and w
+//     through LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fk_solve.h"
+
+namespace fkw {
+
+using namespace fks;
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    int2 w = *reinterpret_cast<int2*>(&v);
+    w.x = __builtin_amdgcn_readlane(w.x, lane);
+    w.y = __builtin_amdgcn_readlane(w.y, lane);
+    return *reinterpret_cast<double*>(&w);
+}
+__device__ __forceinline__ double2 pl32_swap(double x, double y) {
+    int2 a = *reinterpret_cast<int2*>(&x), b = *reinterpret_cast<int2*>(&y);
+    auto lo = __builtin_amdgcn_permlane32_swap(a.x, b.x, false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap(a.y, b.y, false, false);
+    int2 nx = make_int2(lo[0], hi[0]), ny = make_int2(lo[1], hi[1]);
+    return make_double2(*reinterpret_cast<double*>(&nx), *reinterpret_cast<double*>(&ny));
+}
+__device__ __forceinline__ double2 pl16_swap(double x, double y) {
+    int2 a = *reinterpret_cast<int2*>(&x), b = *reinterpret_cast<int2*>(&y);
+    auto lo = __builtin_amdgcn_permlane16_swap(a.x, b.x, false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap(a.y, b.y, false, false);
+    int2 nx = make_int2(lo[0], hi[0]), ny = make_int2(lo[1], hi[1]);
+    return make_double2(*reinterpret_cast<double*>(&nx), *reinterpret_cast<double*>(&ny));
+}
+constexpr int DPP_ROR8 = 0x128;  // row_ror:8, lane i <- lane (i + 8) mod 16 within a 16-lane row
+
+// Reduce-scatter of 8 values per lane over the 64 lanes: afterwards every lane
+// holds the full sum of value number 4*bit5 + 2*bit4 + bit3 of its lane id.
+__device__ __forceinline__ double reduce8(double (&r)[8]) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    // lanes l and l^32: v_permlane32_swap(x, y) leaves x = [x.lo, y.lo], y = [x.hi, y.hi]
+    double s[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const double2 t = pl32_swap(r[a], r[a + 4]);
+        s[a] = t.x + t.y;
+    }
+    // lanes l and l^16: v_permlane16_swap exchanges the odd rows of x with the even rows of y
+    double t2[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const double2 t = pl16_swap(s[a], s[a + 2]);
+        t2[a] = t.x + t.y;
+    }
+    // lanes l and l^8 (row_ror:8 within 16-lane rows)
+    const bool hi8 = (lane & 8) != 0;
+    const double keep = hi8 ? t2[1] : t2[0];
+    const double send = hi8 ? t2[0] : t2[1];
+    double u = keep + dpp_f64<DPP_ROR8>(send);
+    // the 8 lanes sharing bits 5..3 hold partial sums of the same value: all-reduce them
+    u += dpp_f64<DPP_XOR1>(u);
+    u += dpp_f64<DPP_XOR2>(u);
+    u += dpp_f64<DPP_HALF_MIRROR>(u);
+    return u;
+}
+
+struct wrow_solver {
+    static constexpr int NT = 1024;
+    static constexpr int NWV = NT / WAVE;  // 16 waves = row residues
+    static constexpr int NP = 256;
+    static constexpr int NMAX = NP;
+    static constexpr int NA = 16;          // rows per thread
+    static constexpr int NBC = 4;          // column blocks per thread
+    static constexpr int NS = 40;          // slots per thread
+    struct eig_area {
+        double de[2 * NP];
+        double ev[NP];
+        int cnt[NT + 2];
+        double scratch[3 * NWV + 8];
+    };
+    struct smem {
+        double x[2][NP];
+        double rs[NP];
+        double p[NP];
+        double d[NP], e[NP];
+        double red[NWV + 8];
+        union {
+            double cpart[NWV][NP];
+            eig_area ea;
+        } u;
+    };
+    __host__ __device__ static constexpr int sl(int a, int b) {
+        return a < 4 ? a : (a < 8 ? 4 + 2 * (a - 4) + b : (a < 12 ? 12 + 3 * (a - 8) + b : 24 + 4 * (a - 12) + b));
+    }
+    __device__ static const double* dptr(const smem& sm) { return sm.d; }
+    __device__ static const double* eptr(const smem& sm) { return sm.e; }
+    __device__ static double* evptr(smem& sm) { return sm.u.ea.ev; }
+
+    __device__ __forceinline__ static void emit(const double (&A)[NS], smem& sm, double* xbuf, int kk, int N, int w,
+                                                int l) {
+        if (l != (kk & 63)) return;
+        const int bb = kk >> 6;
+#pragma unroll
+        for (int b = 0; b < NBC; ++b) {
+            if (b != bb) continue;
+#pragma unroll
+            for (int a = 4 * b; a < NA; ++a) {
+                const int i = w + 16 * a;
+                if (i > kk && i < N) xbuf[i] = A[sl(a, b)];
+                if (i == kk) sm.d[kk] = A[sl(a, b)];
+            }
+        }
+    }
+
+    __device__ __forceinline__ static void tridiagonalize(double (&A)[NS], smem& sm, int N, double* __restrict__ vout,
+                                                          double* __restrict__ tauout) {
+        const int tid = threadIdx.x;
+        const int l = tid & (WAVE - 1);
+        const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+        for (int i = tid; i < NP; i += NT) { sm.x[0][i] = 0.0; sm.x[1][i] = 0.0; sm.d[i] = 0.0; sm.e[i] = 0.0; }
+        __syncthreads();
+        emit(A, sm, sm.x[0], 0, N, w, l);
+        __syncthreads();
+        for (int k = 0; k + 2 < N; ++k) {
+            const double* xk = sm.x[k & 1];
+            double* xn = sm.x[(k + 1) & 1];
+            // reflector from x = A[k+1:N, k] (every wave redundantly)
+            const double alpha = xk[k + 1];
+            double part = 0.0;
+            for (int i = k + 2 + l; i < N; i += WAVE) part += xk[i] * xk[i];
+            const double xn2 = wave_sum(part);
+            double tau = 0.0, beta = alpha, scal = 0.0;
+            if (xn2 != 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                tau = (beta - alpha) * rcp_refined(beta);
+                scal = rcp_refined(alpha - beta);
+            }
+            if (tid == 0) sm.e[k] = beta;
+            if (vout) {
+                for (int i = tid; i < N; i += NT)
+                    vout[(size_t)k * N + i] = (i == k + 1) ? 1.0 : ((i > k + 1) ? xk[i] * scal : 0.0);
+                if (tid == 0) tauout[k] = tau;
+            }
+            tau = uniform(tau);
+            scal = uniform(scal);
+            const int bmin = k >> 6;  // column blocks below bmin are finished
+            double vc[NBC];
+#pragma unroll
+            for (int b = 0; b < NBC; ++b) {
+                const int j = l + 64 * b;
+                vc[b] = (j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? xk[j] * scal : 0.0);
+            }
+            if (tau != 0.0) {
+                // row sums over the lanes' columns (two groups of 8 rows)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    double rp[8];
+#pragma unroll
+                    for (int aa = 0; aa < 8; ++aa) {
+                        const int a = 8 * h + aa;
+                        double s = 0.0;
+                        if (w + 16 * a > k) {
+#pragma unroll
+                            for (int b = 0; b <= (a >> 2); ++b)
+                                if (b >= bmin) s = fma(A[sl(a, b)], vc[b], s);
+                        }
+                        rp[aa] = s;
+                    }
+                    const double rsum = reduce8(rp);
+                    if ((l & 7) == 0) {
+                        const int a = 8 * h + 4 * ((l >> 5) & 1) + 2 * ((l >> 4) & 1) + ((l >> 3) & 1);
+                        sm.rs[w + 16 * a] = rsum;
+                    }
+                }
+                // column sums of the blocks strictly below the diagonal blocks
+#pragma unroll
+                for (int b = 0; b < NBC; ++b) {
+                    double s = 0.0;
+                    if (b >= bmin) {
+#pragma unroll
+                        for (int a = 4 * (b + 1); a < NA; ++a) {
+                            if (w + 16 * a > k) {
+                                const double vr = readlane_d(vc[a >> 2], (w + 16 * a) & 63);
+                                s = fma(A[sl(a, b)], vr, s);
+                            }
+                        }
+                    }
+                    sm.u.cpart[w][l + 64 * b] = s;
+                }
+            }
+            __syncthreads();
+            if (tau != 0.0 && tid < NP) {
+                const int i = tid;
+                double pi = 0.0;
+                if (i > k && i < N) {
+                    double s = sm.rs[i];
+#pragma unroll
+                    for (int ww = 0; ww < NWV; ++ww) s += sm.u.cpart[ww][i];
+                    pi = tau * s;
+                }
+                sm.p[i] = pi;
+                const double vi = (i == k + 1) ? 1.0 : ((i > k + 1 && i < N) ? xk[i] * scal : 0.0);
+                const double pv = wave_sum(pi * vi);
+                if (l == 0) sm.red[tid / WAVE] = pv;
+            }
+            __syncthreads();
+            if (tau != 0.0) {
+                double pvt = 0.0;  // fixed order: bitwise reproducible
+#pragma unroll
+                for (int q = 0; q < NP / WAVE; ++q) pvt += sm.red[q];
+                const double coef = uniform(0.5 * tau * pvt);
+                double wc[NBC];
+#pragma unroll
+                for (int b = 0; b < NBC; ++b) wc[b] = sm.p[l + 64 * b] - coef * vc[b];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    if (w + 16 * a <= k) continue;
+                    const int src = (w + 16 * a) & 63;
+                    const double vra = readlane_d(vc[a >> 2], src);
+                    const double wra = readlane_d(wc[a >> 2], src);
+#pragma unroll
+                    for (int b = 0; b <= (a >> 2); ++b)
+                        if (b >= bmin) A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
+                }
+            }
+            emit(A, sm, xn, k + 1, N, w, l);
+            __syncthreads();
+        }
+        if (N >= 2) {
+            if (tid == 0) sm.e[N - 2] = sm.x[(N - 2) & 1][N - 1];
+            const int kk = N - 1;
+            if (w == (kk & 15) && l == (kk & 63)) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                    if (a == (kk >> 4)) sm.d[kk] = A[sl(a, a >> 2)];
+            }
+            if (vout) {
+                for (int i = tid; i < N; i += NT) {
+                    vout[(size_t)(N - 2) * N + i] = 0.0;
+                    vout[(size_t)(N - 1) * N + i] = 0.0;
+                }
+                if (tid == 0) { tauout[N - 2] = 0.0; tauout[N - 1] = 0.0; }
+            }
+        }
+        __syncthreads();
+    }
+
+    __device__ __forceinline__ static double solve(smem& sm, const double* __restrict__ hop, const uint8_t* f_lds,
+                                                   int N, double beta, double U, double mu_c, double* ws,
+                                                   double* vout, double* tauout, double* cexp, double* cfermi) {
+        const int tid = threadIdx.x;
+        const int l = tid & (WAVE - 1);
+        const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+        double A[NS];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b <= (a >> 2); ++b) {
+                const int i = w + 16 * a, j = l + 64 * b;
+                double v = 0.0;
+                if (i < N && j < N) {
+                    v = hop[(size_t)i * N + j];
+                    if (i == j) v += -mu_c + U * (double)f_lds[i];
+                }
+                A[sl(a, b)] = v;
+            }
+        tridiagonalize(A, sm, N, vout, tauout);
+        tridiag_eigvals<NT>(sm.d, sm.e, sm.u.ea.de, sm.u.ea.cnt, N, sm.u.ea.ev, sm.u.ea.scratch);
+        return logz_block<NT>(sm.u.ea.ev, N, beta, cexp, cfermi, sm.u.ea.scratch);
+    }
+};
+
+}  // namespace fkw

@@ -117,7 +117,7 @@ bool finite_all(const double* p, int n) {
 // and the kernel is launched repeatedly over slices of the batch.
 // Solver kind for the context's lattice: the forced / autotuned one when it fits N.
 int pick_kind(const fkgpu_ctx* ctx) {
-    if (ctx->solver >= 0 && ctx->solver != 5 && ctx->N <= fk_solver_nmax(ctx->solver)) return ctx->solver;
+    if (ctx->solver >= 0 && ctx->N <= fk_solver_nmax(ctx->solver)) return ctx->solver;
     return fk_solver_kind(ctx->N);
 }
 
@@ -402,7 +402,7 @@ static hipError_t mixed_alloc(fkgpu_ctx* ctx, int B, int N, mixed_bufs** out) {
     mixed_bufs* m = new mixed_bufs();
     m->B = B;
     m->N = N;
-    m->ws_stride = fk_solver_ws_doubles(5, N);
+    m->ws_stride = fk_solver_ws_doubles(9, N);
     m->kind64 = pick_kind(ctx);
     m->ws64_stride = fk_solver_needs_ws(m->kind64) ? fk_solver_ws_doubles(m->kind64, N) : 0;
     const size_t BN = (size_t)B * N, BNN = BN * N;
@@ -455,7 +455,7 @@ static hipError_t mixed_eig_launch(fkgpu_ctx* ctx, mixed_bufs* m, int B, const i
     a.tauout = m->tau;
     a.d_out = m->d;
     a.e_out = m->e;
-    hipError_t e = fk_launch_eig(5, a, B, st);
+    hipError_t e = fk_launch_eig(9, a, B, st);
     if (e != hipSuccess) return e;
     e = fk_launch_evecs(B, N, m->d, m->e, m->evh, m->vh, m->tau, m->X, m->scr, st, skip);
     if (e != hipSuccess) return e;
@@ -485,7 +485,7 @@ int fkgpu_eig_logz_mixed(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
     if (r) return r;
     if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
     const int N = ctx->N;
-    if (N > fk_solver_nmax(5)) return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
+    if (N > fk_solver_nmax(9)) return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
     hipSetDevice(ctx->device);
     const size_t BN = (size_t)batch * N;
     int32_t* df;
@@ -1169,7 +1169,7 @@ int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
     if (precision != FKGPU_PRECISION_FP64 && precision != FKGPU_PRECISION_MIXED)
         return fail(ctx, FKGPU_EINVAL, "unknown precision");
     if (ch->kpm) return fail(ctx, FKGPU_EUNSUPPORTED, "precision applies to the dense eigensolver, not the stochastic trace");
-    if (precision == FKGPU_PRECISION_MIXED && N > fk_solver_nmax(5))
+    if (precision == FKGPU_PRECISION_MIXED && N > fk_solver_nmax(9))
         return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;

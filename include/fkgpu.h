@@ -67,9 +67,11 @@ int fkgpu_msize(const fkgpu_ctx* ctx);
  *   0, 1, 2                 register-resident one-stage, P x P threads holding NB x NB
  *                           blocks: (P, NB) = (16, 4) N <= 64, (16, 8) N <= 128, (32, 8) N <= 256
  *   3                       unblocked, matrix in an HBM workspace, N <= 2048
- *   4                       blocked (panel + MFMA rank-2nb trailing update), HBM workspace, N <= 1024 */
+ *   4                       blocked (panel + MFMA rank-2nb trailing update), HBM workspace, N <= 1024
+ *   5                       register-resident one-stage, 16 waves x 64 lanes (wave = row residue,
+ *                           lane = column residue; row values by v_readlane), N <= 256 */
 #define FKGPU_SOLVER_AUTO (-1)
-#define FKGPU_SOLVER_VARIANTS 5
+#define FKGPU_SOLVER_VARIANTS 6
 int fkgpu_set_solver(fkgpu_ctx* ctx, int variant);
 int fkgpu_get_solver(const fkgpu_ctx* ctx); /* the variant used for the current lattice */
 /* Autotuner: times every variant that fits the lattice's N on `batch` synthetic

@@ -163,20 +163,21 @@ struct wrow_solver {
                 const int j = l + 64 * b;
                 vc[b] = (j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? xk[j] * scal : 0.0);
             }
+            // Slots of finished rows / columns take part with v = w = 0 there (p is zero
+            // on finished indices), which leaves them unchanged; work is skipped only per
+            // 64-wide block, so every wave runs the same instruction stream.
             if (tau != 0.0) {
                 // row sums over the lanes' columns (two groups of 8 rows)
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
+                    if (128 * h + 127 <= k) continue;
                     double rp[8];
 #pragma unroll
                     for (int aa = 0; aa < 8; ++aa) {
                         const int a = 8 * h + aa;
                         double s = 0.0;
-                        if (w + 16 * a > k) {
 #pragma unroll
-                            for (int b = 0; b <= (a >> 2); ++b)
-                                if (b >= bmin) s = fma(A[sl(a, b)], vc[b], s);
-                        }
+                        for (int b = 0; b <= (a >> 2); ++b) s = fma(A[sl(a, b)], vc[b], s);
                         rp[aa] = s;
                     }
                     const double rsum = reduce8(rp);
@@ -192,48 +193,47 @@ struct wrow_solver {
                     if (b >= bmin) {
 #pragma unroll
                         for (int a = 4 * (b + 1); a < NA; ++a) {
-                            if (w + 16 * a > k) {
-                                const double vr = readlane_d(vc[a >> 2], (w + 16 * a) & 63);
-                                s = fma(A[sl(a, b)], vr, s);
-                            }
+                            const double vr = readlane_d(vc[a >> 2], (w + 16 * a) & 63);
+                            s = fma(A[sl(a, b)], vr, s);
                         }
                     }
                     sm.u.cpart[w][l + 64 * b] = s;
                 }
             }
             __syncthreads();
-            if (tau != 0.0 && tid < NP) {
-                const int i = tid;
-                double pi = 0.0;
-                if (i > k && i < N) {
-                    double s = sm.rs[i];
+            if (tau != 0.0) {
+                // p_i = tau (row sum + 16 column partials): four lanes per row
+                const int i = tid >> 2, q4 = tid & 3;
+                double s = (q4 == 0) ? sm.rs[i] : 0.0;
 #pragma unroll
-                    for (int ww = 0; ww < NWV; ++ww) s += sm.u.cpart[ww][i];
-                    pi = tau * s;
-                }
-                sm.p[i] = pi;
+                for (int ww = 0; ww < 4; ++ww) s += sm.u.cpart[4 * q4 + ww][i];
+                s = quad_sum(s);
+                const double pi = (i > k && i < N) ? tau * s : 0.0;
+                if (q4 == 0) sm.p[i] = pi;
                 const double vi = (i == k + 1) ? 1.0 : ((i > k + 1 && i < N) ? xk[i] * scal : 0.0);
-                const double pv = wave_sum(pi * vi);
+                const double pv = wave_sum(q4 == 0 ? pi * vi : 0.0);
                 if (l == 0) sm.red[tid / WAVE] = pv;
             }
             __syncthreads();
             if (tau != 0.0) {
                 double pvt = 0.0;  // fixed order: bitwise reproducible
 #pragma unroll
-                for (int q = 0; q < NP / WAVE; ++q) pvt += sm.red[q];
+                for (int q = 0; q < NWV; ++q) pvt += sm.red[q];
                 const double coef = uniform(0.5 * tau * pvt);
                 double wc[NBC];
 #pragma unroll
                 for (int b = 0; b < NBC; ++b) wc[b] = sm.p[l + 64 * b] - coef * vc[b];
 #pragma unroll
-                for (int a = 0; a < NA; ++a) {
-                    if (w + 16 * a <= k) continue;
-                    const int src = (w + 16 * a) & 63;
-                    const double vra = readlane_d(vc[a >> 2], src);
-                    const double wra = readlane_d(wc[a >> 2], src);
+                for (int g = 0; g < 4; ++g) {
+                    if (g < bmin) continue;
 #pragma unroll
-                    for (int b = 0; b <= (a >> 2); ++b)
-                        if (b >= bmin) A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
+                    for (int a = 4 * g; a < 4 * g + 4; ++a) {
+                        const int src = (w + 16 * a) & 63;
+                        const double vra = readlane_d(vc[g], src);
+                        const double wra = readlane_d(wc[g], src);
+#pragma unroll
+                        for (int b = 0; b <= g; ++b) A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
+                    }
                 }
             }
             emit(A, sm, xn, k + 1, N, w, l);

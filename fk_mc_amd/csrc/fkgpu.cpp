@@ -73,6 +73,13 @@ struct fkgpu_chains {
     // mixed-precision mode (FP32 reduction + FP64 refinement)
     int mixed = 0;
     struct mixed_bufs* mx = nullptr;
+    // rank-one update mode: eigenvectors kept on the device, full re-decomposition
+    // every fu_refresh steps
+    int fu = 0, fu_refresh = 0;
+    long long fu_count = 0;
+    double *fu_V = nullptr, *fu_Q1 = nullptr, *fu_Q2 = nullptr, *fu_T = nullptr;
+    double *fu_vh = nullptr, *fu_tau = nullptr, *fu_d = nullptr, *fu_e = nullptr, *fu_scr = nullptr;
+    int32_t *fu_upd = nullptr, *fu_f32 = nullptr;
 };
 
 namespace {
@@ -978,6 +985,10 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     if (ch->ev0) hipEventDestroy(ch->ev0);
     if (ch->ev1) hipEventDestroy(ch->ev1);
     mixed_free(ch->mx);
+    void* fptrs[] = {ch->fu_V, ch->fu_Q1, ch->fu_Q2, ch->fu_T, ch->fu_vh, ch->fu_tau, ch->fu_d, ch->fu_e, ch->fu_scr,
+                     ch->fu_upd, ch->fu_f32};
+    for (void* p : fptrs)
+        if (p) hipFree(p);
     fkgpu_ctx* ctx = ch->ctx;
     delete ch;
     ctx_release(ctx);
@@ -1110,6 +1121,10 @@ int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, c
         sum += weights[m];
     }
     if (!(sum > 0.0)) return fail(ctx, FKGPU_EINVAL, "move weights sum to zero");
+    if (ch->fu)
+        for (int m = 0; m < nmoves; ++m)
+            if (kinds[m] == FKGPU_MOVE_RANDOMIZE)
+                return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one updates cover moves of one or two sites (not randomize)");
     // libstdc++ discrete_distribution::param_type::_M_initialize: normalise by
     // division, partial sums, last entry forced to 1; a single weight draws nothing.
     sweep_args& s = ch->args;
@@ -1143,8 +1158,43 @@ int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W) {
 
 // One move step of every chain: the fused dense kernel, or propose / stochastic
 // trace / accept for the Chebyshev (KPM) mode.
+// Full eigendecomposition of every chain's current configuration into spectrum, logZ
+// and the eigenvector matrices of the rank-one update mode.
+static hipError_t fu_refresh_launch(fkgpu_chains* ch, hipStream_t st) {
+    fkgpu_ctx* ctx = ch->ctx;
+    const int B = ch->B, N = ch->N;
+    hipError_t e = fk_launch_f8to32(B * N, ch->f, ch->fu_f32, st);
+    if (e != hipSuccess) return e;
+    eig_args a{};
+    a.B = B;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = ch->fu_f32;
+    a.beta = ch->beta;
+    a.U = ch->U;
+    a.mu_c = ch->mu_c;
+    a.evals = ch->spectrum;
+    a.logz = ch->logz;
+    a.ws = ch->ws;
+    a.ws_stride = ch->ws_stride;
+    a.vout = ch->fu_vh;
+    a.tauout = ch->fu_tau;
+    a.d_out = ch->fu_d;
+    a.e_out = ch->fu_e;
+    e = fk_launch_eig(ch->kind, a, ch->grid, st);
+    if (e != hipSuccess) return e;
+    return fk_launch_evecs(B, N, ch->fu_d, ch->fu_e, ch->spectrum, ch->fu_vh, ch->fu_tau, ch->fu_V, ch->fu_scr, st);
+}
+
 static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t st) {
     fkgpu_ctx* ctx = ch->ctx;
+    if (ch->fu) {
+        hipError_t e = fk_launch_fu_step(a, ch->fu_V, ch->fu_Q1, ch->fu_Q2, ch->fu_upd, ch->fu_T, ch->fu_V, st);
+        if (e != hipSuccess) return e;
+        // a.step counts the steps of this call (sweep) or of the captured graph
+        if (ch->fu_refresh > 0 && (a.step + 1) % ch->fu_refresh == 0) e = fu_refresh_launch(ch, st);
+        return e;
+    }
     if (ch->mixed) {
         hipError_t e = fk_launch_kpm_propose(a, ch->k_props, ch->k_fprop, ch->k_skip, st);
         if (e != hipSuccess) return e;
@@ -1162,6 +1212,52 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
     return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->k_lz, st);
 }
 
+int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    const int B = ch->B, N = ch->N;
+    if (mode != FKGPU_UPDATE_DENSE && mode != FKGPU_UPDATE_RANK1) return fail(ctx, FKGPU_EINVAL, "unknown update mode");
+    if (refresh_steps < 0) return fail(ctx, FKGPU_EINVAL, "negative refresh interval");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+    ch->graph_steps = 0;
+    if (mode == FKGPU_UPDATE_DENSE) {
+        ch->fu = 0;
+        return FKGPU_OK;
+    }
+    if (ch->kpm || ch->mixed) return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one updates need the FP64 dense eigensolver");
+    if (!ch->dense_ok || N > fk_fu_nmax() || N > 1024)
+        return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one updates implemented for N <= 1024");
+    for (int m = 0; m < ch->args.ncp; ++m)
+        if (ch->args.kinds[m] == FKGPU_MOVE_RANDOMIZE)
+            return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one updates cover moves of one or two sites (not randomize)");
+    if (!ch->fu_V) {
+        const size_t BN = (size_t)B * N, BNN = BN * N;
+        hipError_t e = hipSuccess;
+        auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+        chk(dalloc(&ch->fu_V, BNN));
+        chk(dalloc(&ch->fu_Q1, BNN));
+        chk(dalloc(&ch->fu_Q2, BNN));
+        chk(dalloc(&ch->fu_T, BNN));
+        chk(dalloc(&ch->fu_vh, BNN));
+        chk(dalloc(&ch->fu_tau, BN));
+        chk(dalloc(&ch->fu_d, BN));
+        chk(dalloc(&ch->fu_e, BN));
+        chk(dalloc(&ch->fu_scr, 3 * BNN));
+        chk(dalloc(&ch->fu_upd, B));
+        chk(dalloc(&ch->fu_f32, BN));
+        if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "rank-one update buffers");
+    }
+    FK_HIP(ctx, fu_refresh_launch(ch, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    ch->fu = 1;
+    ch->fu_refresh = refresh_steps;
+    return FKGPU_OK;
+}
+
 int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
@@ -1169,6 +1265,8 @@ int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
     if (precision != FKGPU_PRECISION_FP64 && precision != FKGPU_PRECISION_MIXED)
         return fail(ctx, FKGPU_EINVAL, "unknown precision");
     if (ch->kpm) return fail(ctx, FKGPU_EUNSUPPORTED, "precision applies to the dense eigensolver, not the stochastic trace");
+    if (ch->fu && precision == FKGPU_PRECISION_MIXED)
+        return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one update mode is FP64; switch it off first");
     if (precision == FKGPU_PRECISION_MIXED && N > fk_solver_nmax(9))
         return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
     hipSetDevice(ctx->device);
@@ -1294,6 +1392,7 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
     FK_HIP(ctx, hipStreamSynchronize(st));
     ch->kpm = 1;
     ch->mixed = 0;
+    ch->fu = 0;
     ch->kR = R;
     ch->kM = M;
     ch->kseed = seed;

@@ -18,6 +18,7 @@ LATTICE_HYPERCUBIC, LATTICE_TRIANGULAR = 0, 1
 MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
 SOLVER_AUTO, SOLVER_VARIANTS = -1, 6
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+UPDATE_DENSE, UPDATE_RANK1 = 0, 1
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -92,6 +93,7 @@ def lib():
         L.fkgpu_cheb_logz.argtypes = [P, _i32, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _d, _d, _d]
         L.fkgpu_eig_logz_mixed.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _i64]
         L.fkgpu_chains_set_precision.argtypes = [P, C.c_int]
+        L.fkgpu_chains_set_update.argtypes = [P, C.c_int, C.c_int]
         L.fkgpu_chains_fallbacks.argtypes = [P, _i64]
         L.fkgpu_set_solver.argtypes = [P, C.c_int]
         L.fkgpu_get_solver.argtypes = [P]
@@ -359,6 +361,9 @@ class Chains:
 
     def set_precision(self, precision):
         self.ctx._check(lib().fkgpu_chains_set_precision(self._h, precision))
+
+    def set_update(self, mode, refresh=256):
+        self.ctx._check(lib().fkgpu_chains_set_update(self._h, mode, refresh))
 
     def fallbacks(self):
         n = np.zeros(1, np.int64)

@@ -180,6 +180,21 @@ int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W);
 #define FKGPU_PRECISION_FP64 0
 #define FKGPU_PRECISION_MIXED 1
 int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision);
+/* How a chain's proposal is diagonalised (no reference counterpart; the reference
+ * re-diagonalises every proposal):
+ *   FKGPU_UPDATE_DENSE  a full FP64 eigensolve per proposed move (default)
+ *   FKGPU_UPDATE_RANK1  the eigenvectors are kept on the device; a move of one or two
+ *                       sites is a rank-one (two rank-one) update of the spectrum by the
+ *                       secular equation, O(N^2) per proposal, and an accepted move
+ *                       updates the eigenvectors by one (two) N x N MFMA products.
+ *                       Every refresh_steps steps of a fkgpu_sweep call (or of a graph
+ *                       replay) the eigendecomposition is recomputed from scratch
+ *                       (0: never).  Moves: addremove, flip, hop.  N <= 1024.
+ * Switching to FKGPU_UPDATE_RANK1 computes the eigendecomposition of the current
+ * configurations. */
+#define FKGPU_UPDATE_DENSE 0
+#define FKGPU_UPDATE_RANK1 1
+int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps);
 /* Configurations the mixed-precision refinement handed to the FP64 solver so far. */
 int fkgpu_chains_fallbacks(const fkgpu_chains* ch, int64_t* n);
 /* Switch the chain set to the stochastic-trace logZ (the Chebyshev moves of

@@ -38,6 +38,7 @@ constexpr int UT = 256;
 constexpr int UW = UT / WAVE;
 constexpr int UNMAX = 1024;
 
+template <int UNMAX>
 struct usmem {
     double d[UNMAX];               // current eigenvalues, ascending
     double dv[UNMAX], zv[UNMAX];   // working poles and weights (sign-mapped)
@@ -82,7 +83,9 @@ __device__ double block_sum(double v, double* red) {
 // Eigen-update of diag(sm.d) + rho z z^T.  On return sm.d holds the new eigenvalues
 // in ascending order and Q (N x N, row-major, HBM) the orthogonal matrix with
 // diag(d) + rho z z^T = Q diag(d_new) Q^T.  All threads of the workgroup call it.
-__device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __restrict__ Q) {
+// Reciprocals are v_rcp_f64 refined by Newton steps (full double precision).
+template <class SM>
+__device__ void rank1(SM& sm, const double* z, double rho, int N, double* __restrict__ Q) {
     const int tid = threadIdx.x;
     const double eps = 2.220446049250313e-16;
     const bool neg = rho < 0.0;
@@ -112,7 +115,7 @@ __device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __r
                 continue;
             }
             if (p >= 0) {
-                const double tau = hypot(sm.zv[p], sm.zv[t]);
+                const double tau = sqrt(sm.zv[p] * sm.zv[p] + sm.zv[t] * sm.zv[t]);  // |z| <= 1: no overflow
                 const double c = sm.zv[t] / tau, s = -sm.zv[p] / tau;
                 const double dt = sm.dv[t] - sm.dv[p];
                 if (fabs(dt * c * s) <= tol) {
@@ -161,7 +164,7 @@ __device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __r
         if (j < m - 1) {
             const double half = 0.5 * (sm.da[j + 1] - sm.da[j]);
             double g = irh;
-            for (int k = 0; k < m; ++k) g += sm.za[k] * sm.za[k] / ((sm.da[k] - sm.da[j]) - half);
+            for (int k = 0; k < m; ++k) g += sm.za[k] * sm.za[k] * rcp_refined((sm.da[k] - sm.da[j]) - half);
             if (g >= 0.0) { o = j; lo = 0.0; hi = half; }
             else { o = j + 1; lo = -half; hi = 0.0; }
         } else {
@@ -171,23 +174,25 @@ __device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __r
         }
         const double zo2 = sm.za[o] * sm.za[o];
         double x = 0.5 * (lo + hi);
-        for (int it = 0; it < 200; ++it) {
-            double S = 0.0, S1 = 0.0;
+        for (int it = 0; it < 64; ++it) {
+            double S = 0.0, S1 = 0.0, Sa = 0.0;
             for (int k = 0; k < m; ++k) {
                 if (k == o) continue;
-                const double q = sm.za[k] / ((sm.da[k] - sm.da[o]) - x);
+                const double q = sm.za[k] * rcp_refined((sm.da[k] - sm.da[o]) - x);
                 S += sm.za[k] * q;
+                Sa += fabs(sm.za[k] * q);
                 S1 += q * q;
             }
             const double base = irh + S;
-            const double g = base - zo2 / x;          // secular function (x != 0 inside the bracket)
-            if (g == 0.0) break;
-            if (g > 0.0) hi = x; else lo = x;
             const double gh = -x * base + zo2;
+            // |gh| at the level of its rounding error: x is a root to working precision
+            if (fabs(gh) <= 16.0 * eps * (fabs(x) * (irh + Sa) + zo2)) break;
+            const double g = base - zo2 / x;  // secular function (x != 0 inside the bracket)
+            if (g > 0.0) hi = x; else lo = x;
             const double dgh = -base - x * S1;
             double xn = (dgh != 0.0) ? x - gh / dgh : 0.5 * (lo + hi);
             if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-            const bool done = fabs(xn - x) <= 2.0 * eps * fabs(xn) || hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi));
+            const bool done = fabs(xn - x) <= 4.0 * eps * fabs(xn) || hi - lo <= 4.0 * eps * fmax(fabs(lo), fabs(hi));
             x = xn;
             if (done) break;
         }
@@ -197,11 +202,20 @@ __device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __r
     __syncthreads();
     // Gu-Eisenstat: zh_k^2 = (lambda_{m-1} - da_k) prod_{j<k} (lambda_j - da_k)/(da_j - da_k)
     //                        prod_{j>k} (lambda_{j-1} - da_k)/(da_j - da_k) / rho
+    // (numerator and denominator multiplied in groups of 8 factors, one division per group)
     for (int k = tid; k < m; k += UT) {
         const double dk = sm.da[k];
         double pr = (sm.da[sm.org[m - 1]] - dk) + sm.mu[m - 1];
-        for (int j = 0; j < k; ++j) pr *= ((sm.da[sm.org[j]] - dk) + sm.mu[j]) / (sm.da[j] - dk);
-        for (int j = k + 1; j < m; ++j) pr *= ((sm.da[sm.org[j - 1]] - dk) + sm.mu[j - 1]) / (sm.da[j] - dk);
+        double nu = 1.0, de = 1.0;
+        int cnt = 0;
+        for (int j = 0; j < m; ++j) {
+            if (j == k) continue;
+            const int jn = j < k ? j : j - 1;  // numerator root paired with pole j
+            nu *= (sm.da[sm.org[jn]] - dk) + sm.mu[jn];
+            de *= sm.da[j] - dk;
+            if (++cnt == 8) { pr *= nu / de; nu = de = 1.0; cnt = 0; }
+        }
+        pr *= nu / de;
         sm.zh[k] = copysign(sqrt(fmax(pr * irh, 0.0)), sm.za[k]);
     }
     __syncthreads();
@@ -209,7 +223,7 @@ __device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __r
         const double dorg = sm.da[sm.org[j]], mj = sm.mu[j];
         double s = 0.0;
         for (int k = 0; k < m; ++k) {
-            const double q = sm.zh[k] / ((sm.da[k] - dorg) - mj);
+            const double q = sm.zh[k] * rcp_refined((sm.da[k] - dorg) - mj);
             s += q * q;
         }
         sm.rn[j] = 1.0 / sqrt(s);
@@ -260,7 +274,7 @@ __device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __r
             double v = 0.0;
             if (sr >= 0) {
                 const int k = sm.cidx[t];
-                if (k >= 0) v = sm.zh[k] / ((sm.da[k] - sm.da[sm.org[sr]]) - sm.mu[sr]) * sm.rn[sr];
+                if (k >= 0) v = sm.zh[k] * rcp_refined((sm.da[k] - sm.da[sm.org[sr]]) - sm.mu[sr]) * sm.rn[sr];
             } else {
                 v = (t == -sr - 1) ? 1.0 : 0.0;
             }
@@ -286,9 +300,10 @@ __device__ void rank1(usmem& sm, const double* z, double rho, int N, double* __r
 }
 
 // One move of every chain.  V: B x N x N eigenvectors (V[c][i][k], column k for d_k).
+template <int NM>
 __global__ void __launch_bounds__(UT) k_fu_step(sweep_args a, const double* __restrict__ Vg, double* __restrict__ Q1g,
                                                 double* __restrict__ Q2g, int32_t* __restrict__ upd) {
-    __shared__ usmem sm;
+    __shared__ usmem<NM> sm;
     const int ch = blockIdx.x, tid = threadIdx.x, N = a.N;
     const size_t NN = (size_t)N * N;
     uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
@@ -420,7 +435,11 @@ __global__ void k_fu_f8to32(int n, const uint8_t* __restrict__ f8, int32_t* __re
 hipError_t fk_launch_fu_step(const sweep_args& a, const double* V, double* Q1, double* Q2, int32_t* upd,
                              double* T, double* Vout, hipStream_t st) {
     if (a.N > fku::UNMAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fku::k_fu_step, dim3(a.B), dim3(fku::UT), 0, st, a, V, Q1, Q2, upd);
+    // shared memory sized to N: four chains per CU for N <= 256
+    if (a.N <= 256)
+        hipLaunchKernelGGL(fku::k_fu_step<256>, dim3(a.B), dim3(fku::UT), 0, st, a, V, Q1, Q2, upd);
+    else
+        hipLaunchKernelGGL(fku::k_fu_step<fku::UNMAX>, dim3(a.B), dim3(fku::UT), 0, st, a, V, Q1, Q2, upd);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     const int nbt = (a.N + fku::GB - 1) / fku::GB;

@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=8, help="mixed-precision timing steps (0: skip)")
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
+    ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); chains are independent, "
@@ -109,6 +110,9 @@ def main():
         plain = {"moves_per_s": round(plain_rate, 2), "ms_per_move": round((tp1 - tp0) * 1e3 / args.plain_moves, 4),
                  "batched_over_plain": round(value / plain_rate, 1)}
 
+    rank1 = None
+    if args.rank1_steps > 0:
+        rank1 = rank1_block(fk, ctx, f, beta, U, seeds, args.rank1_steps, plain, L)
     mixed = None
     if args.mixed_steps > 0:
         mixed = mixed_block(fk, ctx, f, beta, U, seeds, args.mixed_steps, plain)
@@ -140,6 +144,7 @@ def main():
         "accept_rate": round(acc / moves, 4),
         "roofline": roofline,
         "plain": plain,
+        "rank1_update": rank1,
         "mixed": mixed,
         "mixed_cubic3d_L8": c3,
         "cpu_baseline": cpu,
@@ -184,6 +189,41 @@ def run_chains(fk, ctx, f, beta, U, seeds, precision, steps, trace=False, warmup
     nfb = ch.fallbacks()
     ch.close()
     return ms, nfb
+
+
+def rank1_block(fk, ctx, f, beta, U, seeds, steps, plain, L):
+    """The same 1024 chains with rank-one spectral updates (secular equation per
+    proposal, MFMA eigenvector update on acceptance, full re-decomposition every 256
+    steps): moves/s, and the proposals' dOmega against the FP64 oracle."""
+    from oracle import oracle as orc
+    B = len(beta)
+    ch = fk.Chains(ctx, f, beta, U, U / 2, U / 2, seeds)
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    ch.set_update(fk.UPDATE_RANK1, 256)
+    ch.sweep_graph(4)
+    ch.sweep_graph(steps)
+    ms = ch.last_ms() / steps
+    ch.close()
+    out = {"moves_per_s": round(B / ms * 1e3, 2), "ms_per_step": round(ms, 4), "steps": steps,
+           "refresh_steps": 256}
+    if plain:
+        out["over_plain"] = round(out["moves_per_s"] / plain["moves_per_s"], 1)
+    # dOmega of traced proposals against the oracle chain following the same decisions
+    nchk, msteps = 4, 40
+    sub = fk.Chains(ctx, f[:nchk], beta[:nchk], U[:nchk], U[:nchk] / 2, U[:nchk] / 2, seeds[:nchk])
+    sub.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    sub.set_update(fk.UPDATE_RANK1, 256)
+    tr = sub.sweep(msteps, trace=True)
+    sub.close()
+    errs = []
+    for c in range(nchk):
+        ref = orc.run_chain(orc.KIND_HYPERCUBIC, 2, L, beta[c], U[c], U[c] / 2, U[c] / 2, int(seeds[c]), f[c],
+                            [(orc.MOVE_ADDREMOVE, 0.5), (orc.MOVE_HOP, 0.5)], msteps, t=1.0, tp=0.0,
+                            forced=tr["acc"][:, c].astype(np.int32))
+        solved = ref["logz"] != 0.0
+        errs.append(np.max(np.abs(tr["logz"][solved, c] - ref["logz"][solved]) / beta[c], initial=0.0))
+    out["dOmega_max_abs_err_vs_fp64_oracle"] = float(max(errs))
+    return out
 
 
 def mixed_block(fk, ctx, f, beta, U, seeds, steps, plain):

@@ -30,6 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the headline move-step kernel on the default workload (cubic2d
+# L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (scales
+# from a 256 MiB copy, bench.py --copy-calibration); keyed by solver variant
+TRAFFIC_BYTES = {5: 13.26e6}
 KERNEL_NAMES = {0: "fkk::k_sweep<reg_solver<16,4>>", 1: "fkk::k_sweep<reg_solver<16,8>>",
                 2: "fkk::k_sweep<reg_solver<32,8>>", 3: "fkk::k_sweep<glob_solver>",
                 4: "fkk::k_sweep<blocked_solver<double,512>>", 5: "fkk::k_sweep<wrow_solver>"}
@@ -59,7 +63,18 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=8, help="mixed-precision timing steps (0: skip)")
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
     ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB three times (scale of the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copied 256 MiB x3")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); chains are independent, "
               "so several GPUs would each run their own share of the batch.", file=sys.stderr)
@@ -90,7 +105,8 @@ def main():
     flop = solves * (4.0 / 3.0) * N ** 3
     achieved = flop / (kern_ms / 1e3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+                "traffic": TRAFFIC_BYTES.get(ctx.get_solver()) if (L, B) == (16, 1024) else None,
                 "kernel": KERNEL_NAMES.get(ctx.get_solver(), "k_sweep"), "kernel_ms_per_launch": round(kern_ms / args.steps, 4),
                 "flop_per_launch": flop / args.steps, "eigensolves_per_launch": solves / args.steps}
 

@@ -202,15 +202,19 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     __shared__ uint32_t mt[fkrng::MT_WORDS];
     __shared__ move_prop prop;
     __shared__ int acc_s;
+    // the move helpers take the argument block by reference: an LDS copy keeps the
+    // compiler from materialising a private copy of it in every lane
+    __shared__ sweep_args sa;
     const int tid = threadIdx.x;
     const int N = a.N;
     const int ch = blockIdx.x;  // one workgroup per chain
+    if (tid == 0) sa = a;
     uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
     for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;
     __syncthreads();
     const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch], mu_f = a.mu_f[ch];
-    if (tid == 0) prop = propose_move(mt, f_lds, N, a.nf[ch], a);
+    if (tid == 0) prop = propose_move(mt, f_lds, N, a.nf[ch], sa);
     __syncthreads();
     double lz_new = 0.0;
     if (prop.solve) {
@@ -221,7 +225,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
         double logw;
         const int acc = decide_move(prop, prop.sa >= 0 ? f_lds[prop.sa] : 0, lz_new, a.logz[ch], beta, mu_f,
                                     a.nf[ch], &logw);
-        record_move(a, ch, prop, acc, logw, lz_new);
+        record_move(sa, ch, prop, acc, logw, lz_new);
         acc_s = acc;
     }
     __syncthreads();
@@ -242,13 +246,15 @@ __global__ void __launch_bounds__(KC_NT) k_kpm_propose(sweep_args a, move_prop* 
                                                        int32_t* __restrict__ fprop, int32_t* __restrict__ skip) {
     __shared__ uint8_t f_lds[KC_NMAX];
     __shared__ uint32_t mt[fkrng::MT_WORDS];
+    __shared__ sweep_args sa;
     const int ch = blockIdx.x, tid = threadIdx.x, N = a.N;
     uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
+    if (tid == 0) sa = a;
     for (int i = tid; i < fkrng::MT_WORDS; i += KC_NT) mt[i] = mtg[i];
     for (int i = tid; i < N; i += KC_NT) f_lds[i] = a.f[(size_t)ch * N + i];
     __syncthreads();
     if (tid == 0) {
-        const move_prop p = propose_move(mt, f_lds, N, a.nf[ch], a);
+        const move_prop p = propose_move(mt, f_lds, N, a.nf[ch], sa);
         props[ch] = p;
         skip[ch] = !p.solve;
     }
@@ -262,14 +268,16 @@ __global__ void __launch_bounds__(KC_NT) k_kpm_accept(sweep_args a, const move_p
                                                       const double* __restrict__ lz_new_g,
                                                       const double* __restrict__ spec_new) {
     __shared__ int acc_s;
+    __shared__ sweep_args sa;
     const int ch = blockIdx.x, tid = threadIdx.x, N = a.N;
     const move_prop p = props[ch];
     const double lz_new = p.solve ? lz_new_g[ch] : 0.0;
     if (tid == 0) {
+        sa = a;
         double logw;
         const int acc = decide_move(p, p.sa >= 0 ? fprop[(size_t)ch * N + p.sa] : 0, lz_new, a.logz[ch], a.beta[ch],
                                     a.mu_f[ch], a.nf[ch], &logw);
-        record_move(a, ch, p, acc, logw, lz_new);
+        record_move(sa, ch, p, acc, logw, lz_new);
         acc_s = acc;
     }
     __syncthreads();

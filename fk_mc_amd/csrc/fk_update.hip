@@ -304,7 +304,9 @@ template <int NM>
 __global__ void __launch_bounds__(UT) k_fu_step(sweep_args a, const double* __restrict__ Vg, double* __restrict__ Q1g,
                                                 double* __restrict__ Q2g, int32_t* __restrict__ upd) {
     __shared__ usmem<NM> sm;
+    __shared__ sweep_args sa;  // LDS copy for the by-reference move helpers (no private copy per lane)
     const int ch = blockIdx.x, tid = threadIdx.x, N = a.N;
+    if (tid == 0) sa = a;
     const size_t NN = (size_t)N * N;
     uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
     for (int i = tid; i < fkrng::MT_WORDS; i += UT) sm.mt[i] = mtg[i];
@@ -314,7 +316,7 @@ __global__ void __launch_bounds__(UT) k_fu_step(sweep_args a, const double* __re
     }
     __syncthreads();
     const int nf_old = a.nf[ch];
-    if (tid == 0) sm.prop = propose_move(sm.mt, sm.f, N, nf_old, a);
+    if (tid == 0) sm.prop = propose_move(sm.mt, sm.f, N, nf_old, sa);
     __syncthreads();
     const move_prop p = sm.prop;
     const double beta = a.beta[ch], U = a.U[ch];
@@ -347,7 +349,7 @@ __global__ void __launch_bounds__(UT) k_fu_step(sweep_args a, const double* __re
         double logw;
         const int acc = decide_move(p, p.sa >= 0 ? sm.f[p.sa] : 0, lz_new, a.logz[ch], beta, a.mu_f[ch], nf_old,
                                     &logw);
-        record_move(a, ch, p, acc, logw, lz_new);
+        record_move(sa, ch, p, acc, logw, lz_new);
         sm.acc = acc;
         upd[ch] = (acc && p.solve) ? nupd : 0;
     }

@@ -47,30 +47,22 @@ __device__ __forceinline__ double2 pl16_swap(double x, double y) {
 }
 constexpr int DPP_ROR8 = 0x128;  // row_ror:8, lane i <- lane (i + 8) mod 16 within a 16-lane row
 
-// Reduce-scatter of 8 values per lane over the 64 lanes: afterwards every lane
-// holds the full sum of value number 4*bit5 + 2*bit4 + bit3 of its lane id.
-__device__ __forceinline__ double reduce8(double (&r)[8]) {
-    const int lane = threadIdx.x & (WAVE - 1);
+// Reduce-scatter of 4 values per lane over the 64 lanes: afterwards every lane holds
+// the full sum of value number 2*bit5 + bit4 of its lane id.
+__device__ __forceinline__ double reduce4(double (&r)[4]) {
     // lanes l and l^32: v_permlane32_swap(x, y) leaves x = [x.lo, y.lo], y = [x.hi, y.hi]
-    double s[4];
+    double s[2];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const double2 t = pl32_swap(r[a], r[a + 4]);
+    for (int a = 0; a < 2; ++a) {
+        const double2 t = pl32_swap(r[a], r[a + 2]);
         s[a] = t.x + t.y;
     }
     // lanes l and l^16: v_permlane16_swap exchanges the odd rows of x with the even rows of y
-    double t2[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        const double2 t = pl16_swap(s[a], s[a + 2]);
-        t2[a] = t.x + t.y;
-    }
-    // lanes l and l^8 (row_ror:8 within 16-lane rows)
-    const bool hi8 = (lane & 8) != 0;
-    const double keep = hi8 ? t2[1] : t2[0];
-    const double send = hi8 ? t2[0] : t2[1];
-    double u = keep + dpp_f64<DPP_ROR8>(send);
-    // the 8 lanes sharing bits 5..3 hold partial sums of the same value: all-reduce them
+    const double2 t = pl16_swap(s[0], s[1]);
+    double u = t.x + t.y;
+    // the 16 lanes sharing bits 5, 4 hold partial sums of the same value: all-reduce them
+    // (row_ror:8 pairs lanes l, l^8; quad_perm l^1, l^2; row_half_mirror l^7)
+    u += dpp_f64<DPP_ROR8>(u);
     u += dpp_f64<DPP_XOR1>(u);
     u += dpp_f64<DPP_XOR2>(u);
     u += dpp_f64<DPP_HALF_MIRROR>(u);
@@ -167,22 +159,22 @@ struct wrow_solver {
             // on finished indices), which leaves them unchanged; work is skipped only per
             // 64-wide block, so every wave runs the same instruction stream.
             if (tau != 0.0) {
-                // row sums over the lanes' columns (two groups of 8 rows)
+                // row sums over the lanes' columns (four groups of 4 rows)
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (128 * h + 127 <= k) continue;
-                    double rp[8];
+                for (int h = 0; h < 4; ++h) {
+                    if (64 * h + 63 <= k) continue;
+                    double rp[4];
 #pragma unroll
-                    for (int aa = 0; aa < 8; ++aa) {
-                        const int a = 8 * h + aa;
+                    for (int aa = 0; aa < 4; ++aa) {
+                        const int a = 4 * h + aa;
                         double s = 0.0;
 #pragma unroll
                         for (int b = 0; b <= (a >> 2); ++b) s = fma(A[sl(a, b)], vc[b], s);
                         rp[aa] = s;
                     }
-                    const double rsum = reduce8(rp);
-                    if ((l & 7) == 0) {
-                        const int a = 8 * h + 4 * ((l >> 5) & 1) + 2 * ((l >> 4) & 1) + ((l >> 3) & 1);
+                    const double rsum = reduce4(rp);
+                    if ((l & 15) == 0) {
+                        const int a = 4 * h + 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
                         sm.rs[w + 16 * a] = rsum;
                     }
                 }

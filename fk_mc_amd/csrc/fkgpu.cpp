@@ -812,6 +812,56 @@ int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* bet
     return FKGPU_OK;
 }
 
+// Newton iteration on mu_c for a target conduction filling N_c / N, with N_c and
+// dN_c/dmu_c = beta sum_k f_k (1 - f_k) from the adjoint (dOmega/dmu_c = -N_c),
+// safeguarded by bisection on a bracket from the spectrum bounds of H.
+int fkgpu_tune_mu(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U, double* mu_c,
+                  double filling, double tol, int maxit, int32_t* iters) {
+    int r = check_params(ctx, batch, f, beta, U, mu_c);
+    if (r) return r;
+    if (!(filling > 0.0 && filling < 1.0) || !(tol > 0.0) || maxit < 1)
+        return fail(ctx, FKGPU_EINVAL, "filling must lie in (0, 1), tol > 0, maxit >= 1");
+    const int N = ctx->N;
+    double rad = 0.0;  // largest Gershgorin radius of the hopping matrix
+    for (int i = 0; i < N; ++i) {
+        double s = 0.0;
+        for (int q = 0; q < ctx->z[i]; ++q) s += std::fabs(ctx->hopv[(size_t)i * ctx->zmax + q]);
+        rad = std::max(rad, s);
+    }
+    std::vector<double> lo(batch), hi(batch), mu(mu_c, mu_c + batch), lz(batch), dens((size_t)batch * N),
+        gr((size_t)batch * 8);
+    std::vector<int32_t> done(batch, 0), it(batch, 0);
+    for (int b = 0; b < batch; ++b) {
+        const double span = rad + std::fabs(U[b]) + 60.0 / beta[b];
+        lo[b] = std::min(0.0, U[b]) - span;
+        hi[b] = std::max(0.0, U[b]) + span;
+        if (!(mu[b] > lo[b] && mu[b] < hi[b])) mu[b] = 0.5 * (lo[b] + hi[b]);
+    }
+    const double target = filling * N;
+    for (int iter = 0; iter < maxit; ++iter) {
+        if ((r = fkgpu_adjoint(ctx, batch, f, beta, U, mu.data(), lz.data(), dens.data(), gr.data(), nullptr)))
+            return r;
+        int left = 0;
+        for (int b = 0; b < batch; ++b) {
+            if (done[b]) continue;
+            const double nc = gr[(size_t)b * 8 + 4], dn = gr[(size_t)b * 8 + 5];
+            it[b] = iter + 1;
+            if (std::fabs(nc - target) <= tol * N) { done[b] = 1; continue; }
+            if (nc < target) lo[b] = mu[b]; else hi[b] = mu[b];
+            double next = dn > 0.0 ? mu[b] + (target - nc) / dn : 0.5 * (lo[b] + hi[b]);
+            if (!(next > lo[b] && next < hi[b])) next = 0.5 * (lo[b] + hi[b]);
+            mu[b] = next;
+            ++left;
+        }
+        if (!left) break;
+    }
+    for (int b = 0; b < batch; ++b) {
+        mu_c[b] = mu[b];
+        if (iters) iters[b] = done[b] ? it[b] : -it[b];
+    }
+    return FKGPU_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Chebyshev paths
 // ---------------------------------------------------------------------------

@@ -95,6 +95,7 @@ def lib():
         L.fkgpu_chains_set_precision.argtypes = [P, C.c_int]
         L.fkgpu_chains_set_update.argtypes = [P, C.c_int, C.c_int]
         L.fkgpu_chains_fallbacks.argtypes = [P, _i64]
+        L.fkgpu_tune_mu.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_double, C.c_double, C.c_int, _i32]
         L.fkgpu_set_solver.argtypes = [P, C.c_int]
         L.fkgpu_get_solver.argtypes = [P]
         L.fkgpu_autotune.argtypes = [P, C.c_int, _d]
@@ -290,6 +291,17 @@ def _eig_logz_mixed(self, f, beta, U, mu_c):
     return dict(evals=ev, logZ=lz, nfallback=int(nfb[0]))
 
 
+def _tune_mu(self, f, beta, U, mu0=0.0, filling=0.5, tol=1e-12, maxit=50):
+    """Newton-tune mu_c to a conduction filling with the adjoint's dN_c/dmu_c."""
+    f, B, b, u, m = _batch(f, beta, U, mu0)
+    m = m.copy()
+    its = np.zeros(B, np.int32)
+    self._check(lib().fkgpu_tune_mu(self._h, B, _p(f, _i32), _p(b, _d), _p(u, _d), _p(m, _d), filling, tol, maxit,
+                                    _p(its, _i32)))
+    return m, its
+
+
+Context.tune_mu = _tune_mu
 Context.eig_logz_mixed = _eig_logz_mixed
 Context.set_solver = _set_solver
 Context.get_solver = _get_solver

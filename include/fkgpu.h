@@ -128,6 +128,14 @@ int fkgpu_dos(fkgpu_ctx* ctx, int nspectra, const double* spectra, int nw, const
 int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
                   const double* mu_c, double* logZ, double* dens, double* grads, double* rho);
 
+/* mu_c tuned to a conduction filling (no reference counterpart): Newton's method on
+ * N_c(mu_c) with N_c and dN_c/dmu_c from fkgpu_adjoint (dOmega/dmu_c = -N_c), bisection
+ * when a step leaves the bracket.  mu_c[batch] in: starting values, out: tuned values with
+ * |N_c/N - filling| <= tol; iters[b] (nullable) = adjoint evaluations used, negative if
+ * maxit was reached first. */
+int fkgpu_tune_mu(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U, double* mu_c,
+                  double filling, double tol, int maxit, int32_t* iters);
+
 /* ---- Chebyshev paths ----------------------------------------------------------
  * configuration_t::calc_chebyshev (src/configuration.cpp:94-206) with
  * chebyshev_eval(cheb_size, grid_size) (include/fk_mc/chebyshev.hpp:15-63): exact

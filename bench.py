@@ -15,6 +15,13 @@ Printed: one JSON line with the contract fields plus
                 stream) against MI355X's 78.6 TFLOP/s FP64 peak;
   plain         the single-chain drop-in path (calc_hamiltonian + calc_ed
                 through the C ABI with host buffers, one call per move);
+  rank1_update, mixed, mixed_cubic3d_L8
+                the same chains with rank-one spectral updates / mixed precision,
+                and cubic3d L=8 (N=512) mixed against FP64, each with its dOmega
+                against FP64 dense chains on the same seeds;
+  kpm_cubic2d_L64
+                cubic2d L=64 (N=4096) stochastic-trace Chebyshev chains, with
+                every 100th proposal's dOmega against a dense FP64 eigensolve;
   cpu_baseline  the CPU restatement of the reference path (oracle, C++,
                 one chain per std::thread) on a bounded sample.
 """
@@ -63,6 +70,10 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=8, help="mixed-precision timing steps (0: skip)")
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
     ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
+    ap.add_argument("--c5-steps", type=int, default=16,
+                    help="cubic2d L=64 stochastic-trace chain timing steps (0: skip)")
+    ap.add_argument("--c5-check-steps", type=int, default=25,
+                    help="traced L=64 steps whose every 100th proposal is checked against a dense FP64 eigensolve")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (scale of the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -136,6 +147,10 @@ def main():
     if args.mixed_steps > 0 and not args.no_c3:
         c3 = c3_block(fk, max(2, args.mixed_steps // 2))
 
+    c5 = None
+    if args.c5_steps > 0:
+        c5 = c5_block(fk, args.c5_steps, args.c5_check_steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, args.cpu_seconds, ch, ctx, f, beta, U, seeds)
@@ -163,6 +178,7 @@ def main():
         "rank1_update": rank1,
         "mixed": mixed,
         "mixed_cubic3d_L8": c3,
+        "kpm_cubic2d_L64": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -210,8 +226,7 @@ def run_chains(fk, ctx, f, beta, U, seeds, precision, steps, trace=False, warmup
 def rank1_block(fk, ctx, f, beta, U, seeds, steps, plain, L):
     """The same 1024 chains with rank-one spectral updates (secular equation per
     proposal, MFMA eigenvector update on acceptance, full re-decomposition every 256
-    steps): moves/s, and the proposals' dOmega against the FP64 oracle."""
-    from oracle import oracle as orc
+    steps): moves/s, and the proposals' dOmega against FP64 dense chains."""
     B = len(beta)
     ch = fk.Chains(ctx, f, beta, U, U / 2, U / 2, seeds)
     ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
@@ -224,21 +239,17 @@ def rank1_block(fk, ctx, f, beta, U, seeds, steps, plain, L):
            "refresh_steps": 256}
     if plain:
         out["over_plain"] = round(out["moves_per_s"] / plain["moves_per_s"], 1)
-    # dOmega of traced proposals against the oracle chain following the same decisions
-    nchk, msteps = 4, 40
-    sub = fk.Chains(ctx, f[:nchk], beta[:nchk], U[:nchk], U[:nchk] / 2, U[:nchk] / 2, seeds[:nchk])
+    # dOmega of traced proposals against FP64 dense chains on the same seeds (compared up to
+    # each chain's first differing decision)
+    nchk, msteps = 64, 40
+    sl = slice(0, nchk)
+    t64 = run_chains(fk, ctx, f[sl], beta[sl], U[sl], seeds[sl], fk.PRECISION_FP64, msteps, trace=True)
+    sub = fk.Chains(ctx, f[sl], beta[sl], U[sl], U[sl] / 2, U[sl] / 2, seeds[sl])
     sub.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
     sub.set_update(fk.UPDATE_RANK1, 256)
     tr = sub.sweep(msteps, trace=True)
     sub.close()
-    errs = []
-    for c in range(nchk):
-        ref = orc.run_chain(orc.KIND_HYPERCUBIC, 2, L, beta[c], U[c], U[c] / 2, U[c] / 2, int(seeds[c]), f[c],
-                            [(orc.MOVE_ADDREMOVE, 0.5), (orc.MOVE_HOP, 0.5)], msteps, t=1.0, tp=0.0,
-                            forced=tr["acc"][:, c].astype(np.int32))
-        solved = ref["logz"] != 0.0
-        errs.append(np.max(np.abs(tr["logz"][solved, c] - ref["logz"][solved]) / beta[c], initial=0.0))
-    out["dOmega_max_abs_err_vs_fp64_oracle"] = float(max(errs))
+    out.update(compare_traces(t64, tr, beta[sl]))
     return out
 
 
@@ -278,6 +289,86 @@ def c3_block(fk, steps):
            "fp64_solver_variant": ctx.get_solver()}
     out.update(compare_traces(t64, tmx, beta))
     ctx.close()
+    return out
+
+
+def exact_logz_2d(L, f, beta, U, mu):
+    """logZ of cubic2d (t=1, periodic) H = hop + diag(-mu + U f) from a dense FP64
+    eigensolve (LAPACK through numpy): the exact value the stochastic trace estimates."""
+    n = L * L
+    H = np.zeros((n, n))
+    idx = np.arange(n).reshape(L, L)
+    for shift in ((1, 0), (0, 1)):
+        j = np.roll(idx, (-shift[0], -shift[1]), axis=(0, 1)).ravel()
+        H[idx.ravel(), j] -= 1.0
+        H[j, idx.ravel()] -= 1.0
+    H[np.arange(n), np.arange(n)] = -mu + U * f
+    ev = np.linalg.eigvalsh(H)
+    return float(np.logaddexp(0.0, -beta * ev).sum())
+
+
+def c5_block(fk, steps, check_steps, every=100):
+    """cubic2d L=64 (N=4096), 16 chains, beta=10, U=2, mu=U/2 on the stochastic-trace
+    Chebyshev path (R=64 Rademacher probes per chain, M=256 moments, Jackson kernel,
+    FP32 recursion with FP64 moment sums): moves/s, and dOmega of every 100th proposal
+    against an exact FP64 dense eigensolve of the old and new configuration."""
+    L, B, R, M = 64, 16, 64, 256
+    N = L * L
+    beta, U = 10.0, 2.0
+    rng = np.random.default_rng(64)
+    f0 = (rng.random((B, N)) < 0.5).astype(np.int32)
+    seeds = (6400 + np.arange(B)).astype(np.uint32)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    ones = np.ones(B)
+    ch = fk.Chains(ctx, f0, beta * ones, U * ones, U / 2 * ones, U / 2 * ones, seeds)
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    ch.set_kpm(R=R, M=M, seed=17, jackson=True)
+    out = {"N": N, "chains": B, "R": R, "M": M, "beta": beta, "U": U}
+    # validation pass (traced): replay each chain's accepted moves to recover the configurations
+    if check_steps > 0:
+        lz = ch.state()["logZ"].copy()
+        tr = ch.sweep(check_steps, trace=True)
+        f = f0.copy()
+        errs, checks, t_ed = [], [], 0.0
+        due = 0
+        for s in range(check_steps):
+            for c in range(B):
+                k = s * B + c
+                solved = tr["logz"][s, c] != 0.0
+                a, b = tr["a"][s, c], tr["b"][s, c]
+                g = f[c].copy()
+                if tr["kind"][s, c] == fk.MOVE_HOP and b >= 0:
+                    g[a], g[b] = f[c, b], f[c, a]
+                elif a >= 0:
+                    g[a] ^= 1
+                if solved and k >= due:
+                    t = time.perf_counter()
+                    d_ex = -(exact_logz_2d(L, g, beta, U, U / 2) - exact_logz_2d(L, f[c], beta, U, U / 2)) / beta
+                    t_ed += time.perf_counter() - t
+                    d_kpm = -(tr["logz"][s, c] - lz[c]) / beta
+                    errs.append(abs(d_kpm - d_ex))
+                    checks.append({"move": k, "dOmega_kpm": round(d_kpm, 6), "dOmega_exact": round(d_ex, 6)})
+                    due = (k // every + 1) * every
+                if tr["acc"][s, c]:
+                    f[c] = g
+                    if solved:
+                        lz[c] = tr["logz"][s, c]
+        out.update({"checked_moves": len(errs), "dOmega_max_abs_err_vs_dense_fp64": float(max(errs, default=0.0)),
+                    "dOmega_mean_abs_err_vs_dense_fp64": float(np.mean(errs)) if errs else 0.0,
+                    "checks": checks, "dense_check_seconds": round(t_ed, 2),
+                    "accept_rate_traced": round(float(tr["acc"].mean()), 4)})
+    # timed pass: graph-captured steps
+    ch.sweep_graph(2)
+    ch.sweep_graph(steps)
+    ms = ch.last_ms() / steps
+    st = ch.state()
+    ch.close()
+    ctx.close()
+    nnz = 4 * N
+    flop = B * R * (M // 2) * (2 * nnz + 6 * N)  # per step, every proposal solved
+    out.update({"steps": steps, "moves_per_s": round(B / ms * 1e3, 2), "ms_per_step": round(ms, 4),
+                "fp32_tflops_per_step_if_all_solved": round(flop / (ms / 1e3) / 1e12, 3),
+                "logZ_finite": bool(np.all(np.isfinite(st["logZ"])))})
     return out
 
 

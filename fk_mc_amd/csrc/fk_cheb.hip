@@ -90,9 +90,49 @@ __device__ __forceinline__ float rademacher(uint64_t seed, int r, int i) {
 constexpr int KPM_NT = 1024;
 constexpr int KPM_NMAX = 4096;
 constexpr int KPM_MMAX = 512;
+constexpr int KPM_P = 4;  // probe vectors per workgroup, packed as float4 per site
 
-// grid (R, B).  f: B x N occupations, scale[b] = (inv_a, b_shift), diag from f.
+// Reduce-scatter of 8 values per lane over the 64 lanes: afterwards lane l holds the
+// full sum of value number l >> 3 (pairs l^32, l^16 by lane permutes, l^8 by DPP, then
+// an all-reduce over the 8 lanes of a group).
+__device__ __forceinline__ double reduce8(const double (&r)[8]) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    double s1[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const double2 t = pl32_swap(r[a], r[a + 4]);
+        s1[a] = t.x + t.y;  // value a + 4*bit5
+    }
+    double s2[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const double2 t = pl16_swap(s1[a], s1[a + 2]);
+        s2[a] = t.x + t.y;  // value a + 2*bit4 + 4*bit5
+    }
+    const bool hi = (lane >> 3) & 1;
+    const double keep = hi ? s2[1] : s2[0], send = hi ? s2[0] : s2[1];
+    double u = keep + dpp_f64<DPP_ROR8>(send);  // value bit3 + 2*bit4 + 4*bit5
+    u += dpp_f64<DPP_XOR1>(u);
+    u += dpp_f64<DPP_XOR2>(u);
+    u += dpp_f64<DPP_HALF_MIRROR>(u);
+    return u;
+}
+
+__device__ __forceinline__ float4 f4_fma(float h, float4 x, float4 s) {
+    return make_float4(fmaf(h, x.x, s.x), fmaf(h, x.y, s.y), fmaf(h, x.z, s.z), fmaf(h, x.w, s.w));
+}
+__device__ __forceinline__ float f4_get(const float4& v, int p) {
+    return p == 0 ? v.x : p == 1 ? v.y : p == 2 ? v.z : v.w;
+}
+
+// grid (ceil(R / KPM_P), B): workgroup (g, b) runs the probes r = KPM_P g + p of item b.
+// Each thread owns the sites i = tid + u KPM_NT (u < PER) for the whole recursion: their
+// stencil rows (ZM neighbours, zero-padded), diagonal, v_{m-1} and v_m live in VGPRs; v_m
+// of all sites is shared through a double-buffered float4 array in LDS (one barrier per
+// step).  Per step a wave reduces its 2 KPM_P dot products by a reduce-scatter, and wave 0
+// sums the per-wave partials of the previous step in a fixed order.
 // mom_out[(b*R + r)*M + m] = <r|T_m(X)|r> (before the 1/N), FP64.
+template <int ZM>
 __global__ void __launch_bounds__(KPM_NT) k_kpm(int N, int R, int M, const int32_t* __restrict__ f,
                                                 const double* __restrict__ U, const double* __restrict__ mu_c,
                                                 const double* __restrict__ ab, const int32_t* __restrict__ nbr,
@@ -100,97 +140,130 @@ __global__ void __launch_bounds__(KPM_NT) k_kpm(int N, int R, int M, const int32
                                                 const double* __restrict__ hopv, uint64_t seed0,
                                                 int per_item_seed, const int32_t* __restrict__ skip,
                                                 double* __restrict__ mom_out) {
-    __shared__ float vbuf[3][KPM_NMAX];
-    __shared__ double dots[2][KPM_MMAX / 2 + 2][KPM_NT / WAVE];
-    const int r = blockIdx.x, b = blockIdx.y;
+    constexpr int NW = KPM_NT / WAVE;
+    constexpr int PER = KPM_NMAX / KPM_NT;
+    constexpr int NV = 2 * KPM_P;  // dot products per step: <v_m|v_m>, <v_m|v_{m-1}> per probe
+    __shared__ float4 vbuf[2][KPM_NMAX];
+    __shared__ double dots[2][NV][NW];
+    __shared__ double sums[NV][KPM_MMAX / 2 + 2];
+    const int g = blockIdx.x, b = blockIdx.y;
     if (skip && skip[b]) return;  // uniform over the workgroup
     const uint64_t seed = per_item_seed ? seed0 + 0x9E3779B97F4A7C15ull * (uint64_t)(b + 1) : seed0;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const double inv_a = ab[2 * b], shift = ab[2 * b + 1];
-    constexpr int PER = KPM_NMAX / KPM_NT;
-    // per-thread sites and stencil (kept in registers)
-    float dg[PER];
-    int nb[PER][8];
-    float hv[PER][8];
-    int zz[PER];
+    float dg[PER], hv[PER][ZM];
+    int nb[PER][ZM];
+    float4 prev[PER], cur[PER];
+#pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int i = tid + u * KPM_NT;
-        zz[u] = 0;
         dg[u] = 0.0f;
+        prev[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < ZM; ++q) { nb[u][q] = 0; hv[u][q] = 0.0f; }
         if (i < N) {
             dg[u] = (float)((-mu_c[b] + U[b] * (double)f[(size_t)b * N + i] - shift) * inv_a);
-            zz[u] = z[i] < 8 ? z[i] : 8;
-            for (int q = 0; q < 8; ++q) {
-                nb[u][q] = q < zz[u] ? nbr[(size_t)i * zmax + q] : 0;
-                hv[u][q] = q < zz[u] ? (float)(hopv[(size_t)i * zmax + q] * inv_a) : 0.0f;
+            const int zi = z[i];
+#pragma unroll
+            for (int q = 0; q < ZM; ++q)
+                if (q < zi) {
+                    nb[u][q] = nbr[(size_t)i * zmax + q];
+                    hv[u][q] = (float)(hopv[(size_t)i * zmax + q] * inv_a);
+                }
+            float x[KPM_P];
+#pragma unroll
+            for (int p = 0; p < KPM_P; ++p) {
+                const int r = g * KPM_P + p;
+                x[p] = r < R ? rademacher(seed, r, i) : 0.0f;
             }
-            const float x0 = rademacher(seed, r, i);
-            vbuf[0][i] = x0;
+            prev[u] = make_float4(x[0], x[1], x[2], x[3]);
         }
+        vbuf[0][i] = prev[u];
     }
     __syncthreads();
     // v1 = X v0
+#pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int i = tid + u * KPM_NT;
-        if (i >= N) continue;
-        float s = dg[u] * vbuf[0][i];
-        for (int q = 0; q < zz[u]; ++q) s = fmaf(hv[u][q], vbuf[0][nb[u][q]], s);
-        vbuf[1][i] = s;
+        float4 s = make_float4(dg[u] * prev[u].x, dg[u] * prev[u].y, dg[u] * prev[u].z, dg[u] * prev[u].w);
+#pragma unroll
+        for (int q = 0; q < ZM; ++q) s = f4_fma(hv[u][q], vbuf[0][nb[u][q]], s);
+        cur[u] = s;
+        vbuf[1][tid + u * KPM_NT] = s;
     }
     __syncthreads();
-    int p0 = 0, p1 = 1, p2 = 2;  // v_{m-1}, v_m, scratch
     const int half = M / 2;
     for (int m = 1; m <= half; ++m) {
-        // dots of step m: <v_m|v_m>, <v_m|v_{m-1}>  (m = 1 also gives mu_1 = <v_1|v_0>)
-        double d_mm = 0.0, d_mp = 0.0;
-        for (int u = 0; u < PER; ++u) {
-            const int i = tid + u * KPM_NT;
-            if (i >= N) continue;
-            const double a1 = vbuf[p1][i], a0 = vbuf[p0][i];
-            d_mm = fma(a1, a1, d_mm);
-            d_mp = fma(a1, a0, d_mp);
-        }
-        d_mm = wave_sum(d_mm);
-        d_mp = wave_sum(d_mp);
-        if (lane == 0) { dots[0][m][wid] = d_mm; dots[1][m][wid] = d_mp; }
-        if (m == half) break;
-        // v_{m+1} = 2 X v_m - v_{m-1}
-        for (int u = 0; u < PER; ++u) {
-            const int i = tid + u * KPM_NT;
-            if (i >= N) continue;
-            float s = dg[u] * vbuf[p1][i];
-            for (int q = 0; q < zz[u]; ++q) s = fmaf(hv[u][q], vbuf[p1][nb[u][q]], s);
-            vbuf[p2][i] = 2.0f * s - vbuf[p0][i];
+        // dots of step m from the thread's own sites
+        double d[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) d[k] = 0.0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+#pragma unroll
+            for (int p = 0; p < KPM_P; ++p) {
+                const double a1 = f4_get(cur[u], p), a0 = f4_get(prev[u], p);
+                d[p] = fma(a1, a1, d[p]);
+                d[KPM_P + p] = fma(a1, a0, d[KPM_P + p]);
+            }
+        const double red = reduce8(d);
+        if ((lane & 7) == 0) dots[m & 1][lane >> 3][wid] = red;
+        if (m < half) {
+            // v_{m+1} = 2 X v_m - v_{m-1}
+            const float4* src = vbuf[m & 1];
+            float4* dst = vbuf[(m + 1) & 1];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                float4 s = make_float4(dg[u] * cur[u].x, dg[u] * cur[u].y, dg[u] * cur[u].z, dg[u] * cur[u].w);
+#pragma unroll
+                for (int q = 0; q < ZM; ++q) s = f4_fma(hv[u][q], src[nb[u][q]], s);
+                const float4 nx = make_float4(2.0f * s.x - prev[u].x, 2.0f * s.y - prev[u].y,
+                                              2.0f * s.z - prev[u].z, 2.0f * s.w - prev[u].w);
+                prev[u] = cur[u];
+                cur[u] = nx;
+                dst[tid + u * KPM_NT] = nx;
+            }
         }
         __syncthreads();
-        const int t = p0; p0 = p1; p1 = p2; p2 = t;
+        // wave 0 folds the per-wave partials of step m (dots[m & 1] is rewritten at step m + 2,
+        // after the next barrier)
+        if (tid < NV) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += dots[m & 1][tid][w];
+            sums[tid][m] = s;
+        }
     }
     __syncthreads();
-    if (tid < M) {
+    for (int idx = tid; idx < KPM_P * M; idx += KPM_NT) {
+        const int p = idx / M, m = idx % M, r = g * KPM_P + p;
+        if (r >= R) continue;
         // mu_0 = <v0|v0> = N exactly; mu_1 = <v1|v0>
-        const int m = tid;
+        const double mu1 = sums[KPM_P + p][1];
         double val;
-        auto sumw = [&](int k, int h) { double s = 0.0; for (int w = 0; w < KPM_NT / WAVE; ++w) s += dots[k][h][w]; return s; };
-        const double mu1 = sumw(1, 1);
         if (m == 0) val = (double)N;
         else if (m == 1) val = mu1;
-        else if ((m & 1) == 0) val = 2.0 * sumw(0, m / 2) - (double)N;      // 2<v_k|v_k> - mu_0, k = m/2
-        else val = 2.0 * sumw(1, (m + 1) / 2) - mu1;                         // 2<v_{k}|v_{k-1}> - mu_1, k=(m+1)/2
+        else if ((m & 1) == 0) val = 2.0 * sums[p][m / 2] - (double)N;           // 2<v_k|v_k> - mu_0, k = m/2
+        else val = 2.0 * sums[KPM_P + p][(m + 1) / 2] - mu1;                     // 2<v_k|v_{k-1}> - mu_1
         mom_out[((size_t)b * R + r) * M + m] = val;
     }
 }
 
 // Average over the R vectors (fixed order), Jackson damping, logZ = sum_m g_m c_m mu_m.
 // coef[b*M + m] = c_m (Chebyshev coefficient of log(1+e^{-beta(a x + b)})) computed on the host.
-__global__ void k_kpm_finish(int B, int N, int R, int M, const double* __restrict__ mom, const double* __restrict__ coef,
-                             int jackson, const int32_t* __restrict__ skip, double* __restrict__ mu_out,
-                             double* __restrict__ logz) {
-    const int b = blockIdx.x;
-    if (b >= B || threadIdx.x != 0) return;
-    if (skip && skip[b]) return;
+// One workgroup per item; thread t takes the moments m = t, t + 256, ...; the partial sums are
+// combined by a fixed tree, so the result does not depend on scheduling.
+constexpr int KF_NT = 256;
+__global__ void __launch_bounds__(KF_NT) k_kpm_finish(int B, int N, int R, int M, const double* __restrict__ mom,
+                                                      const double* __restrict__ coef, int jackson,
+                                                      const int32_t* __restrict__ skip, double* __restrict__ mu_out,
+                                                      double* __restrict__ logz) {
+    __shared__ double part[KF_NT];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    if (b >= B) return;
+    if (skip && skip[b]) return;  // uniform over the workgroup
     const double PI = 3.14159265358979323846;
     double s = 0.0;
-    for (int m = 0; m < M; ++m) {
+    for (int m = tid; m < M; m += KF_NT) {
         double mu = 0.0;
         for (int r = 0; r < R; ++r) mu += mom[((size_t)b * R + r) * M + m];
         mu /= (double)R * (double)N;
@@ -202,7 +275,13 @@ __global__ void k_kpm_finish(int B, int N, int R, int M, const double* __restric
         if (mu_out) mu_out[(size_t)b * M + m] = mu;
         s += g * coef[(size_t)b * M + m] * mu;
     }
-    logz[b] = s * (double)N;
+    part[tid] = s;
+    __syncthreads();
+    for (int h = KF_NT / 2; h > 0; h >>= 1) {
+        if (tid < h) part[tid] += part[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) logz[b] = part[0] * (double)N;
 }
 
 }  // namespace fkc
@@ -247,11 +326,19 @@ hipError_t fk_launch_kpm(int B, int N, int R, int M, const int32_t* f, const dou
                          const double* ab, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
                          uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,
                          double* mom, double* mu_out, double* logz, hipStream_t st) {
-    if (N > KPM_NMAX || M > KPM_MMAX || M < 2 || (M & 1) || zmax > 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_kpm, dim3(R, B), dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
-                       per_item_seed, skip, mom);
+    if (N > KPM_NMAX || M > KPM_MMAX || M < 2 || (M & 1) || zmax > 8 || R < 1) return hipErrorInvalidValue;
+    const dim3 grid((R + KPM_P - 1) / KPM_P, B);
+    if (zmax <= 4)
+        hipLaunchKernelGGL(k_kpm<4>, grid, dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
+                           per_item_seed, skip, mom);
+    else if (zmax <= 6)
+        hipLaunchKernelGGL(k_kpm<6>, grid, dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
+                           per_item_seed, skip, mom);
+    else
+        hipLaunchKernelGGL(k_kpm<8>, grid, dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
+                           per_item_seed, skip, mom);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(k_kpm_finish, dim3(B), dim3(64), 0, st, B, N, R, M, mom, coef, jackson, skip, mu_out, logz);
+    hipLaunchKernelGGL(k_kpm_finish, dim3(B), dim3(KF_NT), 0, st, B, N, R, M, mom, coef, jackson, skip, mu_out, logz);
     return hipGetLastError();
 }

@@ -50,6 +50,24 @@ __device__ __forceinline__ double2 swap_halves32(double v) {
     int2 lo = make_int2(a[0], b[0]), hi = make_int2(a[1], b[1]);
     return make_double2(*reinterpret_cast<double*>(&lo), *reinterpret_cast<double*>(&hi));
 }
+// Two-register exchanges between lanes l and l^32 (l^16): v_permlane32_swap(x, y)
+// leaves x = [x.lo, y.lo], y = [x.hi, y.hi] over the two halves (rows).
+__device__ __forceinline__ double2 pl32_swap(double x, double y) {
+    int2 a = *reinterpret_cast<int2*>(&x), b = *reinterpret_cast<int2*>(&y);
+    auto lo = __builtin_amdgcn_permlane32_swap(a.x, b.x, false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap(a.y, b.y, false, false);
+    int2 nx = make_int2(lo[0], hi[0]), ny = make_int2(lo[1], hi[1]);
+    return make_double2(*reinterpret_cast<double*>(&nx), *reinterpret_cast<double*>(&ny));
+}
+__device__ __forceinline__ double2 pl16_swap(double x, double y) {
+    int2 a = *reinterpret_cast<int2*>(&x), b = *reinterpret_cast<int2*>(&y);
+    auto lo = __builtin_amdgcn_permlane16_swap(a.x, b.x, false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap(a.y, b.y, false, false);
+    int2 nx = make_int2(lo[0], hi[0]), ny = make_int2(lo[1], hi[1]);
+    return make_double2(*reinterpret_cast<double*>(&nx), *reinterpret_cast<double*>(&ny));
+}
+constexpr int DPP_ROR8 = 0x128;  // row_ror:8, lane i <- lane (i + 8) mod 16 within a 16-lane row
+
 // Butterfly all-reduce over the 64 lanes; every lane ends with the same bits.
 template <class Op>
 __device__ __forceinline__ double wave_reduce(double v, Op op) {

@@ -31,22 +31,6 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     w.y = __builtin_amdgcn_readlane(w.y, lane);
     return *reinterpret_cast<double*>(&w);
 }
-__device__ __forceinline__ double2 pl32_swap(double x, double y) {
-    int2 a = *reinterpret_cast<int2*>(&x), b = *reinterpret_cast<int2*>(&y);
-    auto lo = __builtin_amdgcn_permlane32_swap(a.x, b.x, false, false);
-    auto hi = __builtin_amdgcn_permlane32_swap(a.y, b.y, false, false);
-    int2 nx = make_int2(lo[0], hi[0]), ny = make_int2(lo[1], hi[1]);
-    return make_double2(*reinterpret_cast<double*>(&nx), *reinterpret_cast<double*>(&ny));
-}
-__device__ __forceinline__ double2 pl16_swap(double x, double y) {
-    int2 a = *reinterpret_cast<int2*>(&x), b = *reinterpret_cast<int2*>(&y);
-    auto lo = __builtin_amdgcn_permlane16_swap(a.x, b.x, false, false);
-    auto hi = __builtin_amdgcn_permlane16_swap(a.y, b.y, false, false);
-    int2 nx = make_int2(lo[0], hi[0]), ny = make_int2(lo[1], hi[1]);
-    return make_double2(*reinterpret_cast<double*>(&nx), *reinterpret_cast<double*>(&ny));
-}
-constexpr int DPP_ROR8 = 0x128;  // row_ror:8, lane i <- lane (i + 8) mod 16 within a 16-lane row
-
 // Reduce-scatter of 4 values per lane over the 64 lanes: afterwards every lane holds
 // the full sum of value number 2*bit5 + bit4 of its lane id.
 __device__ __forceinline__ double reduce4(double (&r)[4]) {

@@ -132,43 +132,67 @@ __device__ __forceinline__ float f4_get(const float4& v, int p) {
 // step).  Per step a wave reduces its 2 KPM_P dot products by a reduce-scatter, and wave 0
 // sums the per-wave partials of the previous step in a fixed order.
 // mom_out[(b*R + r)*M + m] = <r|T_m(X)|r> (before the 1/N), FP64.
-template <int ZM>
+struct kpm_hopq {
+    double v[8];
+};
+
+template <int ZM, bool UH>
 __global__ void __launch_bounds__(KPM_NT) k_kpm(int N, int R, int M, const int32_t* __restrict__ f,
                                                 const double* __restrict__ U, const double* __restrict__ mu_c,
                                                 const double* __restrict__ ab, const int32_t* __restrict__ nbr,
                                                 const int32_t* __restrict__ z, int zmax,
-                                                const double* __restrict__ hopv, uint64_t seed0,
+                                                const double* __restrict__ hopv, kpm_hopq hq, uint64_t seed0,
                                                 int per_item_seed, const int32_t* __restrict__ skip,
                                                 double* __restrict__ mom_out) {
     constexpr int NW = KPM_NT / WAVE;
     constexpr int PER = KPM_NMAX / KPM_NT;
     constexpr int NV = 2 * KPM_P;  // dot products per step: <v_m|v_m>, <v_m|v_{m-1}> per probe
-    __shared__ float4 vbuf[2][KPM_NMAX];
-    __shared__ double dots[2][NV][NW];
+    constexpr int HV = UH ? 1 : PER;  // per-site hopping values unless they are the same at every site
+    // The two buffers of v_m interleave in blocks of 64 sites: site i of buffer t is element
+    // vb[lidx(i) + 64 t], so that the buffer is a constant offset (1 KB) in the LDS
+    // instruction and one address register per neighbour serves both.
+    __shared__ float4 vb[2 * KPM_NMAX];
+    __shared__ double dots[8][NV][NW];
     __shared__ double sums[NV][KPM_MMAX / 2 + 2];
+    auto lidx = [](int i) { return ((i >> 6) << 7) + (i & 63); };
     const int g = blockIdx.x, b = blockIdx.y;
     if (skip && skip[b]) return;  // uniform over the workgroup
     const uint64_t seed = per_item_seed ? seed0 + 0x9E3779B97F4A7C15ull * (uint64_t)(b + 1) : seed0;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const double inv_a = ab[2 * b], shift = ab[2 * b + 1];
-    float dg[PER], hv[PER][ZM];
+    float dg[PER], hv[HV][ZM];
     int nb[PER][ZM];
     float4 prev[PER], cur[PER];
+    // The recursion runs on 2X (dg, hv hold twice the diagonal and hopping values, exact in
+    // binary); v1 = X v0 is formed as 0.5 (2X v0), which rounds the same as X v0.
+    if constexpr (UH) {
+#pragma unroll
+        for (int q = 0; q < ZM; ++q) {
+            // wave-uniform: kept in SGPRs (scalar operand of the FMAs)
+            const float h = q < zmax ? 2.0f * (float)(hq.v[q] * inv_a) : 0.0f;
+            hv[0][q] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(h)));
+        }
+    }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int i = tid + u * KPM_NT;
         dg[u] = 0.0f;
         prev[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // unused slots point at the site itself: a site i >= N holds 0 in both buffers for the
+        // whole recursion (its hopping values may be nonzero when they are shared by all sites)
 #pragma unroll
-        for (int q = 0; q < ZM; ++q) { nb[u][q] = 0; hv[u][q] = 0.0f; }
+        for (int q = 0; q < ZM; ++q) {
+            nb[u][q] = lidx(i);
+            if constexpr (!UH) hv[u][q] = 0.0f;
+        }
         if (i < N) {
-            dg[u] = (float)((-mu_c[b] + U[b] * (double)f[(size_t)b * N + i] - shift) * inv_a);
+            dg[u] = 2.0f * (float)((-mu_c[b] + U[b] * (double)f[(size_t)b * N + i] - shift) * inv_a);
             const int zi = z[i];
 #pragma unroll
             for (int q = 0; q < ZM; ++q)
                 if (q < zi) {
-                    nb[u][q] = nbr[(size_t)i * zmax + q];
-                    hv[u][q] = (float)(hopv[(size_t)i * zmax + q] * inv_a);
+                    nb[u][q] = lidx(nbr[(size_t)i * zmax + q]);
+                    if constexpr (!UH) hv[u][q] = 2.0f * (float)(hopv[(size_t)i * zmax + q] * inv_a);
                 }
             float x[KPM_P];
 #pragma unroll
@@ -178,7 +202,7 @@ __global__ void __launch_bounds__(KPM_NT) k_kpm(int N, int R, int M, const int32
             }
             prev[u] = make_float4(x[0], x[1], x[2], x[3]);
         }
-        vbuf[0][i] = prev[u];
+        vb[lidx(i)] = prev[u];
     }
     __syncthreads();
     // v1 = X v0
@@ -186,52 +210,66 @@ __global__ void __launch_bounds__(KPM_NT) k_kpm(int N, int R, int M, const int32
     for (int u = 0; u < PER; ++u) {
         float4 s = make_float4(dg[u] * prev[u].x, dg[u] * prev[u].y, dg[u] * prev[u].z, dg[u] * prev[u].w);
 #pragma unroll
-        for (int q = 0; q < ZM; ++q) s = f4_fma(hv[u][q], vbuf[0][nb[u][q]], s);
+        for (int q = 0; q < ZM; ++q) s = f4_fma(hv[UH ? 0 : u][q], vb[nb[u][q]], s);
+        s = make_float4(0.5f * s.x, 0.5f * s.y, 0.5f * s.z, 0.5f * s.w);
         cur[u] = s;
-        vbuf[1][tid + u * KPM_NT] = s;
+        vb[lidx(tid + u * KPM_NT) + 64] = s;
     }
     __syncthreads();
     const int half = M / 2;
-    for (int m = 1; m <= half; ++m) {
-        // dots of step m from the thread's own sites
+    // One step m: dots of (x = v_m, y = v_{m-1}), then y <- v_{m+1} from buffer offset src
+    // (holding v_m) into buffer offset dst.  Per-wave dot sums go to ring slot m & 7; after
+    // the barrier that closes a block of 4 steps (or the last step) one wave folds the
+    // block's 16 partials per value in a fixed order.  A slot is rewritten 8 steps later,
+    // after 4 more barriers.
+    auto step = [&](float4 (&x)[PER], float4 (&y)[PER], const int src, const int dst, int m) {
         double d[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) d[k] = 0.0;
+        // per site slot: neighbour loads first, the FP64 dot products while they are in
+        // flight, then the stencil, so that LDS and VALU work overlap within the step (the
+        // last step also forms v_{M/2+1}, which is not used: one extra step of 128)
 #pragma unroll
-        for (int u = 0; u < PER; ++u)
+        for (int u = 0; u < PER; ++u) {
+            float4 nv[ZM];
+#pragma unroll
+            for (int q = 0; q < ZM; ++q) nv[q] = vb[nb[u][q] + src];
+            // keep the loads ahead of the FP64 work (where the registers allow it)
+            if constexpr (UH && ZM <= 4) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int p = 0; p < KPM_P; ++p) {
-                const double a1 = f4_get(cur[u], p), a0 = f4_get(prev[u], p);
+                const double a1 = f4_get(x[u], p), a0 = f4_get(y[u], p);
                 d[p] = fma(a1, a1, d[p]);
                 d[KPM_P + p] = fma(a1, a0, d[KPM_P + p]);
             }
+            if constexpr (UH && ZM <= 4) __builtin_amdgcn_sched_barrier(0);
+            const float dgu = dg[u];
+            float4 s = make_float4(fmaf(dgu, x[u].x, -y[u].x), fmaf(dgu, x[u].y, -y[u].y),
+                                   fmaf(dgu, x[u].z, -y[u].z), fmaf(dgu, x[u].w, -y[u].w));
+#pragma unroll
+            for (int q = 0; q < ZM; ++q) s = f4_fma(hv[UH ? 0 : u][q], nv[q], s);
+            y[u] = s;
+            vb[lidx(tid + u * KPM_NT) + dst] = s;
+        }
         const double red = reduce8(d);
-        if ((lane & 7) == 0) dots[m & 1][lane >> 3][wid] = red;
-        if (m < half) {
-            // v_{m+1} = 2 X v_m - v_{m-1}
-            const float4* src = vbuf[m & 1];
-            float4* dst = vbuf[(m + 1) & 1];
+        if ((lane & 7) == 0) dots[m & 7][lane >> 3][wid] = red;
+        __syncthreads();
+        if ((m & 3) == 0 || m == half) {
+            const int m0 = ((m - 1) & ~3) + 1;
+            const int ms = m0 + (lane >> 3), v = lane & 7;
+            if (wid == ((m >> 2) & (NW - 1)) && lane < 32 && ms <= m) {
+                double t = 0.0;
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                float4 s = make_float4(dg[u] * cur[u].x, dg[u] * cur[u].y, dg[u] * cur[u].z, dg[u] * cur[u].w);
-#pragma unroll
-                for (int q = 0; q < ZM; ++q) s = f4_fma(hv[u][q], src[nb[u][q]], s);
-                const float4 nx = make_float4(2.0f * s.x - prev[u].x, 2.0f * s.y - prev[u].y,
-                                              2.0f * s.z - prev[u].z, 2.0f * s.w - prev[u].w);
-                prev[u] = cur[u];
-                cur[u] = nx;
-                dst[tid + u * KPM_NT] = nx;
+                for (int w = 0; w < NW; ++w) t += dots[ms & 7][v][w];
+                sums[v][ms] = t;
             }
         }
-        __syncthreads();
-        // wave 0 folds the per-wave partials of step m (dots[m & 1] is rewritten at step m + 2,
-        // after the next barrier)
-        if (tid < NV) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) s += dots[m & 1][tid][w];
-            sums[tid][m] = s;
-        }
+    };
+    // steps alternate the roles of the register sets and of the two buffers
+    for (int m = 1; m <= half; m += 2) {
+        step(cur, prev, 64, 0, m);          // prev <- v_{m+1}
+        if (m + 1 > half) break;
+        step(prev, cur, 0, 64, m + 1);      // cur <- v_{m+2}
     }
     __syncthreads();
     for (int idx = tid; idx < KPM_P * M; idx += KPM_NT) {
@@ -324,19 +362,24 @@ hipError_t fk_cheb_exact_moments(int N, const int32_t* nbr, const int32_t* z, in
 
 hipError_t fk_launch_kpm(int B, int N, int R, int M, const int32_t* f, const double* U, const double* mu_c,
                          const double* ab, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
-                         uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,
-                         double* mom, double* mu_out, double* logz, hipStream_t st) {
+                         const double* hopq, uint64_t seed, int per_item_seed, const int32_t* skip,
+                         const double* coef, int jackson, double* mom, double* mu_out, double* logz, hipStream_t st) {
     if (N > KPM_NMAX || M > KPM_MMAX || M < 2 || (M & 1) || zmax > 8 || R < 1) return hipErrorInvalidValue;
     const dim3 grid((R + KPM_P - 1) / KPM_P, B);
-    if (zmax <= 4)
-        hipLaunchKernelGGL(k_kpm<4>, grid, dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
-                           per_item_seed, skip, mom);
-    else if (zmax <= 6)
-        hipLaunchKernelGGL(k_kpm<6>, grid, dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
-                           per_item_seed, skip, mom);
-    else
-        hipLaunchKernelGGL(k_kpm<8>, grid, dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, seed,
-                           per_item_seed, skip, mom);
+    kpm_hopq hq{};
+    if (hopq)
+        for (int q = 0; q < zmax; ++q) hq.v[q] = hopq[q];
+#define FK_KPM_LAUNCH(ZM, UH)                                                                                  \
+    hipLaunchKernelGGL((k_kpm<ZM, UH>), grid, dim3(KPM_NT), 0, st, N, R, M, f, U, mu_c, ab, nbr, z, zmax, hopv, hq, \
+                       seed, per_item_seed, skip, mom)
+    if (zmax <= 4) {
+        if (hopq) FK_KPM_LAUNCH(4, true); else FK_KPM_LAUNCH(4, false);
+    } else if (zmax <= 6) {
+        if (hopq) FK_KPM_LAUNCH(6, true); else FK_KPM_LAUNCH(6, false);
+    } else {
+        if (hopq) FK_KPM_LAUNCH(8, true); else FK_KPM_LAUNCH(8, false);
+    }
+#undef FK_KPM_LAUNCH
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     hipLaunchKernelGGL(k_kpm_finish, dim3(B), dim3(KF_NT), 0, st, B, N, R, M, mom, coef, jackson, skip, mu_out, logz);

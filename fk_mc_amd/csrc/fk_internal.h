@@ -107,7 +107,9 @@ hipError_t fk_launch_adjoint(int B, int N, const double* V, const double* fermi,
 hipError_t fk_cheb_exact_moments(int N, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
                                  const double* diag, double a, double b, int M, double* T0, double* T1,
                                  double* parts, double* scal, double* traces, hipStream_t st);
+// hopq (host pointer, zmax values or nullptr): the hopping value of neighbour slot q when it
+// is the same at every site (every site then has zmax neighbours)
 hipError_t fk_launch_kpm(int B, int N, int R, int M, const int32_t* f, const double* U, const double* mu_c,
                          const double* ab, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
-                         uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,
+                         const double* hopq, uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,
                          double* mom, double* mu_out, double* logz, hipStream_t st);

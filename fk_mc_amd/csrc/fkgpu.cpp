@@ -35,6 +35,7 @@ struct fkgpu_ctx {
     std::vector<double> hop;
     std::vector<int32_t> nbr, z;
     std::vector<double> hopv;
+    std::vector<double> hopq;  // per-slot hopping value when it is the same at every site, else empty
     // grow-on-demand scratch for the host-pointer entry points
     struct buf { void* p = nullptr; size_t n = 0; };
     buf b_f, b_beta, b_U, b_mu, b_ev, b_lz, b_ce, b_cf, b_ws;
@@ -167,6 +168,8 @@ void build_hypercubic(fkgpu_ctx* ctx, int kind, int D, int L, double t, double t
     }
 }
 
+static const double* hopq_ptr(const fkgpu_ctx* ctx) { return ctx->hopq.empty() ? nullptr : ctx->hopq.data(); }
+
 int upload_lattice(fkgpu_ctx* ctx) {
     if (ctx->d_hop) hipFree(ctx->d_hop);
     if (ctx->d_hopv) hipFree(ctx->d_hopv);
@@ -185,6 +188,12 @@ int upload_lattice(fkgpu_ctx* ctx) {
             for (int q2 = 0; q2 < ctx->z[i]; ++q2) mult += ctx->nbr[i * ctx->zmax + q2] == j;
             ctx->hopv[i * ctx->zmax + q] = ctx->hop[i * n + j] / mult;
         }
+    ctx->hopq.assign(ctx->hopv.begin(), ctx->hopv.begin() + std::min<size_t>(ctx->zmax, ctx->hopv.size()));
+    for (size_t i = 0; i < n && !ctx->hopq.empty(); ++i) {
+        bool same = ctx->z[i] == ctx->zmax;
+        for (int q = 0; same && q < ctx->zmax; ++q) same = ctx->hopv[i * ctx->zmax + q] == ctx->hopq[q];
+        if (!same) ctx->hopq.clear();
+    }
     FK_HIP(ctx, hipMalloc(&ctx->d_hopv, sizeof(double) * ctx->hopv.size()));
     FK_HIP(ctx, hipMemcpy(ctx->d_hopv, ctx->hopv.data(), sizeof(double) * ctx->hopv.size(), hipMemcpyHostToDevice));
     FK_HIP(ctx, hipMalloc(&ctx->d_hop, sizeof(double) * n * n));
@@ -1006,7 +1015,7 @@ int fkgpu_kpm_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     FK_HIP(ctx, hipMemcpyAsync(dmu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dab, abv.data(), sizeof(double) * 2 * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * batch * M, hipMemcpyHostToDevice, st));
-    FK_HIP(ctx, fk_launch_kpm(batch, N, R, M, df, dU, dmu, dab, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, seed, 0,
+    FK_HIP(ctx, fk_launch_kpm(batch, N, R, M, df, dU, dmu, dab, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, hopq_ptr(ctx), seed, 0,
                               nullptr, dcoef, jackson, dmom, dmuo, dlz, st));
     FK_HIP(ctx, hipMemcpyAsync(logZ, dlz, sizeof(double) * batch, hipMemcpyDeviceToHost, st));
     if (moments) FK_HIP(ctx, hipMemcpyAsync(moments, dmuo, sizeof(double) * batch * M, hipMemcpyDeviceToHost, st));
@@ -1256,7 +1265,7 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
     hipError_t e = fk_launch_kpm_propose(a, ch->k_props, ch->k_fprop, ch->k_skip, st);
     if (e != hipSuccess) return e;
     e = fk_launch_kpm(ch->B, ch->N, ch->kR, ch->kM, ch->k_fprop, ch->U, ch->mu_c, ch->k_ab, ctx->d_nbr, ctx->d_z,
-                      ctx->zmax, ctx->d_hopv, ch->kseed, 1, ch->k_skip, ch->k_coef, ch->kjack, ch->k_mom, nullptr,
+                      ctx->zmax, ctx->d_hopv, hopq_ptr(ctx), ch->kseed, 1, ch->k_skip, ch->k_coef, ch->kjack, ch->k_mom, nullptr,
                       ch->k_lz, st);
     if (e != hipSuccess) return e;
     return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->k_lz, st);
@@ -1438,7 +1447,7 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
     std::vector<int32_t> f32(f8.begin(), f8.end());
     FK_HIP(ctx, hipMemcpyAsync(ch->k_fprop, f32.data(), sizeof(int32_t) * f32.size(), hipMemcpyHostToDevice, st));
     FK_HIP(ctx, fk_launch_kpm(B, N, R, M, ch->k_fprop, ch->U, ch->mu_c, ch->k_ab, ctx->d_nbr, ctx->d_z, ctx->zmax,
-                              ctx->d_hopv, seed, 1, nullptr, ch->k_coef, jackson, ch->k_mom, nullptr, ch->logz, st));
+                              ctx->d_hopv, hopq_ptr(ctx), seed, 1, nullptr, ch->k_coef, jackson, ch->k_mom, nullptr, ch->logz, st));
     FK_HIP(ctx, hipStreamSynchronize(st));
     ch->kpm = 1;
     ch->mixed = 0;

@@ -69,6 +69,53 @@ def test_kpm_moments_and_logz(fk, orc):
     ctx.close()
 
 
+def _csr_hop(L, diag, seed):
+    """Periodic L x L graph, nearest neighbours (and the two diagonals when diag) with
+    random symmetric hopping values: a lattice whose hopping differs from site to site."""
+    rng = np.random.default_rng(seed)
+    n = L * L
+    h = np.zeros((n, n))
+    offs = [(1, 0), (0, 1)] + ([(1, 1), (1, -1)] if diag else [])
+    for x in range(L):
+        for y in range(L):
+            for dx, dy in offs:
+                i, j = x * L + y, ((x + dx) % L) * L + (y + dy) % L
+                h[i, j] = h[j, i] = -rng.uniform(0.5, 1.5)
+    return h
+
+
+@pytest.mark.parametrize("kind", ["csr4", "csr8", "triangular", "cubic3d"])
+def test_kpm_moments_lattices(fk, kind):
+    """Stochastic-trace moments on lattices with 4, 6 and 8 neighbour slots, with the same
+    hopping at every site and with hopping that varies from site to site: against the exact
+    spectral moments within the estimator's error."""
+    print("case kpm moments", kind, flush=True)
+    ctx = fk.Context(0)
+    if kind == "csr4":
+        ctx.set_lattice_csr(_csr_hop(32, False, 1))
+    elif kind == "csr8":
+        ctx.set_lattice_csr(_csr_hop(32, True, 2))
+    elif kind == "triangular":
+        ctx.set_lattice(fk.LATTICE_TRIANGULAR, 2, 32, 1.0, 0.6)
+    else:
+        ctx.set_lattice(fk.LATTICE_HYPERCUBIC, 3, 10, 1.0, 0.0)
+    n = ctx.N
+    rng = np.random.default_rng(8)
+    f = rng.integers(0, 2, (2, n)).astype(np.int32)
+    R, M, beta, U = 64, 128, 4.0, 2.0
+    out = ctx.kpm_logz(f, beta, U, U / 2, R=R, M=M, seed=11, jackson=True)
+    tol = 6 * np.sqrt(2.0 / (n * R))
+    for b in range(2):
+        ed = ctx.eig_logz(f[b:b + 1], beta, U, U / 2)
+        x = (ed["evals"][0] - out["b"][b]) / out["a"][b]
+        assert np.abs(x).max() < 1.0
+        exact = np.array([np.mean(np.cos(m * np.arccos(x))) for m in range(M)])
+        assert abs(out["moments"][b][0] - 1.0) < 1e-15
+        assert np.abs(out["moments"][b] - exact).max() < tol
+        assert abs(out["logZ"][b] - ed["logZ"][0]) / ed["logZ"][0] < 3e-2
+    ctx.close()
+
+
 def test_kpm_chains(fk, orc):
     """Chebyshev-move chains (src/moves_chebyshev.cpp) with the stochastic trace:
     draws follow the reference order, dlogZ of each solved proposal agrees with

@@ -91,9 +91,11 @@ size_t fk_solver_ws_doubles(int kind, int N);
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
 
+// scratch: fk_evecs_scratch_doubles(B, N) doubles (at least 3 B N^2)
 hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,
                            const double* tau, double* Z, double* scratch, hipStream_t st,
                            const int32_t* skip = nullptr);
+size_t fk_evecs_scratch_doubles(int B, int N);
 hipError_t fk_launch_ipr(int B, int N, const double* V, double* out, hipStream_t st);
 hipError_t fk_launch_stiffness(int B, int N, int stride0, int L0, double t, const double* V, const double* ev,
                                const double* fermi, int nw, const double* wgrid, double offset, double* JV,

@@ -7,12 +7,19 @@
 //                   eigenvalues closer than ortol form clusters whose members
 //                   are computed in rounds and orthogonalised (modified
 //                   Gram-Schmidt) against the earlier members.
-//   k_backtransform V = Q Z with Q = H_0 H_1 ... H_{N-3} the Householder
-//                   reflectors stored by the tridiagonaliser: Z <- H_k Z for
-//                   k = N-3 .. 0, one thread per column of Z (coalesced rows).
+//   k_bt_tfac, k_bt_apply
+//                   V = Q Z with Q = H_0 H_1 ... H_{N-3} the Householder
+//                   reflectors stored by the tridiagonaliser, applied in
+//                   blocks of 16 (compact WY: H_k0 ... H_k0+15 = I - Y T Y^T,
+//                   Golub & Van Loan 5.1.7 / LAPACK dlarft, forward), last block
+//                   first.  A workgroup holds a panel of 16 or 32 columns of Z
+//                   in MFMA accumulator registers for the whole product; per
+//                   block W = Y^T Z and Z -= Y (T W) are 16x16x4 FP64 MFMAs.
 // Layouts: Z and V row-major N x N per matrix (column k = vector k); the
 // per-thread solve scratch is [i][j] so that a step i is one coalesced row.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "fk_internal.h"
@@ -192,7 +199,168 @@ __global__ void __launch_bounds__(1024) k_backtransform(int B, int N, const doub
     }
 }
 
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int BT_NB = 16;  // reflectors per block
+
+__device__ __forceinline__ v4d mfma64(double a, double b, v4d c) {
+    // 16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16]; accumulator
+    // register r of lane l is D[(l/16) + 4r][l%16]
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// T factor of each block: grid (B, nblk), 256 threads.  Reflector k (row k of vh) is
+// zero for k >= N - 2.  T[a][c] (upper triangular) with H_k0 ... H_k0+15 = I - Y T Y^T:
+// T[c][c] = tau_c, T[0:c, c] = -tau_c T[0:c, 0:c] (Y^T y_c)[0:c].
+__global__ void __launch_bounds__(256) k_bt_tfac(int B, int N, int nblk, const double* __restrict__ vg,
+                                                 const double* __restrict__ taug, double* __restrict__ Tg,
+                                                 const int32_t* __restrict__ skip) {
+    constexpr int CH = 256, LDY = CH + 1;
+    __shared__ double ych[BT_NB * LDY];
+    __shared__ double G[BT_NB][BT_NB + 1];
+    __shared__ double T[BT_NB][BT_NB + 1];
+    const int b = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    if (b >= B || (skip && skip[b])) return;
+    const int k0 = s * BT_NB;
+    const double* V = vg + (size_t)b * N * N;
+    const int a = tid / BT_NB, c = tid % BT_NB;
+    double g = 0.0;
+    for (int i0 = 0; i0 < N; i0 += CH) {
+        __syncthreads();
+        for (int idx = tid; idx < BT_NB * CH; idx += 256) {
+            const int m = idx / CH, i = i0 + idx % CH, k = k0 + m;
+            ych[m * LDY + idx % CH] = (k < N - 2 && i < N) ? V[(size_t)k * N + i] : 0.0;
+        }
+        __syncthreads();
+        for (int i = 0; i < CH; ++i) g = fma(ych[a * LDY + i], ych[c * LDY + i], g);
+    }
+    G[a][c] = g;
+    if (tid < BT_NB) {
+        const int k = k0 + tid;
+        for (int q = 0; q < BT_NB; ++q) T[q][tid] = 0.0;
+        T[tid][tid] = k < N - 2 ? taug[(size_t)b * N + k] : 0.0;
+    }
+    __syncthreads();
+    for (int cc = 1; cc < BT_NB; ++cc) {
+        if (tid < cc) {
+            double t = 0.0;
+            for (int m = tid; m < cc; ++m) t = fma(T[tid][m], G[m][cc], t);
+            T[tid][cc] = -T[cc][cc] * t;
+        }
+        __syncthreads();
+    }
+    Tg[((size_t)b * nblk + s) * BT_NB * BT_NB + tid] = T[tid / BT_NB][tid % BT_NB];
+}
+
+// Z <- Q Z for a panel of 16 CT columns: grid (B, ceil(N / (16 CT))), 1024 threads.
+// Wave w holds the row tiles R = 16 w + 256 t (t < RT) of the panel as accumulators
+// acc[t][c] (rows R + (l/16) + 4r, column 16 c + l%16).  Per block of reflectors
+// (last first): P = Y^T Z over the wave's rows (the accumulator register r is the
+// MFMA B operand of rows R + 4r .. R + 4r + 3), partials summed over the waves in LDS,
+// W = T P, then Z -= Y W.  Tiles wholly above row k0 + 1 are skipped (Y is zero there).
+template <int RT, int CT>
+__global__ void __launch_bounds__(1024) k_bt_apply(int B, int N, int nblk, const double* __restrict__ vg,
+                                                   const double* __restrict__ Tg, double* __restrict__ Zg,
+                                                   const int32_t* __restrict__ skip) {
+    constexpr int NW = 16;
+    __shared__ double part[NW][CT][256];
+    __shared__ double Ps[CT][BT_NB][BT_NB + 1];
+    __shared__ double Ws[CT][BT_NB][BT_NB + 1];
+    __shared__ double Ts[BT_NB][BT_NB + 1];
+    const int b = blockIdx.x;
+    if (b >= B || (skip && skip[b])) return;
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    const int c0 = blockIdx.y * 16 * CT;
+    const double* V = vg + (size_t)b * N * N;
+    double* Z = Zg + (size_t)b * N * N;
+    const int lr = l >> 4, lc = l & 15;
+    v4d acc[RT][CT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * w + 256 * t + lr + 4 * r, col = c0 + 16 * c + lc;
+                acc[t][c][r] = (row < N && col < N) ? Z[(size_t)row * N + col] : 0.0;
+            }
+    for (int s = nblk - 1; s >= 0; --s) {
+        const int k0 = s * BT_NB;
+        // P = Y^T Z over this wave's rows
+        v4d p[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) p[c] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int R = 16 * w + 256 * t;
+            if (R + 15 <= k0) continue;  // uniform over the wave
+            const int k = k0 + lc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = R + 4 * r + lr;
+                const double y = (k < N - 2 && row < N) ? V[(size_t)k * N + row] : 0.0;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) p[c] = mfma64(y, acc[t][c][r], p[c]);
+            }
+            if constexpr (RT > 2) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) part[w][c][(lr + 4 * r) * BT_NB + lc] = p[c][r];
+        if (tid < BT_NB * BT_NB) Ts[tid / BT_NB][tid % BT_NB] = Tg[((size_t)b * nblk + s) * BT_NB * BT_NB + tid];
+        __syncthreads();
+        // sum of the wave partials (fixed order)
+        if (tid < CT * 256) {
+            const int c = tid >> 8, e = tid & 255;
+            double v = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) v += part[ww][c][e];
+            Ps[c][e / BT_NB][e % BT_NB] = v;
+        }
+        __syncthreads();
+        // W = T P (T upper triangular)
+        if (tid < CT * 256) {
+            const int c = tid >> 8, a = (tid >> 4) & 15, j = tid & 15;
+            double v = 0.0;
+            for (int m = a; m < BT_NB; ++m) v = fma(Ts[a][m], Ps[c][m][j], v);
+            Ws[c][a][j] = v;
+        }
+        __syncthreads();
+        // Z -= Y W: A = -Y[R + l%16][k0 + 4q + l/16], B = W[4q + l/16][l%16]
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int R = 16 * w + 256 * t;
+            if (R + 15 <= k0) continue;
+            const int row = R + lc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = k0 + 4 * q + lr;
+                const double y = (k < N - 2 && row < N) ? -V[(size_t)k * N + row] : 0.0;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) acc[t][c] = mfma64(y, Ws[c][4 * q + lr][lc], acc[t][c]);
+            }
+            if constexpr (RT > 2) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * w + 256 * t + lr + 4 * r, col = c0 + 16 * c + lc;
+                if (row < N && col < N) Z[(size_t)row * N + col] = acc[t][c][r];
+            }
+}
+
 }  // namespace fkv
+
+size_t fk_evecs_scratch_doubles(int B, int N) {
+    // the inverse-iteration LU (3 N^2 per matrix), then the T factors (256 per block of 16 reflectors)
+    const size_t nblk = N > 2 ? (size_t)(N - 2 + fkv::BT_NB - 1) / fkv::BT_NB : 0;
+    return (size_t)B * std::max<size_t>(3 * (size_t)N * N, nblk * fkv::BT_NB * fkv::BT_NB);
+}
 
 hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,
                            const double* tau, double* Z, double* scratch, hipStream_t st, const int32_t* skip) {
@@ -201,7 +369,23 @@ hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const
                        skip);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
-    const int nt = N <= 1024 ? ((1024 / N) * N > 0 ? (1024 / N) * N : N) : 1024;
-    hipLaunchKernelGGL(fkv::k_backtransform, dim3(B), dim3(nt <= 1024 ? nt : 1024), 0, st, B, N, vh, tau, Z, skip);
+    if (N < 3) return hipSuccess;  // no reflectors
+    // T factors go to the solve scratch (B x 3 x N x N doubles), free once the vectors of T are formed
+    const int nblk = (N - 2 + fkv::BT_NB - 1) / fkv::BT_NB;
+    hipLaunchKernelGGL(fkv::k_bt_tfac, dim3(B, nblk), dim3(256), 0, st, B, N, nblk, vh, tau, scratch, skip);
+    r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    if (N <= 256)
+        hipLaunchKernelGGL((fkv::k_bt_apply<1, 2>), dim3(B, (N + 31) / 32), dim3(1024), 0, st, B, N, nblk, vh, scratch,
+                           Z, skip);
+    else if (N <= 512)
+        hipLaunchKernelGGL((fkv::k_bt_apply<2, 2>), dim3(B, (N + 31) / 32), dim3(1024), 0, st, B, N, nblk, vh, scratch,
+                           Z, skip);
+    else if (N <= 1024)
+        hipLaunchKernelGGL((fkv::k_bt_apply<4, 1>), dim3(B, (N + 15) / 16), dim3(1024), 0, st, B, N, nblk, vh, scratch,
+                           Z, skip);
+    else
+        hipLaunchKernelGGL((fkv::k_bt_apply<8, 1>), dim3(B, (N + 15) / 16), dim3(1024), 0, st, B, N, nblk, vh, scratch,
+                           Z, skip);
     return hipGetLastError();
 }

@@ -375,7 +375,7 @@ static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
         if (N > 1024) return fail(ctx, FKGPU_EUNSUPPORTED, "eigenvectors are implemented for N <= 1024");
         if ((r = ensure(ctx, ctx->b_vh, BNN, &vh)) || (r = ensure(ctx, ctx->b_tau, BN, &tau)) ||
             (r = ensure(ctx, ctx->b_d, BN, &dd)) || (r = ensure(ctx, ctx->b_e, BN, &ee)) ||
-            (r = ensure(ctx, ctx->b_Z, BNN, &Z)) || (r = ensure(ctx, ctx->b_scr, 3 * BNN, &scr)))
+            (r = ensure(ctx, ctx->b_Z, BNN, &Z)) || (r = ensure(ctx, ctx->b_scr, fk_evecs_scratch_doubles(batch, N), &scr)))
             return r;
         a.vout = vh;
         a.tauout = tau;
@@ -432,7 +432,7 @@ static hipError_t mixed_alloc(fkgpu_ctx* ctx, int B, int N, mixed_bufs** out) {
     chk(dalloc(&m->evh, BN));
     chk(dalloc(&m->lzh, B));
     chk(dalloc(&m->X, BNN));
-    chk(dalloc(&m->scr, 3 * BNN));
+    chk(dalloc(&m->scr, fk_evecs_scratch_doubles(B, N)));
     chk(dalloc(&m->Q, BN * fk_mixed_cluster_max()));
     chk(dalloc(&m->ev, BN));
     chk(dalloc(&m->lz, B));
@@ -1305,7 +1305,7 @@ int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
         chk(dalloc(&ch->fu_tau, BN));
         chk(dalloc(&ch->fu_d, BN));
         chk(dalloc(&ch->fu_e, BN));
-        chk(dalloc(&ch->fu_scr, 3 * BNN));
+        chk(dalloc(&ch->fu_scr, fk_evecs_scratch_doubles(B, N)));
         chk(dalloc(&ch->fu_upd, B));
         chk(dalloc(&ch->fu_f32, BN));
         if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "rank-one update buffers");

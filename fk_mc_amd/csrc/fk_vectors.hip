@@ -29,36 +29,61 @@ namespace fkv {
 
 using namespace fks;
 
-// Solve (T - lam) z = x in place (x overwritten by z).  T = (d, e).  The LU
-// with partial pivoting is formed on the fly; U's three diagonals go to
-// u0/u1/u2 (stride ld, this thread's column).  Zero pivots are replaced by
-// +-tiny so the solve always completes (the usual inverse-iteration rule).
-__device__ void tri_solve(const double* __restrict__ d, const double* __restrict__ e, int N, double lam, double tiny,
-                          double* x, double* u0, double* u1, double* u2, size_t ld) {
-    // forward elimination: rows k, k+1
+// Deterministic start vector of inverse iteration for eigenvalue j (an LCG sequence).
+struct start_vec {
+    uint32_t h;
+    __device__ explicit start_vec(int j) : h(2654435761u * (uint32_t)(j + 1)) {}
+    __device__ double next() {
+        h = h * 1664525u + 1013904223u;
+        return 1.0 + 0.5 * ((double)(h >> 8) / 16777216.0 - 0.5);
+    }
+};
+
+// Solve (T - lam) z = s x in place (x overwritten by z), T = (d, e), with the LU of
+// T - lam by partial pivoting formed on the fly (U's three diagonals to u0/u1/u2, stride
+// ld, this thread's column; zero pivots replaced by +-tiny, the usual inverse-iteration
+// rule).  With gen the right-hand side is the start vector of eigenvalue j and x is not
+// read.  Loads are issued in chunks of CH ahead of their use (the recurrences themselves
+// are sequential).  Returns ||z||^2; *zbig gets the entry of largest magnitude.
+constexpr int TS_CH = 4;
+__device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int N, double lam,
+                            double tiny, double* x, double s, bool gen, int j, double* u0, double* u1, double* u2,
+                            size_t ld, double* zbig) {
+    start_vec sv(j);
     double a = d[0] - lam;                 // current pivot-row diagonal
     double b = N > 1 ? e[0] : 0.0;         // current pivot-row superdiagonal
-    double xk = x[0];
-    for (int k = 0; k + 1 < N; ++k) {
-        const double c = e[k];             // subdiagonal below the pivot
-        const double dn = d[k + 1] - lam;  // next diagonal
-        const double en = (k + 2 < N) ? e[k + 1] : 0.0;
-        double xn = x[(size_t)(k + 1) * ld];
-        if (fabs(a) >= fabs(c)) {
-            double aa = a == 0.0 ? tiny : a;
-            const double m = c / aa;
-            u0[(size_t)k * ld] = aa; u1[(size_t)k * ld] = b; u2[(size_t)k * ld] = 0.0;
-            x[(size_t)k * ld] = xk;
-            a = dn - m * b;
-            b = en;
-            xk = xn - m * xk;
-        } else {
-            const double m = a / c;
-            u0[(size_t)k * ld] = c; u1[(size_t)k * ld] = dn; u2[(size_t)k * ld] = en;
-            x[(size_t)k * ld] = xn;
-            a = b - m * dn;
-            b = -m * en;
-            xk = xk - m * xn;
+    double xk = gen ? sv.next() : s * x[0];
+    for (int k0 = 0; k0 + 1 < N; k0 += TS_CH) {
+        double xb[TS_CH];
+#pragma unroll
+        for (int q = 0; q < TS_CH; ++q) {
+            const int i = k0 + 1 + q;
+            xb[q] = i < N ? (gen ? sv.next() : s * x[(size_t)i * ld]) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < TS_CH; ++q) {
+            const int k = k0 + q;
+            if (k + 1 >= N) break;
+            const double c = e[k];             // subdiagonal below the pivot
+            const double dn = d[k + 1] - lam;  // next diagonal
+            const double en = (k + 2 < N) ? e[k + 1] : 0.0;
+            const double xn = xb[q];
+            if (fabs(a) >= fabs(c)) {
+                const double aa = a == 0.0 ? tiny : a;
+                const double m = c / aa;
+                u0[(size_t)k * ld] = aa; u1[(size_t)k * ld] = b; u2[(size_t)k * ld] = 0.0;
+                x[(size_t)k * ld] = xk;
+                a = dn - m * b;
+                b = en;
+                xk = xn - m * xk;
+            } else {
+                const double m = a / c;
+                u0[(size_t)k * ld] = c; u1[(size_t)k * ld] = dn; u2[(size_t)k * ld] = en;
+                x[(size_t)k * ld] = xn;
+                a = b - m * dn;
+                b = -m * en;
+                xk = xk - m * xn;
+            }
         }
     }
     if (a == 0.0) a = tiny;
@@ -66,25 +91,42 @@ __device__ void tri_solve(const double* __restrict__ d, const double* __restrict
     x[(size_t)(N - 1) * ld] = xk;
     // back substitution with U (diagonals u0, u1, u2)
     double z1 = 0.0, z2 = 0.0;  // z[k+1], z[k+2]
-    for (int k = N - 1; k >= 0; --k) {
-        double s = x[(size_t)k * ld];
-        if (k + 1 < N) s -= u1[(size_t)k * ld] * z1;
-        if (k + 2 < N) s -= u2[(size_t)k * ld] * z2;
-        const double z = s / u0[(size_t)k * ld];
-        x[(size_t)k * ld] = z;
-        z2 = z1;
-        z1 = z;
+    double nrm2 = 0.0, big = 0.0;
+    for (int k0 = N - 1; k0 >= 0; k0 -= TS_CH) {
+        double xb[TS_CH], p0[TS_CH], p1[TS_CH], p2[TS_CH];
+#pragma unroll
+        for (int q = 0; q < TS_CH; ++q) {
+            const int k = k0 - q;
+            if (k >= 0) {
+                xb[q] = x[(size_t)k * ld];
+                p0[q] = u0[(size_t)k * ld];
+                p1[q] = u1[(size_t)k * ld];
+                p2[q] = u2[(size_t)k * ld];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TS_CH; ++q) {
+            const int k = k0 - q;
+            if (k < 0) break;
+            double t = xb[q];
+            if (k + 1 < N) t -= p1[q] * z1;
+            if (k + 2 < N) t -= p2[q] * z2;
+            const double z = t / p0[q];
+            x[(size_t)k * ld] = z;
+            nrm2 = fma(z, z, nrm2);
+            if (fabs(z) > fabs(big)) big = z;
+            z2 = z1;
+            z1 = z;
+        }
     }
-}
-
-__device__ inline double col_norm2(const double* x, int N, size_t ld) {
-    double s = 0.0;
-    for (int i = 0; i < N; ++i) s = fma(x[(size_t)i * ld], x[(size_t)i * ld], s);
-    return s;
+    *zbig = big;
+    return nrm2;
 }
 
 // One workgroup per matrix.  d, e, ev: B x N.  Z: B x N x N output.
-// scratch: B x 3 x N x N.
+// scratch: B x 3 x N x N.  Eigenvalues closer than ortol form clusters; members are
+// computed in rounds (position in the cluster) and orthogonalised against the earlier
+// members.  Isolated eigenvalues take two solves, cluster members three.
 __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* __restrict__ dg,
                                                     const double* __restrict__ eg, const double* __restrict__ evg,
                                                     double* __restrict__ Zg, double* __restrict__ scr,
@@ -128,34 +170,30 @@ __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* 
         for (int j = tid; j < N; j += NT) {
             const int pos = j - cstart[j];
             if (pos != round) continue;
+            const bool isolated = pos == 0 && (j + 1 == N || cstart[j + 1] != j);
             double* x = Z + j;  // column j, stride N
-            // deterministic start vector, different per eigenvalue
-            uint32_t h = 2654435761u * (uint32_t)(j + 1);
-            for (int i = 0; i < N; ++i) {
-                h = h * 1664525u + 1013904223u;
-                x[(size_t)i * N] = 1.0 + 0.5 * ((double)(h >> 8) / 16777216.0 - 0.5);
-            }
-            for (int it = 0; it < 3; ++it) {
-                const double nrm = sqrt(col_norm2(x, N, N));
-                const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
-                for (int i = 0; i < N; ++i) x[(size_t)i * N] *= sc;
-                tri_solve(d, e, N, ev[j], tiny, x, u0 + j, u1 + j, u2 + j, N);
+            double nrm2 = 0.0, big = 0.0;
+            const int its = isolated ? 2 : 3;
+            for (int it = 0; it < its; ++it) {
+                const double sc = it == 0 ? 1.0 : (nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0);
+                nrm2 = tri_solve(d, e, N, ev[j], tiny, x, sc, it == 0, j, u0 + j, u1 + j, u2 + j, N, &big);
                 // orthogonalise against earlier members of the cluster
                 for (int p = cstart[j]; p < j; ++p) {
                     const double* y = Z + p;
                     double dot = 0.0;
                     for (int i = 0; i < N; ++i) dot = fma(x[(size_t)i * N], y[(size_t)i * N], dot);
-                    for (int i = 0; i < N; ++i) x[(size_t)i * N] = fma(-dot, y[(size_t)i * N], x[(size_t)i * N]);
+                    nrm2 = 0.0;
+                    big = 0.0;
+                    for (int i = 0; i < N; ++i) {
+                        const double v = fma(-dot, y[(size_t)i * N], x[(size_t)i * N]);
+                        x[(size_t)i * N] = v;
+                        nrm2 = fma(v, v, nrm2);
+                        if (fabs(v) > fabs(big)) big = v;
+                    }
                 }
             }
-            const double nrm = sqrt(col_norm2(x, N, N));
-            const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
-            // sign convention: largest-magnitude component positive
-            double big = 0.0;
-            for (int i = 0; i < N; ++i) {
-                const double v = x[(size_t)i * N];
-                if (fabs(v) > fabs(big)) big = v;
-            }
+            // unit norm, largest-magnitude component positive
+            const double sc = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0;
             const double sg = big < 0.0 ? -sc : sc;
             for (int i = 0; i < N; ++i) x[(size_t)i * N] *= sg;
         }

@@ -386,7 +386,8 @@ __device__ __forceinline__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], t
 //      (the Sturm count says which side it is on); a step that leaves the
 //      bracket is replaced by bisection;
 //   3. once a step is tiny, one two-shift Sturm pass checks that l_j lies in
-//      (s - t, s + t]; if not, the bracket is updated and the loop goes on.
+//      (s - t, s + t]; if not, the bracket is updated and the iteration goes on
+//      from the window's edge on the eigenvalue's side.
 // de: LDS, >= 2*N doubles; cnt: LDS, >= NT+2 ints; scratch >= 3*NT/WAVE.
 // ---------------------------------------------------------------------------
 struct sturm_res {
@@ -537,9 +538,11 @@ __device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e
                             int ca, cb;
                             sturm_count2(de, N, sn - vtol, sn + vtol, pivmin, ca, cb);
                             if (ca <= j && cb > j) { res = sn; break; }
-                            if (ca > j) { u = sn - vtol; cu = ca; }
-                            else if (cb <= j) { l = sn + vtol; cl = cb; }
-                            next = 0.5 * (l + u);
+                            // the eigenvalue lies just outside the window: go on from the window's
+                            // edge on its side (a restart from the bracket midpoint costs 2-3 passes)
+                            if (ca > j) { u = sn - vtol; cu = ca; next = u; }
+                            else if (cb <= j) { l = sn + vtol; cl = cb; next = l; }
+                            else next = 0.5 * (l + u);
                         } else {
                             next = sn;
                         }

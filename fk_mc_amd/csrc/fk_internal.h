@@ -80,7 +80,9 @@ hipError_t fk_launch_mixed_refine(int B, int N, const double* X, const int32_t* 
 int fk_mixed_cluster_max();
 hipError_t fk_launch_fu_step(const sweep_args& a, const double* V, double* Q1, double* Q2, int32_t* upd,
                              double* T, double* Vout, hipStream_t st);
-hipError_t fk_launch_f8to32(int n, const uint8_t* f8, int32_t* f32, hipStream_t st);
+hipError_t fk_launch_f8to32(int n, const uint8_t* f8, int32_t* f32, hipStream_t st, const int32_t* skip0 = nullptr);
+// rank-one chains: count += 1 on the device, skip[b] = (count % period != 0) for b < B
+hipError_t fk_launch_fu_tick(int B, int period, long long* count, int32_t* skip, hipStream_t st);
 int fk_fu_nmax();
 
 int fk_solver_kind(int N);          // -1: unsupported size

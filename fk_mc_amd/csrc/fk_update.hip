@@ -427,9 +427,26 @@ __global__ void __launch_bounds__(256) k_fu_gemm(int B, int N, const int32_t* __
     store(c11, I0 + 32 * wr + 16, J0 + 32 * wc + 16);
 }
 
-__global__ void k_fu_f8to32(int n, const uint8_t* __restrict__ f8, int32_t* __restrict__ f32) {
+// skip0 (nullable): nothing to do when skip0[0] is set (the refresh is not due)
+__global__ void k_fu_f8to32(int n, const uint8_t* __restrict__ f8, int32_t* __restrict__ f32,
+                            const int32_t* __restrict__ skip0) {
+    if (skip0 && skip0[0]) return;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < (size_t)n) f32[i] = f8[i];
+}
+
+// Move-step counter of a rank-one chain set: count += 1; the periodic re-decomposition
+// is due when count is a multiple of period; skip[b] = !due for every chain.  One block.
+__global__ void __launch_bounds__(256) k_fu_tick(int B, int period, long long* __restrict__ count,
+                                                 int32_t* __restrict__ skip) {
+    __shared__ int due;
+    if (threadIdx.x == 0) {
+        const long long c = count[0] + 1;
+        count[0] = c;
+        due = (c % period) == 0;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < B; b += blockDim.x) skip[b] = !due;
 }
 
 }  // namespace fku
@@ -450,8 +467,14 @@ hipError_t fk_launch_fu_step(const sweep_args& a, const double* V, double* Q1, d
     hipLaunchKernelGGL(fku::k_fu_gemm, dim3(nbt * nbt, a.B), dim3(256), 0, st, a.B, a.N, upd, 2, T, Q2, Vout);
     return hipGetLastError();
 }
-hipError_t fk_launch_f8to32(int n, const uint8_t* f8, int32_t* f32, hipStream_t st) {
-    hipLaunchKernelGGL(fku::k_fu_f8to32, dim3((n + 255) / 256), dim3(256), 0, st, n, f8, f32);
+hipError_t fk_launch_fu_tick(int B, int period, long long* count, int32_t* skip, hipStream_t st) {
+    if (period < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fku::k_fu_tick, dim3(1), dim3(256), 0, st, B, period, count, skip);
+    return hipGetLastError();
+}
+
+hipError_t fk_launch_f8to32(int n, const uint8_t* f8, int32_t* f32, hipStream_t st, const int32_t* skip0) {
+    hipLaunchKernelGGL(fku::k_fu_f8to32, dim3((n + 255) / 256), dim3(256), 0, st, n, f8, f32, skip0);
     return hipGetLastError();
 }
 int fk_fu_nmax() { return fku::UNMAX; }

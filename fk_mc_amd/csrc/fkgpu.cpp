@@ -77,10 +77,11 @@ struct fkgpu_chains {
     // rank-one update mode: eigenvectors kept on the device, full re-decomposition
     // every fu_refresh steps
     int fu = 0, fu_refresh = 0;
-    long long fu_count = 0;
     double *fu_V = nullptr, *fu_Q1 = nullptr, *fu_Q2 = nullptr, *fu_T = nullptr;
     double *fu_vh = nullptr, *fu_tau = nullptr, *fu_d = nullptr, *fu_e = nullptr, *fu_scr = nullptr;
     int32_t *fu_upd = nullptr, *fu_f32 = nullptr;
+    int32_t* fu_skip = nullptr;      // per chain: the periodic re-decomposition is not due this step
+    long long* fu_count = nullptr;   // move steps since the rank-one mode was set (device counter)
 };
 
 namespace {
@@ -1045,7 +1046,7 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     if (ch->ev1) hipEventDestroy(ch->ev1);
     mixed_free(ch->mx);
     void* fptrs[] = {ch->fu_V, ch->fu_Q1, ch->fu_Q2, ch->fu_T, ch->fu_vh, ch->fu_tau, ch->fu_d, ch->fu_e, ch->fu_scr,
-                     ch->fu_upd, ch->fu_f32};
+                     ch->fu_upd, ch->fu_f32, ch->fu_skip, ch->fu_count};
     for (void* p : fptrs)
         if (p) hipFree(p);
     fkgpu_ctx* ctx = ch->ctx;
@@ -1219,10 +1220,12 @@ int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W) {
 // trace / accept for the Chebyshev (KPM) mode.
 // Full eigendecomposition of every chain's current configuration into spectrum, logZ
 // and the eigenvector matrices of the rank-one update mode.
-static hipError_t fu_refresh_launch(fkgpu_chains* ch, hipStream_t st) {
+// Full re-decomposition of the chains' current configurations; with skip, only of the
+// chains whose skip flag is clear (all or none: the flags come from the step counter).
+static hipError_t fu_refresh_launch(fkgpu_chains* ch, hipStream_t st, const int32_t* skip = nullptr) {
     fkgpu_ctx* ctx = ch->ctx;
     const int B = ch->B, N = ch->N;
-    hipError_t e = fk_launch_f8to32(B * N, ch->f, ch->fu_f32, st);
+    hipError_t e = fk_launch_f8to32(B * N, ch->f, ch->fu_f32, st, skip);
     if (e != hipSuccess) return e;
     eig_args a{};
     a.B = B;
@@ -1240,9 +1243,11 @@ static hipError_t fu_refresh_launch(fkgpu_chains* ch, hipStream_t st) {
     a.tauout = ch->fu_tau;
     a.d_out = ch->fu_d;
     a.e_out = ch->fu_e;
+    a.skip = skip;
     e = fk_launch_eig(ch->kind, a, ch->grid, st);
     if (e != hipSuccess) return e;
-    return fk_launch_evecs(B, N, ch->fu_d, ch->fu_e, ch->spectrum, ch->fu_vh, ch->fu_tau, ch->fu_V, ch->fu_scr, st);
+    return fk_launch_evecs(B, N, ch->fu_d, ch->fu_e, ch->spectrum, ch->fu_vh, ch->fu_tau, ch->fu_V, ch->fu_scr, st,
+                           skip);
 }
 
 static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t st) {
@@ -1250,8 +1255,13 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
     if (ch->fu) {
         hipError_t e = fk_launch_fu_step(a, ch->fu_V, ch->fu_Q1, ch->fu_Q2, ch->fu_upd, ch->fu_T, ch->fu_V, st);
         if (e != hipSuccess) return e;
-        // a.step counts the steps of this call (sweep) or of the captured graph
-        if (ch->fu_refresh > 0 && (a.step + 1) % ch->fu_refresh == 0) e = fu_refresh_launch(ch, st);
+        // the step counter lives on the device, so the period runs across calls and graph
+        // replays; the re-decomposition kernels are launched every step and exit at once
+        // unless it is due
+        if (ch->fu_refresh > 0) {
+            e = fk_launch_fu_tick(ch->B, ch->fu_refresh, ch->fu_count, ch->fu_skip, st);
+            if (e == hipSuccess) e = fu_refresh_launch(ch, st, ch->fu_skip);
+        }
         return e;
     }
     if (ch->mixed) {
@@ -1308,8 +1318,22 @@ int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
         chk(dalloc(&ch->fu_scr, fk_evecs_scratch_doubles(B, N)));
         chk(dalloc(&ch->fu_upd, B));
         chk(dalloc(&ch->fu_f32, BN));
-        if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "rank-one update buffers");
+        chk(dalloc(&ch->fu_skip, B));
+        chk(dalloc(&ch->fu_count, 1));
+        if (e != hipSuccess) {
+            // release what was allocated so that a later call starts from scratch
+            void** bufs[] = {(void**)&ch->fu_V, (void**)&ch->fu_Q1, (void**)&ch->fu_Q2, (void**)&ch->fu_T,
+                             (void**)&ch->fu_vh, (void**)&ch->fu_tau, (void**)&ch->fu_d, (void**)&ch->fu_e,
+                             (void**)&ch->fu_scr, (void**)&ch->fu_upd, (void**)&ch->fu_f32, (void**)&ch->fu_skip,
+                             (void**)&ch->fu_count};
+            for (void** q : bufs) {
+                if (*q) hipFree(*q);
+                *q = nullptr;
+            }
+            return fail(ctx, FKGPU_ENOMEM, "rank-one update buffers");
+        }
     }
+    FK_HIP(ctx, hipMemsetAsync(ch->fu_count, 0, sizeof(long long), st));
     FK_HIP(ctx, fu_refresh_launch(ch, st));
     FK_HIP(ctx, hipStreamSynchronize(st));
     ch->fu = 1;

@@ -195,9 +195,10 @@ int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision);
  *                       sites is a rank-one (two rank-one) update of the spectrum by the
  *                       secular equation, O(N^2) per proposal, and an accepted move
  *                       updates the eigenvectors by one (two) N x N MFMA products.
- *                       Every refresh_steps steps of a fkgpu_sweep call (or of a graph
- *                       replay) the eigendecomposition is recomputed from scratch
- *                       (0: never).  Moves: addremove, flip, hop.  N <= 1024.
+ *                       Every refresh_steps move steps the eigendecomposition is
+ *                       recomputed from scratch (0: never); the steps are counted on the
+ *                       device from this call on, across fkgpu_sweep calls and graph
+ *                       replays.  Moves: addremove, flip, hop.  N <= 1024.
  * Switching to FKGPU_UPDATE_RANK1 computes the eigendecomposition of the current
  * configurations. */
 #define FKGPU_UPDATE_DENSE 0

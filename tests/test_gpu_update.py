@@ -102,3 +102,40 @@ def test_rank1_rejects_randomize(fk):
         ch.set_update(fk.UPDATE_RANK1, 0)
     ch.close()
     ctx.close()
+
+
+def test_rank1_refresh_period_across_calls(fk):
+    """The re-decomposition period is counted on the device: 16 steps in one call, in four
+    calls of 4 steps and in graph replays give bit-identical chains (re-decomposed after
+    steps 8 and 16 in every case)."""
+    D, L, B = 2, 8, 4
+    N = L * L
+    rng = np.random.default_rng(77)
+    f0 = rng.integers(0, 2, (B, N)).astype(np.int32)
+    beta = np.array([0.5, 2.0, 8.0, 20.0])
+    U = np.array([1.0, 2.0, 4.0, 8.0])
+    seeds = np.arange(B, dtype=np.uint32) + 90
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, D, L, 1.0, 0.0)
+    print("case rank1 refresh period", flush=True)
+    states = []
+    for mode in ("one", "four", "graph"):
+        ch = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
+        ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+        ch.set_update(fk.UPDATE_RANK1, 8)
+        if mode == "one":
+            ch.sweep(16)
+        elif mode == "four":
+            for _ in range(4):
+                ch.sweep(4)
+        else:
+            for _ in range(4):
+                ch.sweep_graph(4)
+        states.append(ch.state())
+        ch.close()
+    for k in ("f", "logZ", "spectrum", "naccept"):
+        assert np.array_equal(states[0][k], states[1][k]), k
+        assert np.array_equal(states[0][k], states[2][k]), k
+    # and the last re-decomposition (after step 16) left the spectrum of a fresh solve
+    ref = ctx.eig_logz(states[0]["f"], beta, U, U / 2)
+    assert np.array_equal(states[0]["spectrum"], ref["evals"])
+    ctx.close()

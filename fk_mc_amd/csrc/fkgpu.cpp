@@ -1499,6 +1499,9 @@ int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
             void* ps[] = {ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw, ch->t_logz, ch->t_acc};
             for (void* p : ps)
                 if (p) hipFree(p);
+            ch->t_kind = ch->t_a = ch->t_b = nullptr;
+            ch->t_u = ch->t_logw = ch->t_logz = nullptr;
+            ch->t_acc = nullptr;
             ch->trace_cap = 0;
             hipError_t e = hipSuccess;
             auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
@@ -1559,9 +1562,9 @@ int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
         if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
         sweep_args a = ch->args;
         a.trace = 0;
-        FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         if (!ch->kpm && !ch->dense_ok)
             return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
+        FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < nsteps; ++s) {
             a.step = s;
             hipError_t e = launch_step(ch, a, st);

@@ -37,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the headline move-step kernel on the default workload (cubic2d
 # L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (scales
 # from a 256 MiB copy, bench.py --copy-calibration); keyed by solver variant
@@ -359,15 +360,21 @@ def c5_block(fk, steps, check_steps, every=100):
                     "accept_rate_traced": round(float(tr["acc"].mean()), 4)})
     # timed pass: graph-captured steps
     ch.sweep_graph(2)
+    ns0 = ch.state()["nsolve"].copy()
     ch.sweep_graph(steps)
     ms = ch.last_ms() / steps
     st = ch.state()
     ch.close()
     ctx.close()
     nnz = 4 * N
-    flop = B * R * (M // 2) * (2 * nnz + 6 * N)  # per step, every proposal solved
+    solved = float((st["nsolve"] - ns0).sum()) / steps  # proposals traced per step (null moves skip)
+    flop = solved * R * (M // 2) * (2 * nnz + 6 * N)  # FP32 stencil recursion per step
+    achieved = flop / (ms / 1e3) / 1e12
     out.update({"steps": steps, "moves_per_s": round(B / ms * 1e3, 2), "ms_per_step": round(ms, 4),
-                "fp32_tflops_per_step_if_all_solved": round(flop / (ms / 1e3) / 1e12, 3),
+                "proposals_traced_per_step": round(solved, 2),
+                "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                             "work": "R (M/2) (2 nnz + 6 N) FP32 flop per traced proposal, whole step time"},
                 "logZ_finite": bool(np.all(np.isfinite(st["logZ"])))})
     return out
 

@@ -202,42 +202,6 @@ __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* 
     }
 }
 
-// Z <- Q Z.  One workgroup per matrix; thread t handles column j = t % N with
-// part q = t / N of the row range for the dot products (parts reduced in LDS).
-__global__ void __launch_bounds__(1024) k_backtransform(int B, int N, const double* __restrict__ vg,
-                                                        const double* __restrict__ taug, double* __restrict__ Zg,
-                                                        const int32_t* __restrict__ skip) {
-    __shared__ double v[2048];
-    __shared__ double part[1024];
-    const int b = blockIdx.x;
-    if (b >= B || (skip && skip[b])) return;
-    const int tid = threadIdx.x, NT = blockDim.x;
-    const int parts = max(1, NT / N);
-    const int j = tid % N, q = tid / N;
-    const bool active = q < parts && j < N;
-    double* Z = Zg + (size_t)b * N * N;
-    const double* V = vg + (size_t)b * N * N;
-    for (int k = N - 3; k >= 0; --k) {
-        const double tau = taug[(size_t)b * N + k];
-        if (tau == 0.0) continue;  // uniform across the block
-        for (int i = tid; i < N; i += NT) v[i] = V[(size_t)k * N + i];
-        __syncthreads();
-        double s = 0.0;
-        if (active)
-            for (int i = k + 1 + q; i < N; i += parts) s = fma(v[i], Z[(size_t)i * N + j], s);
-        if (active) part[q * N + j] = s;
-        __syncthreads();
-        if (active) {
-            double w = 0.0;
-            for (int p = 0; p < parts; ++p) w += part[p * N + j];
-            w *= tau;
-            for (int i = k + 1 + q; i < N; i += parts) Z[(size_t)i * N + j] = fma(-v[i], w, Z[(size_t)i * N + j]);
-        }
-        __syncthreads();
-    }
-}
-
-
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int BT_NB = 16;  // reflectors per block
 
@@ -408,7 +372,7 @@ hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     if (N < 3) return hipSuccess;  // no reflectors
-    // T factors go to the solve scratch (B x 3 x N x N doubles), free once the vectors of T are formed
+    // T factors go to the solve scratch, free once the vectors of T are formed
     const int nblk = (N - 2 + fkv::BT_NB - 1) / fkv::BT_NB;
     hipLaunchKernelGGL(fkv::k_bt_tfac, dim3(B, nblk), dim3(256), 0, st, B, N, nblk, vh, tau, scratch, skip);
     r = hipGetLastError();

@@ -43,9 +43,10 @@ struct start_vec {
 // T - lam by partial pivoting formed on the fly (U's three diagonals to u0/u1/u2, stride
 // ld, this thread's column; zero pivots replaced by +-tiny, the usual inverse-iteration
 // rule).  With gen the right-hand side is the start vector of eigenvalue j and x is not
-// read.  Loads are issued in chunks of CH ahead of their use (the recurrences themselves
-// are sequential).  Returns ||z||^2; *zbig gets the entry of largest magnitude.
-constexpr int TS_CH = 4;
+// read.  Loads are issued in chunks ahead of their use, branch-free so that a chunk is in
+// flight as a whole (the recurrences themselves are sequential).  Returns ||z||^2; *zbig gets the entry of largest magnitude.
+constexpr int TS_CF = 8, TS_CB = 4;  // chunks of the forward and backward sweeps
+constexpr int TS_CV = 8;  // chunk of the vector passes (orthogonalisation, final scaling)
 __device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int N, double lam,
                             double tiny, double* x, double s, bool gen, int j, double* u0, double* u1, double* u2,
                             size_t ld, double* zbig) {
@@ -53,15 +54,19 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
     double a = d[0] - lam;                 // current pivot-row diagonal
     double b = N > 1 ? e[0] : 0.0;         // current pivot-row superdiagonal
     double xk = gen ? sv.next() : s * x[0];
-    for (int k0 = 0; k0 + 1 < N; k0 += TS_CH) {
-        double xb[TS_CH];
+    for (int k0 = 0; k0 + 1 < N; k0 += TS_CF) {
+        double xb[TS_CF];
+        // branch-free loads (index clamped to the last row; the surplus values are not used),
+        // so that all of them are in flight before the first is waited for
+        if (gen) {
 #pragma unroll
-        for (int q = 0; q < TS_CH; ++q) {
-            const int i = k0 + 1 + q;
-            xb[q] = i < N ? (gen ? sv.next() : s * x[(size_t)i * ld]) : 0.0;
+            for (int q = 0; q < TS_CF; ++q) xb[q] = k0 + 1 + q < N ? sv.next() : 0.0;
+        } else {
+#pragma unroll
+            for (int q = 0; q < TS_CF; ++q) xb[q] = s * x[(size_t)min(k0 + 1 + q, N - 1) * ld];
         }
 #pragma unroll
-        for (int q = 0; q < TS_CH; ++q) {
+        for (int q = 0; q < TS_CF; ++q) {
             const int k = k0 + q;
             if (k + 1 >= N) break;
             const double c = e[k];             // subdiagonal below the pivot
@@ -92,20 +97,18 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
     // back substitution with U (diagonals u0, u1, u2)
     double z1 = 0.0, z2 = 0.0;  // z[k+1], z[k+2]
     double nrm2 = 0.0, big = 0.0;
-    for (int k0 = N - 1; k0 >= 0; k0 -= TS_CH) {
-        double xb[TS_CH], p0[TS_CH], p1[TS_CH], p2[TS_CH];
+    for (int k0 = N - 1; k0 >= 0; k0 -= TS_CB) {
+        double xb[TS_CB], p0[TS_CB], p1[TS_CB], p2[TS_CB];
 #pragma unroll
-        for (int q = 0; q < TS_CH; ++q) {
-            const int k = k0 - q;
-            if (k >= 0) {
-                xb[q] = x[(size_t)k * ld];
-                p0[q] = u0[(size_t)k * ld];
-                p1[q] = u1[(size_t)k * ld];
-                p2[q] = u2[(size_t)k * ld];
-            }
+        for (int q = 0; q < TS_CB; ++q) {
+            const size_t o = (size_t)max(k0 - q, 0) * ld;  // branch-free (row 0 repeated past the end)
+            xb[q] = x[o];
+            p0[q] = u0[o];
+            p1[q] = u1[o];
+            p2[q] = u2[o];
         }
 #pragma unroll
-        for (int q = 0; q < TS_CH; ++q) {
+        for (int q = 0; q < TS_CB; ++q) {
             const int k = k0 - q;
             if (k < 0) break;
             double t = xb[q];
@@ -123,17 +126,79 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
     return nrm2;
 }
 
-// One workgroup per matrix.  d, e, ev: B x N.  Z: B x N x N output.
-// scratch: B x 3 x N x N.  Eigenvalues closer than ortol form clusters; members are
-// computed in rounds (position in the cluster) and orthogonalised against the earlier
-// members.  Isolated eigenvalues take two solves, cluster members three.
+// Orthogonalise x (column stride N) against the columns p in [p0, p1) of Z (unit vectors),
+// modified Gram-Schmidt, loads in branch-free chunks.  Returns ||x||^2; *xbig gets the entry
+// of largest magnitude.
+__device__ double orth_against(double* x, const double* Z, int p0, int p1, int N, double nrm2, double* xbig) {
+    for (int p = p0; p < p1; ++p) {
+        const double* y = Z + p;
+        double dot = 0.0;
+        for (int i0 = 0; i0 < N; i0 += TS_CV) {
+            double xb[TS_CV], yb[TS_CV];
+#pragma unroll
+            for (int q = 0; q < TS_CV; ++q) {
+                const size_t o = (size_t)min(i0 + q, N - 1) * N;
+                xb[q] = x[o];
+                yb[q] = y[o];
+            }
+#pragma unroll
+            for (int q = 0; q < TS_CV; ++q)
+                if (i0 + q < N) dot = fma(xb[q], yb[q], dot);
+        }
+        nrm2 = 0.0;
+        double big = 0.0;
+        for (int i0 = 0; i0 < N; i0 += TS_CV) {
+            double xb[TS_CV], yb[TS_CV];
+#pragma unroll
+            for (int q = 0; q < TS_CV; ++q) {
+                const size_t o = (size_t)min(i0 + q, N - 1) * N;
+                xb[q] = x[o];
+                yb[q] = y[o];
+            }
+#pragma unroll
+            for (int q = 0; q < TS_CV; ++q) {
+                if (i0 + q >= N) break;
+                const double v = fma(-dot, yb[q], xb[q]);
+                x[(size_t)(i0 + q) * N] = v;
+                nrm2 = fma(v, v, nrm2);
+                if (fabs(v) > fabs(big)) big = v;
+            }
+        }
+        *xbig = big;
+    }
+    return nrm2;
+}
+
+// x <- x sign(big) / sqrt(nrm2): unit norm, largest-magnitude component positive.
+__device__ void finish_vec(double* x, int N, double nrm2, double big) {
+    const double sc = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0;
+    const double sg = big < 0.0 ? -sc : sc;
+    for (int i0 = 0; i0 < N; i0 += TS_CV) {
+        double xb[TS_CV];
+#pragma unroll
+        for (int q = 0; q < TS_CV; ++q) xb[q] = x[(size_t)min(i0 + q, N - 1) * N];
+#pragma unroll
+        for (int q = 0; q < TS_CV; ++q)
+            if (i0 + q < N) x[(size_t)(i0 + q) * N] = sg * xb[q];
+    }
+}
+
+// One workgroup per matrix.  d, e, ev: B x N.  Z: B x N x N output.  scratch: B x 3 x N x N.
+// Two cluster levels (by gaps between consecutive eigenvalues):
+//   * tight (gap <= 1e-7 ||T||): inverse iteration cannot separate the members, so they are
+//     computed in rounds (position in the cluster), three solves each, every iterate
+//     orthogonalised against the earlier members (as LAPACK dstein);
+//   * loose (gap <= 1e-3 ||T||, dstein's threshold): inverse iteration separates them by a
+//     factor ~ eps ||T|| / gap <= 2e-9 per solve, so every eigenvalue outside a tight
+//     cluster is computed at once (two solves), and the members of each loose cluster are
+//     orthogonalised afterwards in cheap rounds of Gram-Schmidt against the earlier members.
 __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* __restrict__ dg,
                                                     const double* __restrict__ eg, const double* __restrict__ evg,
                                                     double* __restrict__ Zg, double* __restrict__ scr,
                                                     const int32_t* __restrict__ skip) {
     __shared__ double d[2048], e[2048];
-    __shared__ int cstart[2048];
-    __shared__ int maxpos;
+    __shared__ short cstart[2048], tstart[2048];
+    __shared__ int maxpos_l, maxpos_t;
     const int b = blockIdx.x;
     const int tid = threadIdx.x, NT = blockDim.x;
     if (b >= B || (skip && skip[b])) return;
@@ -148,54 +213,66 @@ __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* 
         d[i] = dg[(size_t)b * N + i];
         e[i] = eg[(size_t)b * N + i];
     }
-    if (tid == 0) maxpos = 0;
     __syncthreads();
     for (int i = 0; i < N; ++i) tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < N ? fabs(e[i]) : 0.0));
     const double eps = 2.220446049250313e-16;
-    const double ortol = 1e-3 * tn;            // cluster threshold (as LAPACK dstein)
+    const double ortol = 1e-3 * tn;  // loose clusters (LAPACK dstein's threshold)
+    const double ttol = 1e-7 * tn;   // tight clusters
     const double tiny = eps * fmax(tn, 1e-300);
-    // cluster starts (sequential scan by one thread; N is small)
+    // cluster starts (sequential scan by one thread over eigenvalues staged in LDS)
     if (tid == 0) {
-        int cs = 0, mp = 0;
-        for (int j = 0; j < N; ++j) {
-            if (j > 0 && ev[j] - ev[j - 1] > ortol) cs = j;
-            cstart[j] = cs;
-            mp = max(mp, j - cs);
+        int cs = 0, ts = 0, mpl = 0, mpt = 0;
+        double prev = 0.0;
+        for (int j0 = 0; j0 < N; j0 += TS_CV) {
+            double eb[TS_CV];
+#pragma unroll
+            for (int q = 0; q < TS_CV; ++q) eb[q] = ev[min(j0 + q, N - 1)];
+#pragma unroll
+            for (int q = 0; q < TS_CV; ++q) {
+                const int j = j0 + q;
+                if (j >= N) break;
+                if (j > 0 && eb[q] - prev > ortol) cs = j;
+                if (j > 0 && eb[q] - prev > ttol) ts = j;
+                cstart[j] = (short)cs;
+                tstart[j] = (short)ts;
+                mpl = max(mpl, j - cs);
+                mpt = max(mpt, j - ts);
+                prev = eb[q];
+            }
         }
-        maxpos = mp;
+        maxpos_l = mpl;
+        maxpos_t = mpt;
     }
     __syncthreads();
-    const int rounds = maxpos + 1;
-    for (int round = 0; round < rounds; ++round) {
+    // inverse iteration: tight clusters in rounds, everything else in round 0
+    for (int round = 0; round <= maxpos_t; ++round) {
         for (int j = tid; j < N; j += NT) {
-            const int pos = j - cstart[j];
+            const int pos = j - tstart[j];
             if (pos != round) continue;
-            const bool isolated = pos == 0 && (j + 1 == N || cstart[j + 1] != j);
+            const bool tight = pos > 0 || (j + 1 < N && tstart[j + 1] == j);
             double* x = Z + j;  // column j, stride N
             double nrm2 = 0.0, big = 0.0;
-            const int its = isolated ? 2 : 3;
+            const int its = tight ? 3 : 2;
             for (int it = 0; it < its; ++it) {
                 const double sc = it == 0 ? 1.0 : (nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0);
                 nrm2 = tri_solve(d, e, N, ev[j], tiny, x, sc, it == 0, j, u0 + j, u1 + j, u2 + j, N, &big);
-                // orthogonalise against earlier members of the cluster
-                for (int p = cstart[j]; p < j; ++p) {
-                    const double* y = Z + p;
-                    double dot = 0.0;
-                    for (int i = 0; i < N; ++i) dot = fma(x[(size_t)i * N], y[(size_t)i * N], dot);
-                    nrm2 = 0.0;
-                    big = 0.0;
-                    for (int i = 0; i < N; ++i) {
-                        const double v = fma(-dot, y[(size_t)i * N], x[(size_t)i * N]);
-                        x[(size_t)i * N] = v;
-                        nrm2 = fma(v, v, nrm2);
-                        if (fabs(v) > fabs(big)) big = v;
-                    }
-                }
+                if (pos > 0) nrm2 = orth_against(x, Z, tstart[j], j, N, nrm2, &big);
             }
-            // unit norm, largest-magnitude component positive
-            const double sc = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0;
-            const double sg = big < 0.0 ? -sc : sc;
-            for (int i = 0; i < N; ++i) x[(size_t)i * N] *= sg;
+            finish_vec(x, N, nrm2, big);
+        }
+        __threadfence();
+        __syncthreads();
+    }
+    // loose clusters: Gram-Schmidt against the earlier members, by position (members of a
+    // tight cluster are already orthogonal to their tight predecessors; the pass repeats that
+    // harmlessly)
+    for (int round = 1; round <= maxpos_l; ++round) {
+        for (int j = tid; j < N; j += NT) {
+            if (j - cstart[j] != round) continue;
+            double* x = Z + j;
+            double big = 0.0;
+            const double nrm2 = orth_against(x, Z, cstart[j], j, N, 1.0, &big);
+            finish_vec(x, N, nrm2, big);
         }
         __threadfence();
         __syncthreads();

@@ -80,6 +80,19 @@ __device__ double block_sum(double v, double* red) {
     return r;
 }
 
+// Chunked reads of two LDS arrays (indices clamped to the last valid entry, no branches), so
+// that a chunk's loads are in flight together before the terms that use them.
+constexpr int RC = 8;
+__device__ __forceinline__ void load_chunk(const double* a, const double* b, int k0, int m, double (&ab)[RC],
+                                           double (&bb)[RC]) {
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+        const int k = min(k0 + q, m - 1);
+        ab[q] = a[k];
+        bb[q] = b[k];
+    }
+}
+
 // Eigen-update of diag(sm.d) + rho z z^T.  On return sm.d holds the new eigenvalues
 // in ascending order and Q (N x N, row-major, HBM) the orthogonal matrix with
 // diag(d) + rho z z^T = Q diag(d_new) Q^T.  All threads of the workgroup call it.
@@ -164,7 +177,14 @@ __device__ void rank1(SM& sm, const double* z, double rho, int N, double* __rest
         if (j < m - 1) {
             const double half = 0.5 * (sm.da[j + 1] - sm.da[j]);
             double g = irh;
-            for (int k = 0; k < m; ++k) g += sm.za[k] * sm.za[k] * rcp_refined((sm.da[k] - sm.da[j]) - half);
+            const double dj = sm.da[j];
+            for (int k0 = 0; k0 < m; k0 += RC) {
+                double db[RC], zb[RC];
+                load_chunk(sm.da, sm.za, k0, m, db, zb);
+#pragma unroll
+                for (int q = 0; q < RC; ++q)
+                    if (k0 + q < m) g += zb[q] * zb[q] * rcp_refined((db[q] - dj) - half);
+            }
             if (g >= 0.0) { o = j; lo = 0.0; hi = half; }
             else { o = j + 1; lo = -half; hi = 0.0; }
         } else {
@@ -176,12 +196,20 @@ __device__ void rank1(SM& sm, const double* z, double rho, int N, double* __rest
         double x = 0.5 * (lo + hi);
         for (int it = 0; it < 64; ++it) {
             double S = 0.0, S1 = 0.0, Sa = 0.0;
-            for (int k = 0; k < m; ++k) {
-                if (k == o) continue;
-                const double q = sm.za[k] * rcp_refined((sm.da[k] - sm.da[o]) - x);
-                S += sm.za[k] * q;
-                Sa += fabs(sm.za[k] * q);
-                S1 += q * q;
+            const double dao = sm.da[o];
+            for (int k0 = 0; k0 < m; k0 += RC) {
+                double db[RC], zb[RC];
+                load_chunk(sm.da, sm.za, k0, m, db, zb);
+#pragma unroll
+                for (int q = 0; q < RC; ++q) {
+                    const int k = k0 + q;
+                    if (k >= m || k == o) continue;
+                    // pole distances relative to the origin pole first (exact for near poles)
+                    const double qq = zb[q] * rcp_refined((db[q] - dao) - x);
+                    S += zb[q] * qq;
+                    Sa += fabs(zb[q] * qq);
+                    S1 += qq * qq;
+                }
             }
             const double base = irh + S;
             const double gh = -x * base + zo2;
@@ -333,9 +361,22 @@ __global__ void __launch_bounds__(UT) k_fu_step(sweep_args a, const double* __re
         if (s2 >= 0) {
             const double* Q1 = Q1g + (size_t)ch * NN;
             const double* zr = V + (size_t)s2 * N;
+            // z2 = Q1^T zr, column c per thread; loads in branch-free chunks so that a chunk is
+            // in flight together (rows clamped to the last one, their terms dropped)
             for (int c = tid; c < N; c += UT) {
                 double s = 0.0;
-                for (int i = 0; i < N; ++i) s = fma(Q1[(size_t)i * N + c], zr[i], s);
+                for (int i0 = 0; i0 < N; i0 += RC) {
+                    double qb[RC], zb[RC];
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) {
+                        const int i = min(i0 + q, N - 1);
+                        qb[q] = Q1[(size_t)i * N + c];
+                        zb[q] = zr[i];
+                    }
+#pragma unroll
+                    for (int q = 0; q < RC; ++q)
+                        if (i0 + q < N) s = fma(qb[q], zb[q], s);
+                }
                 sm.z2[c] = s;
             }
             __syncthreads();
@@ -427,6 +468,76 @@ __global__ void __launch_bounds__(256) k_fu_gemm(int B, int N, const int32_t* __
     store(c11, I0 + 32 * wr + 16, J0 + 32 * wc + 16);
 }
 
+// The same product for N a multiple of GB (no bounds checks): the next K chunk is loaded into
+// registers (16-byte loads) while the MFMAs of the current one run from LDS, so the global
+// load latency is hidden behind the matrix work instead of exposed once per chunk.
+__global__ void __launch_bounds__(256) k_fu_gemm_full(int B, int N, const int32_t* __restrict__ upd, int stage,
+                                                      const double* __restrict__ Ag, const double* __restrict__ Bg,
+                                                      double* __restrict__ Cg) {
+    const int b = blockIdx.y;
+    const int u = upd[b];
+    if (u < 1) return;
+    const size_t NN = (size_t)N * N;
+    const double* A = Ag + (size_t)b * NN;
+    const double* Bm = Bg + (size_t)b * NN;
+    double* C = Cg + (size_t)b * NN;
+    const int nbt = N / GB;
+    const int I0 = (blockIdx.x / nbt) * GB, J0 = (blockIdx.x % nbt) * GB;
+    const int tid = threadIdx.x;
+    if (stage == 2 && u == 1) {
+        for (int e = tid; e < GB * GB / 2; e += 256) {
+            const int i = I0 + e / (GB / 2), j = J0 + 2 * (e % (GB / 2));
+            *reinterpret_cast<double2*>(C + (size_t)i * N + j) = *reinterpret_cast<const double2*>(A + (size_t)i * N + j);
+        }
+        return;
+    }
+    __shared__ double As[GB][GK + 1];
+    __shared__ double Bs[GK][GB + 1];
+    const int lane = tid & (WAVE - 1), wv = tid / WAVE;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int li = lane & 15, kq = lane >> 4;
+    // this thread's share of a chunk: two double2 of A (rows ar, ar + 32; columns 2 ac, 2 ac + 1)
+    // and two of B (rows bk, bk + 8; columns 2 bc, 2 bc + 1)
+    const int ar = tid / (GK / 2), ac = tid % (GK / 2);
+    const int bk = tid / (GB / 2), bc = tid % (GB / 2);
+    double2 ra0, ra1, rb0, rb1;
+    auto fetch = [&](int k0) {
+        ra0 = *reinterpret_cast<const double2*>(A + (size_t)(I0 + ar) * N + k0 + 2 * ac);
+        ra1 = *reinterpret_cast<const double2*>(A + (size_t)(I0 + ar + 32) * N + k0 + 2 * ac);
+        rb0 = *reinterpret_cast<const double2*>(Bm + (size_t)(k0 + bk) * N + J0 + 2 * bc);
+        rb1 = *reinterpret_cast<const double2*>(Bm + (size_t)(k0 + bk + 8) * N + J0 + 2 * bc);
+    };
+    fkb::v4d c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+    fetch(0);
+    for (int k0 = 0; k0 < N; k0 += GK) {
+        As[ar][2 * ac] = ra0.x; As[ar][2 * ac + 1] = ra0.y;
+        As[ar + 32][2 * ac] = ra1.x; As[ar + 32][2 * ac + 1] = ra1.y;
+        Bs[bk][2 * bc] = rb0.x; Bs[bk][2 * bc + 1] = rb0.y;
+        Bs[bk + 8][2 * bc] = rb1.x; Bs[bk + 8][2 * bc + 1] = rb1.y;
+        __syncthreads();
+        if (k0 + GK < N) fetch(k0 + GK);  // in flight during the MFMAs below
+#pragma unroll
+        for (int ks = 0; ks < GK / 4; ++ks) {
+            const int kx = 4 * ks + kq;
+            const double a0 = As[32 * wr + li][kx], a1 = As[32 * wr + 16 + li][kx];
+            const double b0 = Bs[kx][32 * wc + li], b1 = Bs[kx][32 * wc + 16 + li];
+            c00 = fkb::mf<double>::op(a0, b0, c00);
+            c01 = fkb::mf<double>::op(a0, b1, c01);
+            c10 = fkb::mf<double>::op(a1, b0, c10);
+            c11 = fkb::mf<double>::op(a1, b1, c11);
+        }
+        __syncthreads();
+    }
+    auto store = [&](const fkb::v4d& acc, int r0, int c0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) C[(size_t)(r0 + fkb::mf<double>::row(lane, q)) * N + c0 + li] = acc[q];
+    };
+    store(c00, I0 + 32 * wr, J0 + 32 * wc);
+    store(c01, I0 + 32 * wr, J0 + 32 * wc + 16);
+    store(c10, I0 + 32 * wr + 16, J0 + 32 * wc);
+    store(c11, I0 + 32 * wr + 16, J0 + 32 * wc + 16);
+}
+
 // skip0 (nullable): nothing to do when skip0[0] is set (the refresh is not due)
 __global__ void k_fu_f8to32(int n, const uint8_t* __restrict__ f8, int32_t* __restrict__ f32,
                             const int32_t* __restrict__ skip0) {
@@ -462,9 +573,11 @@ hipError_t fk_launch_fu_step(const sweep_args& a, const double* V, double* Q1, d
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     const int nbt = (a.N + fku::GB - 1) / fku::GB;
-    hipLaunchKernelGGL(fku::k_fu_gemm, dim3(nbt * nbt, a.B), dim3(256), 0, st, a.B, a.N, upd, 1, V, Q1, T);
+    auto gemm = fku::k_fu_gemm;
+    if (a.N % fku::GB == 0) gemm = fku::k_fu_gemm_full;
+    hipLaunchKernelGGL(gemm, dim3(nbt * nbt, a.B), dim3(256), 0, st, a.B, a.N, upd, 1, V, Q1, T);
     if ((r = hipGetLastError()) != hipSuccess) return r;
-    hipLaunchKernelGGL(fku::k_fu_gemm, dim3(nbt * nbt, a.B), dim3(256), 0, st, a.B, a.N, upd, 2, T, Q2, Vout);
+    hipLaunchKernelGGL(gemm, dim3(nbt * nbt, a.B), dim3(256), 0, st, a.B, a.N, upd, 2, T, Q2, Vout);
     return hipGetLastError();
 }
 hipError_t fk_launch_fu_tick(int B, int period, long long* count, int32_t* skip, hipStream_t st) {

@@ -241,16 +241,22 @@ struct wrow_solver {
         const int l = tid & (WAVE - 1);
         const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
         double A[NS];
+        // all 40 template loads in flight at once (indices clamped, out-of-range entries zeroed
+        // afterwards), then the diagonal
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b <= (a >> 2); ++b) {
+                const int i = min(w + 16 * a, N - 1), j = min(l + 64 * b, N - 1);
+                A[sl(a, b)] = hop[(size_t)i * N + j];
+            }
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
             for (int b = 0; b <= (a >> 2); ++b) {
                 const int i = w + 16 * a, j = l + 64 * b;
-                double v = 0.0;
-                if (i < N && j < N) {
-                    v = hop[(size_t)i * N + j];
-                    if (i == j) v += -mu_c + U * (double)f_lds[i];
-                }
+                double v = (i < N && j < N) ? A[sl(a, b)] : 0.0;
+                if (i == j && i < N) v += -mu_c + U * (double)f_lds[i];
                 A[sl(a, b)] = v;
             }
         tridiagonalize(A, sm, N, vout, tauout);

@@ -15,6 +15,10 @@ Printed: one JSON line with the contract fields plus
                 stream) against MI355X's 78.6 TFLOP/s FP64 peak;
   plain         the single-chain drop-in path (calc_hamiltonian + calc_ed
                 through the C ABI with host buffers, one call per move);
+  measured_sweeps
+                sweeps of 16 moves each followed by the on-device measures
+                (energy, spectrum mean, spectrum / occupation / IPR histories) of
+                all chains, with the overhead over unmeasured sweeps;
   rank1_update, mixed, mixed_cubic3d_L8
                 the same chains with rank-one spectral updates / mixed precision,
                 and cubic3d L=8 (N=512) mixed against FP64, each with its dOmega
@@ -71,6 +75,8 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=8, help="mixed-precision timing steps (0: skip)")
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
     ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
+    ap.add_argument("--measure-sweeps", type=int, default=4,
+                    help="measured sweeps of 16 moves with on-device measures (0: skip)")
     ap.add_argument("--c5-steps", type=int, default=16,
                     help="cubic2d L=64 stochastic-trace chain timing steps (0: skip)")
     ap.add_argument("--c5-check-steps", type=int, default=25,
@@ -141,6 +147,9 @@ def main():
     rank1 = None
     if args.rank1_steps > 0:
         rank1 = rank1_block(fk, ctx, f, beta, U, seeds, args.rank1_steps, plain, L)
+    measured = None
+    if args.measure_sweeps > 0:
+        measured = measured_block(fk, ctx, f, beta, U, seeds, args.measure_sweeps)
     mixed = None
     if args.mixed_steps > 0:
         mixed = mixed_block(fk, ctx, f, beta, U, seeds, args.mixed_steps, plain)
@@ -177,6 +186,7 @@ def main():
         "roofline": roofline,
         "plain": plain,
         "rank1_update": rank1,
+        "measured_sweeps": measured,
         "mixed": mixed,
         "mixed_cubic3d_L8": c3,
         "kpm_cubic2d_L64": c5,
@@ -251,6 +261,40 @@ def rank1_block(fk, ctx, f, beta, U, seeds, steps, plain, L):
     tr = sub.sweep(msteps, trace=True)
     sub.close()
     out.update(compare_traces(t64, tr, beta[sl]))
+    return out
+
+
+def measured_block(fk, ctx, f, beta, U, seeds, sweeps, sweep_len=16):
+    """The reference's measured schedule on the device: sweeps of sweep_len moves of every
+    chain, each followed by measure_energy, measure_spectrum and the spectrum and occupation
+    histories of all 1024 chains (records stay on the device until read).  FP64 dense
+    chains, and rank-one chains that also record the IPR from their resident eigenvectors.
+    The overhead is against the same chains' unmeasured graph replay."""
+    B = len(beta)
+    out = {"sweep_len": sweep_len, "sweeps": sweeps}
+    for name, rank1, flags in (("fp64", False, fk.MEASURE_SPECTRUM_HISTORY | fk.MEASURE_FOCC_HISTORY),
+                               ("rank1_ipr", True, fk.MEASURE_SPECTRUM_HISTORY | fk.MEASURE_FOCC_HISTORY
+                                | fk.MEASURE_IPR)):
+        res = {}
+        for measured in (True, False):
+            ch = fk.Chains(ctx, f, beta, U, U / 2, U / 2, seeds)
+            ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+            if rank1:
+                ch.set_update(fk.UPDATE_RANK1, 256)
+            if measured:
+                ch.measure_enable(sweeps + 1, flags)
+                ch.sweep_measure(1, sweep_len)
+                ch.sweep_measure(sweeps, sweep_len)
+                res["count"] = ch.energies()[0]
+            else:
+                ch.sweep_graph(sweep_len)
+                ch.sweep_graph(sweeps * sweep_len)
+            res[measured] = ch.last_ms()
+            ch.close()
+        out[name] = {"moves_per_s": round(B * sweeps * sweep_len / res[True] * 1e3, 2),
+                     "ms_per_sweep": round(res[True] / sweeps, 4),
+                     "measure_overhead": round(res[True] / res[False] - 1.0, 4),
+                     "measurements": res["count"]}
     return out
 
 

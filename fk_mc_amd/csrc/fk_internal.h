@@ -63,6 +63,29 @@ struct sweep_args {
     int8_t* tr_acc;
 };
 
+// One measurement of every chain of a chain set, recorded on the device at record
+// m = count[0] (series and histories only while m < cap).
+struct chain_meas_args {
+    int B, N, cap;
+    int nW;                        // f-f interaction range (1-D lattices only)
+    double W[8];
+    const uint8_t* f;              // B x N
+    const int32_t* nf;             // B
+    const double* spectrum;        // B x N, ascending
+    const double* beta;            // B
+    const double* mu_f;            // B
+    const double* ipr;             // B x N of this measurement or null
+    const long long* count;        // measurements taken so far
+    double *e, *d2e, *ce;          // B x cap
+    double *esum, *d2sum;          // B
+    double* spec_mean;             // B x N running mean
+    long long* focc_sum;           // B x N
+    double* spec_hist;             // B x cap x N or null
+    uint8_t* focc_hist;            // B x cap x N or null
+    double* ipr_hist;              // B x cap x N or null
+};
+hipError_t fk_launch_chain_measure(const chain_meas_args& a, long long* count, hipStream_t st);
+
 // One proposed move (kept between the launches of a stochastic-trace move step).
 struct move_prop {
     int kind, sa, sb, solve, mode, nf_new;  // mode: 0 weighed, 1 zero weight, 2 unit weight

@@ -13,10 +13,17 @@
 //   k_adjoint        dOmega/dH = rho = V f(Lambda) V^T restricted to the diagonal and
 //                    the hopping bonds: local densities, dOmega/dU, dOmega/dt,
 //                    dOmega/dtp, dOmega/dmu_c, and dN_c/dmu_c = beta sum f(1-f).
+//   k_chain_measure  one measurement of every chain of a chain set, recorded on the
+//                    device: measure_energy (src/measures/energy.cpp:6-26),
+//                    measure_spectrum (src/measures/spectrum.cpp:13-21),
+//                    measure_spectrum_history (src/measures/spectrum_history.cpp:13-19),
+//                    measure_focc (src/measures/focc_history.cpp:8-13) and the IPR
+//                    history of measure_ipr.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "fk_internal.h"
+#include "fk_moves.h"
 #include "fk_solve.h"
 
 namespace fkm {
@@ -247,6 +254,73 @@ __global__ void __launch_bounds__(GT * GT) k_rho_full(int B, int N, const double
     if (r0 + ty < N && c0 + tx < N) R[(size_t)b * N * N + (size_t)(r0 + ty) * N + c0 + tx] = acc;
 }
 
+// One workgroup per chain.  Record m = count[0]: E_c = sum_k e_k f(e_k),
+// d2E = sum_k e_k^2 f(1-f) (written, as the reference does, as
+// e^2 / (1 + (x + 1/x)/2) / 2 with x = exp(beta e)), E = E_c - mu_f N_f + E_ff; the
+// spectrum's running mean (avg Z + e) / (Z + 1) in the reference's order of operations
+// (no fused multiply-add, so the bits equal a host loop's); occupation counts; and the
+// histories when their buffers are given.
+constexpr int CM_T = 256;
+__device__ inline double running_mean(double avg, double z, double e) {
+#pragma clang fp contract(off)
+    return (avg * z + e) / (z + 1.0);
+}
+
+__global__ void __launch_bounds__(CM_T) k_chain_measure(chain_meas_args a) {
+    __shared__ double red[2][CM_T / WAVE];
+    const int b = blockIdx.x, N = a.N, tid = threadIdx.x;
+    const long long m = a.count[0];
+    const bool rec = m < (long long)a.cap;
+    const double beta = a.beta[b];
+    const double z = (double)m;
+    const size_t row = (size_t)b * N;
+    const size_t hrow = ((size_t)b * a.cap + (size_t)(rec ? m : 0)) * N;
+    double ec = 0.0, d2 = 0.0;
+    for (int k = tid; k < N; k += CM_T) {
+        const double e = a.spectrum[row + k];
+        const double x = exp(beta * e);
+        ec += e / (1.0 + x);
+        d2 += e * e / (1.0 + 0.5 * (x + 1.0 / x));
+        a.spec_mean[row + k] = running_mean(a.spec_mean[row + k], z, e);
+        if (rec && a.spec_hist) a.spec_hist[hrow + k] = e;
+        if (a.ipr && rec && a.ipr_hist) a.ipr_hist[hrow + k] = a.ipr[row + k];
+        const uint8_t fi = a.f[row + k];
+        a.focc_sum[row + k] += fi;
+        if (rec && a.focc_hist) a.focc_hist[hrow + k] = fi;
+    }
+    ec = wave_sum(ec);
+    d2 = wave_sum(d2);
+    const int w = tid / WAVE;
+    if ((tid & (WAVE - 1)) == 0) {
+        red[0][w] = ec;
+        red[1][w] = d2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        ec = 0.0;
+        d2 = 0.0;
+        for (int i = 0; i < CM_T / WAVE; ++i) {
+            ec += red[0][i];
+            d2 += red[1][i];
+        }
+        const double ff = a.nW > 0 ? fkk::ff_energy_dev(a.f + row, N, a.nW, a.W) : 0.0;
+        const double e = ec - a.mu_f[b] * (double)a.nf[b] + ff;
+        d2 *= 0.5;
+        if (rec) {
+            a.e[(size_t)b * a.cap + m] = e;
+            a.d2e[(size_t)b * a.cap + m] = d2;
+            a.ce[(size_t)b * a.cap + m] = ec;
+        }
+        a.esum[b] += e;
+        a.d2sum[b] += d2;
+    }
+}
+
+// count += 1, after every chain has recorded the measurement.  One thread.
+__global__ void k_chain_measure_tick(long long* count) {
+    if (threadIdx.x == 0) count[0] += 1;
+}
+
 }  // namespace fkm
 
 using namespace fkm;
@@ -291,4 +365,13 @@ hipError_t fk_launch_adjoint(int B, int N, const double* V, const double* fermi,
         r = hipGetLastError();
     }
     return r;
+}
+
+hipError_t fk_launch_chain_measure(const chain_meas_args& a, long long* count, hipStream_t st) {
+    if (a.B <= 0 || a.N <= 0 || a.cap < 0 || a.nW < 0 || a.nW > 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_chain_measure, dim3(a.B), dim3(CM_T), 0, st, a);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_chain_measure_tick, dim3(1), dim3(64), 0, st, count);
+    return hipGetLastError();
 }

@@ -42,6 +42,34 @@ struct fkgpu_ctx {
     buf b_vh, b_tau, b_d, b_e, b_Z, b_scr, b_m1, b_m2, b_m3, b_m4, b_m5, b_out, b_w, b_c1, b_c2, b_c3, b_c4;
 };
 
+// Measurement records of a chain set (fkgpu_chains_measure_enable): per-chain series of
+// E, d2E and E_c, running means and optional histories, all on the device.
+struct chain_meas_bufs {
+    int cap = 0, flags = 0;
+    long long* count = nullptr;
+    double *e = nullptr, *d2e = nullptr, *ce = nullptr;  // B x cap
+    double *esum = nullptr, *d2sum = nullptr;            // B
+    double* spec_mean = nullptr;                         // B x N
+    long long* focc_sum = nullptr;                       // B x N
+    double* spec_hist = nullptr;                         // B x cap x N
+    uint8_t* focc_hist = nullptr;                        // B x cap x N
+    double *ipr = nullptr, *ipr_hist = nullptr;          // B x N, B x cap x N
+    // decomposition of the current configurations, for chain sets that keep no
+    // eigenvalues (stochastic trace) or no eigenvectors (IPR outside the rank-one mode)
+    int32_t* f32 = nullptr;
+    double *ev = nullptr, *lz = nullptr, *vh = nullptr, *tau = nullptr, *d = nullptr, *eo = nullptr;
+    double *V = nullptr, *scr = nullptr;
+};
+
+static void meas_free(chain_meas_bufs* m) {
+    if (!m) return;
+    void* ps[] = {m->count, m->e, m->d2e, m->ce, m->esum, m->d2sum, m->spec_mean, m->focc_sum, m->spec_hist,
+                  m->focc_hist, m->ipr, m->ipr_hist, m->f32, m->ev, m->lz, m->vh, m->tau, m->d, m->eo, m->V, m->scr};
+    for (void* p : ps)
+        if (p) hipFree(p);
+    delete m;
+}
+
 struct fkgpu_chains {
     fkgpu_ctx* ctx = nullptr;
     int B = 0, N = 0, kind = -1, grid = 0;
@@ -82,7 +110,23 @@ struct fkgpu_chains {
     int32_t *fu_upd = nullptr, *fu_f32 = nullptr;
     int32_t* fu_skip = nullptr;      // per chain: the periodic re-decomposition is not due this step
     long long* fu_count = nullptr;   // move steps since the rank-one mode was set (device counter)
+    // on-device measurements and the graph of one measured sweep
+    chain_meas_bufs* ms = nullptr;
+    hipGraphExec_t mexec = nullptr;
+    hipGraph_t mgraph = nullptr;
+    int mgraph_len = 0;
 };
+
+// Captured graphs depend on the move set, the f-f terms and the update mode: drop them
+// whenever one of these changes.
+static void drop_graphs(fkgpu_chains* ch) {
+    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+    ch->graph_steps = 0;
+    if (ch->mexec) { hipGraphExecDestroy(ch->mexec); ch->mexec = nullptr; }
+    if (ch->mgraph) { hipGraphDestroy(ch->mgraph); ch->mgraph = nullptr; }
+    ch->mgraph_len = 0;
+}
 
 namespace {
 
@@ -1034,8 +1078,8 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
         hipSetDevice(ch->ctx->device);
         hipStreamSynchronize(ch->ctx->stream);
     }
-    if (ch->gexec) hipGraphExecDestroy(ch->gexec);
-    if (ch->graph) hipGraphDestroy(ch->graph);
+    drop_graphs(ch);
+    meas_free(ch->ms);
     void* ptrs[] = {ch->f, ch->nf, ch->logz, ch->spectrum, ch->mt, ch->beta, ch->U, ch->mu_c, ch->mu_f,
                     ch->naccept, ch->nsolve, ch->ws, ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw,
                     ch->t_logz, ch->t_acc, ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef,
@@ -1197,9 +1241,7 @@ int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, c
     }
     s.cp[nmoves - 1] = 1.0;
     if (nmoves < 2) s.ncp = 1;
-    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
-    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
-    ch->graph_steps = 0;
+    drop_graphs(ch);
     return FKGPU_OK;
 }
 
@@ -1210,9 +1252,7 @@ int fkgpu_chains_set_ff(fkgpu_chains* ch, int nW, const double* W) {
     if (nW > 0 && ch->ctx->lat_ndim != 1) nW = 0;  // calc_ff_energy returns 0 unless ndim == 1
     ch->args.nW = nW;
     for (int i = 0; i < nW; ++i) ch->args.W[i] = W[i];
-    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
-    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
-    ch->graph_steps = 0;
+    drop_graphs(ch);
     return FKGPU_OK;
 }
 
@@ -1290,9 +1330,7 @@ int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     FK_HIP(ctx, hipStreamSynchronize(st));
-    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
-    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
-    ch->graph_steps = 0;
+    drop_graphs(ch);
     if (mode == FKGPU_UPDATE_DENSE) {
         ch->fu = 0;
         return FKGPU_OK;
@@ -1355,9 +1393,7 @@ int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     FK_HIP(ctx, hipStreamSynchronize(st));
-    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
-    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
-    ch->graph_steps = 0;
+    drop_graphs(ch);
     if (!ch->k_props) {
         hipError_t e = hipSuccess;
         auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
@@ -1480,9 +1516,7 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
     ch->kM = M;
     ch->kseed = seed;
     ch->kjack = jackson;
-    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
-    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
-    ch->graph_steps = 0;
+    drop_graphs(ch);
     return FKGPU_OK;
 }
 
@@ -1612,6 +1646,295 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
     if (spectrum) FK_HIP(ctx, hipMemcpy(spectrum, ch->spectrum, sizeof(double) * BN, hipMemcpyDeviceToHost));
     if (naccept) FK_HIP(ctx, hipMemcpy(naccept, ch->naccept, sizeof(int64_t) * ch->B, hipMemcpyDeviceToHost));
     if (nsolve) FK_HIP(ctx, hipMemcpy(nsolve, ch->nsolve, sizeof(int64_t) * ch->B, hipMemcpyDeviceToHost));
+    return FKGPU_OK;
+}
+
+// ---- on-device measurements ----------------------------------------------------
+
+// The decomposition buffers a measurement needs beyond the chain set's own state, allocated
+// before any graph capture: the spectrum of stochastic-trace chains and, for the IPR, the
+// eigenvectors of chains outside the rank-one mode.
+static int meas_prepare(fkgpu_chains* ch) {
+    fkgpu_ctx* ctx = ch->ctx;
+    chain_meas_bufs* m = ch->ms;
+    if (!m) return fail(ctx, FKGPU_EINVAL, "measurements are not enabled on this chain set");
+    if (!ch->kpm && !ch->dense_ok)
+        return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
+    const bool ipr = m->flags & FKGPU_MEASURE_IPR;
+    if (!ch->kpm && !(ipr && !ch->fu)) return FKGPU_OK;
+    if (!ch->dense_ok) return fail(ctx, FKGPU_EUNSUPPORTED, "measurements need the dense eigensolver (N <= 2048)");
+    const int B = ch->B, N = ch->N;
+    const size_t BN = (size_t)B * N, BNN = BN * N;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    if (!m->f32) chk(dalloc(&m->f32, BN));
+    if (!m->ev) chk(dalloc(&m->ev, BN));
+    if (!m->lz) chk(dalloc(&m->lz, B));
+    if (ipr && !m->V) {
+        chk(dalloc(&m->vh, BNN));
+        chk(dalloc(&m->tau, BN));
+        chk(dalloc(&m->d, BN));
+        chk(dalloc(&m->eo, BN));
+        chk(dalloc(&m->V, BNN));
+        chk(dalloc(&m->scr, fk_evecs_scratch_doubles(B, N)));
+    }
+    if (e != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "measurement decomposition buffers");
+    return FKGPU_OK;
+}
+
+// One measurement of every chain, stream-ordered (capturable: no allocation, no memset).
+static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
+    fkgpu_ctx* ctx = ch->ctx;
+    chain_meas_bufs* m = ch->ms;
+    const int B = ch->B, N = ch->N;
+    const bool ipr = m->flags & FKGPU_MEASURE_IPR;
+    const double* spec = ch->spectrum;
+    const double* V = (ipr && ch->fu) ? ch->fu_V : nullptr;
+    hipError_t e;
+    if (ch->kpm || (ipr && !ch->fu)) {
+        if ((e = fk_launch_f8to32(B * N, ch->f, m->f32, st)) != hipSuccess) return e;
+        eig_args a{};
+        a.B = B;
+        a.N = N;
+        a.hop = ctx->d_hop;
+        a.f = m->f32;
+        a.beta = ch->beta;
+        a.U = ch->U;
+        a.mu_c = ch->mu_c;
+        a.evals = m->ev;
+        a.logz = m->lz;
+        a.ws = ch->ws;
+        a.ws_stride = ch->ws_stride;
+        if (ipr) {
+            a.vout = m->vh;
+            a.tauout = m->tau;
+            a.d_out = m->d;
+            a.e_out = m->eo;
+        }
+        if ((e = fk_launch_eig(ch->kind, a, ch->grid, st)) != hipSuccess) return e;
+        if (ipr) {
+            e = fk_launch_evecs(B, N, m->d, m->eo, m->ev, m->vh, m->tau, m->V, m->scr, st);
+            if (e != hipSuccess) return e;
+            V = m->V;
+        }
+        if (ch->kpm) spec = m->ev;
+    }
+    if (V && (e = fk_launch_ipr(B, N, V, m->ipr, st)) != hipSuccess) return e;
+    chain_meas_args a{};
+    a.B = B;
+    a.N = N;
+    a.cap = m->cap;
+    a.nW = ch->args.nW;
+    for (int i = 0; i < 8; ++i) a.W[i] = ch->args.W[i];
+    a.f = ch->f;
+    a.nf = ch->nf;
+    a.spectrum = spec;
+    a.beta = ch->beta;
+    a.mu_f = ch->mu_f;
+    a.ipr = V ? m->ipr : nullptr;
+    a.count = m->count;
+    a.e = m->e;
+    a.d2e = m->d2e;
+    a.ce = m->ce;
+    a.esum = m->esum;
+    a.d2sum = m->d2sum;
+    a.spec_mean = m->spec_mean;
+    a.focc_sum = m->focc_sum;
+    a.spec_hist = m->spec_hist;
+    a.focc_hist = m->focc_hist;
+    a.ipr_hist = m->ipr_hist;
+    return fk_launch_chain_measure(a, m->count, st);
+}
+
+static int meas_count(fkgpu_chains* ch, long long* count) {
+    fkgpu_ctx* ctx = ch->ctx;
+    if (!ch->ms) return fail(ctx, FKGPU_EINVAL, "measurements are not enabled on this chain set");
+    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FK_HIP(ctx, hipMemcpy(count, ch->ms->count, sizeof(long long), hipMemcpyDeviceToHost));
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | FKGPU_MEASURE_IPR;
+    if (capacity < 0 || (flags & ~all)) return fail(ctx, FKGPU_EINVAL, "invalid capacity or measurement flags");
+    if ((flags & FKGPU_MEASURE_IPR) && (!ch->dense_ok || ch->N > 1024))
+        return fail(ctx, FKGPU_EUNSUPPORTED, "the IPR measure needs eigenvectors, implemented for N <= 1024");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    drop_graphs(ch);
+    meas_free(ch->ms);
+    ch->ms = nullptr;
+    const int B = ch->B, N = ch->N;
+    const size_t BN = (size_t)B * N, Bc = (size_t)B * capacity, BcN = Bc * N;
+    chain_meas_bufs* m = new chain_meas_bufs();
+    m->cap = capacity;
+    m->flags = flags;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    chk(dalloc(&m->count, 1));
+    chk(dalloc(&m->e, Bc));
+    chk(dalloc(&m->d2e, Bc));
+    chk(dalloc(&m->ce, Bc));
+    chk(dalloc(&m->esum, B));
+    chk(dalloc(&m->d2sum, B));
+    chk(dalloc(&m->spec_mean, BN));
+    chk(dalloc(&m->focc_sum, BN));
+    if (flags & FKGPU_MEASURE_SPECTRUM_HISTORY) chk(dalloc(&m->spec_hist, BcN));
+    if (flags & FKGPU_MEASURE_FOCC_HISTORY) chk(dalloc(&m->focc_hist, BcN));
+    if (flags & FKGPU_MEASURE_IPR) {
+        chk(dalloc(&m->ipr, BN));
+        chk(dalloc(&m->ipr_hist, BcN));
+    }
+    if (e != hipSuccess) {
+        meas_free(m);
+        return fail(ctx, FKGPU_ENOMEM, std::string("measurement records: ") + hipGetErrorString(e));
+    }
+    chk(hipMemsetAsync(m->count, 0, sizeof(long long), st));
+    chk(hipMemsetAsync(m->esum, 0, sizeof(double) * B, st));
+    chk(hipMemsetAsync(m->d2sum, 0, sizeof(double) * B, st));
+    chk(hipMemsetAsync(m->spec_mean, 0, sizeof(double) * BN, st));
+    chk(hipMemsetAsync(m->focc_sum, 0, sizeof(long long) * BN, st));
+    chk(hipStreamSynchronize(st));
+    if (e != hipSuccess) {
+        meas_free(m);
+        return hipfail(ctx, e, "measurement records");
+    }
+    ch->ms = m;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_measure(fkgpu_chains* ch) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    int r = meas_prepare(ch);
+    if (r) return r;
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, meas_launch(ch, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
+int fkgpu_sweep_measure(fkgpu_chains* ch, int nsweeps, int sweep_len) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    if (nsweeps < 0 || sweep_len <= 0) return fail(ctx, FKGPU_EINVAL, "invalid sweep count or length");
+    if (!ch->kpm && !ch->dense_ok)
+        return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
+    int r = meas_prepare(ch);
+    if (r) return r;
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    if (!ch->mexec || ch->mgraph_len != sweep_len) {
+        if (ch->mexec) { hipGraphExecDestroy(ch->mexec); ch->mexec = nullptr; }
+        if (ch->mgraph) { hipGraphDestroy(ch->mgraph); ch->mgraph = nullptr; }
+        sweep_args a = ch->args;
+        a.trace = 0;
+        FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int s = 0; s < sweep_len && e == hipSuccess; ++s) {
+            a.step = s;
+            e = launch_step(ch, a, st);
+        }
+        if (e == hipSuccess) e = meas_launch(ch, st);
+        if (e != hipSuccess) {
+            hipGraph_t g = nullptr;
+            hipStreamEndCapture(st, &g);
+            if (g) hipGraphDestroy(g);
+            return hipfail(ctx, e, "graph capture");
+        }
+        FK_HIP(ctx, hipStreamEndCapture(st, &ch->mgraph));
+        FK_HIP(ctx, hipGraphInstantiate(&ch->mexec, ch->mgraph, nullptr, nullptr, 0));
+        ch->mgraph_len = sweep_len;
+    }
+    FK_HIP(ctx, hipEventRecord(ch->ev0, st));
+    for (int i = 0; i < nsweeps; ++i) FK_HIP(ctx, hipGraphLaunch(ch->mexec, st));
+    FK_HIP(ctx, hipEventRecord(ch->ev1, st));
+    FK_HIP(ctx, hipEventSynchronize(ch->ev1));
+    float ms = 0.f;
+    FK_HIP(ctx, hipEventElapsedTime(&ms, ch->ev0, ch->ev1));
+    ch->last_ms = ms;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_energies(fkgpu_chains* ch, int64_t* count, double* energies, double* d2energies,
+                          double* c_energies) {
+    if (!ch) return FKGPU_EINVAL;
+    long long c = 0;
+    int r = meas_count(ch, &c);
+    if (r) return r;
+    fkgpu_ctx* ctx = ch->ctx;
+    chain_meas_bufs* m = ch->ms;
+    if (count) *count = c;
+    const size_t n = (size_t)std::min<long long>(c, m->cap);
+    if (n == 0) return FKGPU_OK;
+    const double* src[3] = {m->e, m->d2e, m->ce};
+    double* dst[3] = {energies, d2energies, c_energies};
+    for (int i = 0; i < 3; ++i)
+        if (dst[i])
+            FK_HIP(ctx, hipMemcpy2D(dst[i], sizeof(double) * n, src[i], sizeof(double) * m->cap, sizeof(double) * n,
+                                    ch->B, hipMemcpyDeviceToHost));
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_means(fkgpu_chains* ch, int64_t* count, double* energy, double* d2energy, double* spectrum,
+                       double* focc) {
+    if (!ch) return FKGPU_EINVAL;
+    long long c = 0;
+    int r = meas_count(ch, &c);
+    if (r) return r;
+    fkgpu_ctx* ctx = ch->ctx;
+    chain_meas_bufs* m = ch->ms;
+    if (count) *count = c;
+    const int B = ch->B;
+    const size_t BN = (size_t)B * ch->N;
+    const double z = c > 0 ? (double)c : 1.0;
+    std::vector<double> tmp(B);
+    if (energy) {
+        FK_HIP(ctx, hipMemcpy(tmp.data(), m->esum, sizeof(double) * B, hipMemcpyDeviceToHost));
+        for (int b = 0; b < B; ++b) energy[b] = tmp[b] / z;
+    }
+    if (d2energy) {
+        FK_HIP(ctx, hipMemcpy(tmp.data(), m->d2sum, sizeof(double) * B, hipMemcpyDeviceToHost));
+        for (int b = 0; b < B; ++b) d2energy[b] = tmp[b] / z;
+    }
+    if (spectrum) FK_HIP(ctx, hipMemcpy(spectrum, m->spec_mean, sizeof(double) * BN, hipMemcpyDeviceToHost));
+    if (focc) {
+        std::vector<long long> cnt(BN);
+        FK_HIP(ctx, hipMemcpy(cnt.data(), m->focc_sum, sizeof(long long) * BN, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < BN; ++i) focc[i] = (double)cnt[i] / z;
+    }
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_histories(fkgpu_chains* ch, int64_t* count, double* spectrum, int32_t* focc, double* ipr) {
+    if (!ch) return FKGPU_EINVAL;
+    long long c = 0;
+    int r = meas_count(ch, &c);
+    if (r) return r;
+    fkgpu_ctx* ctx = ch->ctx;
+    chain_meas_bufs* m = ch->ms;
+    if ((spectrum && !m->spec_hist) || (focc && !m->focc_hist) || (ipr && !m->ipr_hist))
+        return fail(ctx, FKGPU_EINVAL, "history requested that was not enabled");
+    if (count) *count = c;
+    const size_t n = (size_t)std::min<long long>(c, m->cap), N = ch->N;
+    if (n == 0) return FKGPU_OK;
+    const size_t row = n * N, srow = (size_t)m->cap * N;
+    if (spectrum)
+        FK_HIP(ctx, hipMemcpy2D(spectrum, sizeof(double) * row, m->spec_hist, sizeof(double) * srow,
+                                sizeof(double) * row, ch->B, hipMemcpyDeviceToHost));
+    if (ipr)
+        FK_HIP(ctx, hipMemcpy2D(ipr, sizeof(double) * row, m->ipr_hist, sizeof(double) * srow, sizeof(double) * row,
+                                ch->B, hipMemcpyDeviceToHost));
+    if (focc) {
+        std::vector<uint8_t> f8(row * ch->B);
+        FK_HIP(ctx, hipMemcpy2D(f8.data(), row, m->focc_hist, srow, row, ch->B, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f8.size(); ++i) focc[i] = f8[i];
+    }
     return FKGPU_OK;
 }
 

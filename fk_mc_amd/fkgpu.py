@@ -19,6 +19,7 @@ MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
 SOLVER_AUTO, SOLVER_VARIANTS = -1, 6
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 UPDATE_DENSE, UPDATE_RANK1 = 0, 1
+MEASURE_SPECTRUM_HISTORY, MEASURE_FOCC_HISTORY, MEASURE_IPR = 1, 2, 4
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -100,6 +101,12 @@ def lib():
         L.fkgpu_get_solver.argtypes = [P]
         L.fkgpu_autotune.argtypes = [P, C.c_int, _d]
         L.fkgpu_kpm_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, C.c_int, C.c_uint64, C.c_int, _d, _d, _d]
+        L.fkgpu_chains_measure_enable.argtypes = [P, C.c_int, C.c_int]
+        L.fkgpu_chains_measure.argtypes = [P]
+        L.fkgpu_sweep_measure.argtypes = [P, C.c_int, C.c_int]
+        L.fkgpu_chains_energies.argtypes = [P, _i64, _d, _d, _d]
+        L.fkgpu_chains_means.argtypes = [P, _i64, _d, _d, _d, _d]
+        L.fkgpu_chains_histories.argtypes = [P, _i64, _d, _i32, _d]
         _LIB = L
     return _LIB
 
@@ -401,6 +408,49 @@ class Chains:
         self.ctx._check(lib().fkgpu_chains_state(self._h, _p(f, _i32), _p(nf, _i32), _p(lz, _d), _p(sp, _d),
                                                  _p(na, _i64), _p(ns, _i64)))
         return dict(f=f, nf=nf, logZ=lz, spectrum=sp, naccept=na, nsolve=ns)
+
+    # ---- on-device measurements (fkgpu_chains_measure_enable and friends) ----
+    def measure_enable(self, capacity, flags=0):
+        self.ctx._check(lib().fkgpu_chains_measure_enable(self._h, capacity, flags))
+        self._mcap, self._mflags = capacity, flags
+
+    def measure(self):
+        self.ctx._check(lib().fkgpu_chains_measure(self._h))
+
+    def sweep_measure(self, nsweeps, sweep_len):
+        self.ctx._check(lib().fkgpu_sweep_measure(self._h, nsweeps, sweep_len))
+
+    def _mcount(self):
+        c = np.zeros(1, np.int64)
+        self.ctx._check(lib().fkgpu_chains_energies(self._h, _p(c, _i64), None, None, None))
+        return int(c[0])
+
+    def energies(self):
+        """(count, energies, d2energies, c_energies), each series [B, min(count, capacity)]."""
+        n = min(self._mcount(), self._mcap)
+        e, d2, ce = (np.zeros((self.B, n)) for _ in range(3))
+        c = np.zeros(1, np.int64)
+        self.ctx._check(lib().fkgpu_chains_energies(self._h, _p(c, _i64), _p(e, _d), _p(d2, _d), _p(ce, _d)))
+        return int(c[0]), e, d2, ce
+
+    def means(self):
+        B, N = self.B, self.N
+        c = np.zeros(1, np.int64)
+        e, d2 = np.zeros(B), np.zeros(B)
+        sp, fo = np.zeros((B, N)), np.zeros((B, N))
+        self.ctx._check(lib().fkgpu_chains_means(self._h, _p(c, _i64), _p(e, _d), _p(d2, _d), _p(sp, _d), _p(fo, _d)))
+        return dict(count=int(c[0]), energy=e, d2energy=d2, spectrum=sp, focc=fo)
+
+    def histories(self):
+        """Enabled histories, each [B, min(count, capacity), N] (None where not enabled)."""
+        B, N = self.B, self.N
+        n = min(self._mcount(), self._mcap)
+        sp = np.zeros((B, n, N)) if self._mflags & MEASURE_SPECTRUM_HISTORY else None
+        fo = np.zeros((B, n, N), np.int32) if self._mflags & MEASURE_FOCC_HISTORY else None
+        ip = np.zeros((B, n, N)) if self._mflags & MEASURE_IPR else None
+        c = np.zeros(1, np.int64)
+        self.ctx._check(lib().fkgpu_chains_histories(self._h, _p(c, _i64), _p(sp, _d), _p(fo, _i32), _p(ip, _d)))
+        return dict(count=int(c[0]), spectrum=sp, focc=fo, ipr=ip)
 
 
 def rng_draws(seed, mode, count, hi=0, probs=None):

@@ -225,6 +225,47 @@ int fkgpu_chains_last_ms(const fkgpu_chains* ch, double* ms);
 int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, double* spectrum,
                        int64_t* naccept, int64_t* nsolve);
 
+/* ---- On-device measurements of a chain set ----------------------------------
+ * The measures that mc_metropolis::measure (src/mc_metropolis.cpp:53-60) calls after
+ * every sweep, taken for every chain on the device and kept there until read:
+ *   measure_energy::accumulate            src/measures/energy.cpp:6-26
+ *     E = sum_k e_k f(e_k) - mu_f N_f + E_ff, d2E = sum_k e_k^2 f(1 - f), E_c
+ *   measure_spectrum::accumulate          src/measures/spectrum.cpp:13-21 (running mean)
+ *   measure_spectrum_history::accumulate  src/measures/spectrum_history.cpp:13-19
+ *   measure_focc::accumulate              src/measures/focc_history.cpp:8-13
+ *   measure_ipr::accumulate               include/fk_mc/measures/ipr.hpp:39-56
+ * The spectrum is the chain's own (the rank-one, mixed and FP64 modes keep it); chain
+ * sets on the stochastic trace diagonalise their configurations at each measurement,
+ * as the reference's calc_ed(false) does, and the IPR takes the eigenvectors of the
+ * rank-one mode or of an FP64 decomposition (N <= 1024).  A measurement changes no
+ * chain state: the chains draw and decide exactly as without it.
+ * The records of the first `capacity` measurements are kept (series and histories,
+ * chain-major); later ones enter the means and the count only. */
+#define FKGPU_MEASURE_SPECTRUM_HISTORY 1
+#define FKGPU_MEASURE_FOCC_HISTORY 2
+#define FKGPU_MEASURE_IPR 4
+/* Allocate the records and clear them (measurement count 0).  Replaces the
+ * observables the reference's measures are registered on (include/fk_mc/fk_mc.hxx:97-123). */
+int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags);
+/* One measurement of every chain now (measure_*::accumulate). */
+int fkgpu_chains_measure(fkgpu_chains* ch);
+/* nsweeps x (sweep_len move steps, then one measurement), replayed from one captured
+ * HIP graph: the measured part of mcbase::run with a sweep of sweep_len moves. */
+int fkgpu_sweep_measure(fkgpu_chains* ch, int nsweeps, int sweep_len);
+/* Measurements taken so far (*count), and the energy series of the first
+ * n = min(count, capacity) of them, [nchains][n] (nullable outputs):
+ * the reference's _energies, _d2energies, _c_energies. */
+int fkgpu_chains_energies(fkgpu_chains* ch, int64_t* count, double* energies, double* d2energies,
+                          double* c_energies);
+/* Means over all measurements (nullable outputs): energy and d2energy [nchains]
+ * (_average_energy / _Z, _average_d2energy / _Z), spectrum [nchains*N] (the reference's
+ * running mean, same order of operations), occupation of each site [nchains*N]. */
+int fkgpu_chains_means(fkgpu_chains* ch, int64_t* count, double* energy, double* d2energy, double* spectrum,
+                       double* focc);
+/* Histories of the first n = min(count, capacity) measurements, [nchains][n][N]
+ * (nullable outputs; each must have been enabled): spectra, occupations, IPRs. */
+int fkgpu_chains_histories(fkgpu_chains* ch, int64_t* count, double* spectrum, int32_t* focc, double* ipr);
+
 /* Diagnostic (host only, no device needed; no reference counterpart): draws
  * from the library's port of std::mt19937(seed) and the libstdc++
  * distributions used by the reference -- mode 0 raw words, 1

@@ -5,12 +5,13 @@ Workload (the batched FP64 configuration): cubic2d L=16 (N=256 sites),
 1024 independent chains, beta_c = 0.5 * 40**(c/1023) (log-spaced in [0.5, 20]),
 U_c = {1,2,4,8}[c mod 4], mu_c = mu_f = U/2, f drawn Bernoulli(0.5) per chain,
 moves: single-site flip (move_addremove) and nearest-neighbour exchange at
-probability 0.5 / 0.5.  A step is one Metropolis move of every chain: one launch
-of the fused kernel (proposal, H assembly, FP64 Householder tridiagonalisation,
-tridiagonal eigenvalues, logZ, accept test).  Inputs are resident in HBM.
+probability 0.5 / 0.5.  A step is one Metropolis move of every chain (proposal,
+H assembly, FP64 Householder tridiagonalisation, tridiagonal eigenvalues, logZ,
+accept test); the fused kernel runs 16 steps per launch, one workgroup per
+(step, chain) item with each chain's steps in order.  Inputs are resident in HBM.
 
 Printed: one JSON line with the contract fields plus
-  roofline      FP64 rate of the sweep kernel: (4/3) N^3 flop per eigensolve
+  roofline      FP64 rate of the move-step kernel: (4/3) N^3 flop per eigensolve
                 x eigensolves done / kernel time (HIP events on the library
                 stream) against MI355X's 78.6 TFLOP/s FP64 peak;
   plain         the single-chain drop-in path (calc_hamiltonian + calc_ed
@@ -42,13 +43,14 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per launch of the headline move-step kernel on the default workload (cubic2d
-# L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (scales
-# from a 256 MiB copy, bench.py --copy-calibration); keyed by solver variant
-TRAFFIC_BYTES = {5: 13.26e6}
-KERNEL_NAMES = {0: "fkk::k_sweep<reg_solver<16,4>>", 1: "fkk::k_sweep<reg_solver<16,8>>",
-                2: "fkk::k_sweep<reg_solver<32,8>>", 3: "fkk::k_sweep<glob_solver>",
-                4: "fkk::k_sweep<blocked_solver<double,512>>", 5: "fkk::k_sweep<wrow_solver>"}
+# HBM bytes per launch (16 move steps) of the headline move-step kernel on the default
+# workload (cubic2d L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 +
+# WRITE_SIZE x 1.00 (scales from a 256 MiB copy, bench.py --copy-calibration); keyed by
+# solver variant
+TRAFFIC_BYTES = {5: 223.86e6}
+KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",
+                2: "fkk::k_sweep_q<reg_solver<32,8>>", 3: "fkk::k_sweep_q<glob_solver>",
+                4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver>"}
 
 
 def workload(L=16, B=1024, seed=20240601):
@@ -122,11 +124,17 @@ def main():
     acc = int((s1["naccept"] - s0["naccept"]).sum())
     flop = solves * (4.0 / 3.0) * N ** 3
     achieved = flop / (kern_ms / 1e3) / 1e12
+    # one launch runs steps_per_launch move steps of every chain (one workgroup per
+    # (step, chain) item); the HIP-event time also holds one tiny queue-reset launch each
+    spl = ch.steps_per_launch
+    launches = -(-args.steps // spl)
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                "traffic": TRAFFIC_BYTES.get(ctx.get_solver()) if (L, B) == (16, 1024) else None,
-                "kernel": KERNEL_NAMES.get(ctx.get_solver(), "k_sweep"), "kernel_ms_per_launch": round(kern_ms / args.steps, 4),
-                "flop_per_launch": flop / args.steps, "eigensolves_per_launch": solves / args.steps}
+                "traffic": TRAFFIC_BYTES.get(ctx.get_solver())
+                if (L, B, spl) == (16, 1024, 16) and args.steps % spl == 0 else None,
+                "kernel": KERNEL_NAMES.get(ctx.get_solver(), "k_sweep_q"), "steps_per_launch": spl,
+                "kernel_ms_per_launch": round(kern_ms / launches, 4),
+                "flop_per_launch": flop / launches, "eigensolves_per_launch": solves / launches}
 
     # plain single-chain drop-in: one calc_hamiltonian+calc_ed per move, host buffers
     plain = None

@@ -115,6 +115,15 @@ int fk_solver_nmax(int kind);
 size_t fk_solver_ws_doubles(int kind, int N);
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
+// Work queue of a multi-step sweep launch: an item counter, per-chain step marks (B),
+// an error flag set when a wait exceeds wait_ticks of the wall clock.
+struct sweep_queue {
+    unsigned* ctr;
+    int* done;
+    int* err;
+    long long wait_ticks;
+};
+hipError_t fk_launch_sweep_q(int kind, const sweep_args& a, int nsub, const sweep_queue& q, hipStream_t st);
 
 // scratch: fk_evecs_scratch_doubles(B, N) doubles (at least 3 B N^2)
 hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,

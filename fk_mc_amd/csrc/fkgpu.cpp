@@ -110,6 +110,11 @@ struct fkgpu_chains {
     int32_t *fu_upd = nullptr, *fu_f32 = nullptr;
     int32_t* fu_skip = nullptr;      // per chain: the periodic re-decomposition is not due this step
     long long* fu_count = nullptr;   // move steps since the rank-one mode was set (device counter)
+    // several dense FP64 move steps per launch (k_sweep_q work queue)
+    int spl = 16;                    // move steps per launch
+    unsigned* q_ctr = nullptr;
+    int32_t *q_done = nullptr, *q_err = nullptr;
+    long long q_ticks = 0;           // bound of one wait, in wall-clock ticks
     // on-device measurements and the graph of one measured sweep
     chain_meas_bufs* ms = nullptr;
     hipGraphExec_t mexec = nullptr;
@@ -1083,7 +1088,7 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     void* ptrs[] = {ch->f, ch->nf, ch->logz, ch->spectrum, ch->mt, ch->beta, ch->U, ch->mu_c, ch->mu_f,
                     ch->naccept, ch->nsolve, ch->ws, ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw,
                     ch->t_logz, ch->t_acc, ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef,
-                    ch->k_mom, ch->k_lz};
+                    ch->k_mom, ch->k_lz, ch->q_ctr, ch->q_done, ch->q_err};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ch->ev0) hipEventDestroy(ch->ev0);
@@ -1132,6 +1137,15 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     if (ch->dense_ok && fk_solver_needs_ws(ch->kind)) chk(dalloc(&ch->ws, (size_t)ch->grid * ch->ws_stride));
     chk(hipEventCreate(&ch->ev0));
     chk(hipEventCreate(&ch->ev1));
+    chk(dalloc(&ch->q_ctr, 1));
+    chk(dalloc(&ch->q_done, B));
+    chk(dalloc(&ch->q_err, 1));
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || khz <= 0)
+            khz = 100000;
+        ch->q_ticks = 30LL * 1000LL * khz;  // 30 s: far beyond any single move step
+    }
     if (e != hipSuccess) {
         fkgpu_chains_destroy(ch);
         return fail(ctx, FKGPU_ENOMEM, std::string("chain allocation: ") + hipGetErrorString(e));
@@ -1156,6 +1170,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     chk(hipMemcpyAsync(ch->mu_f, mu_f, sizeof(double) * B, hipMemcpyHostToDevice, st));
     chk(hipMemcpyAsync(ch->naccept, zeros.data(), sizeof(long long) * B, hipMemcpyHostToDevice, st));
     chk(hipMemcpyAsync(ch->nsolve, zeros.data(), sizeof(long long) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemsetAsync(ch->q_err, 0, sizeof(int32_t), st));
     if (e != hipSuccess) {
         fkgpu_chains_destroy(ch);
         return hipfail(ctx, e, "chain upload");
@@ -1319,6 +1334,52 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
                       ch->k_lz, st);
     if (e != hipSuccess) return e;
     return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->k_lz, st);
+}
+
+// K move steps of every chain from trace row a.step on: one work-queue launch for the
+// dense FP64 mode, one launch_step per step otherwise.
+static hipError_t launch_steps(fkgpu_chains* ch, sweep_args a, int K, hipStream_t st) {
+    if (K > 1 && !ch->fu && !ch->mixed && !ch->kpm) {
+        sweep_queue q{ch->q_ctr, ch->q_done, ch->q_err, ch->q_ticks};
+        return fk_launch_sweep_q(ch->kind, a, K, q, st);
+    }
+    const int step0 = a.step;
+    for (int s = 0; s < K; ++s) {
+        a.step = step0 + s;
+        hipError_t e = launch_step(ch, a, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// nsteps move steps in launches of at most spl steps each.
+static hipError_t launch_sweep_chunks(fkgpu_chains* ch, const sweep_args& a0, int nsteps, hipStream_t st) {
+    sweep_args a = a0;
+    for (int s = 0; s < nsteps; s += ch->spl) {
+        a.step = s;
+        hipError_t e = launch_steps(ch, a, std::min(ch->spl, nsteps - s), st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// The work queue's wait bound expired in a launch since the chain set was created.
+static int queue_check(fkgpu_chains* ch) {
+    int32_t err = 0;
+    FK_HIP(ch->ctx, hipMemcpy(&err, ch->q_err, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) return fail(ch->ctx, FKGPU_EHIP, "move-step work queue: a wait exceeded its bound; chain state undefined");
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_set_steps_per_launch(fkgpu_chains* ch, int steps) {
+    if (!ch) return FKGPU_EINVAL;
+    if (steps < 1 || (long long)steps * ch->B > 0x7fffffffLL)
+        return fail(ch->ctx, FKGPU_EINVAL, "steps per launch must be >= 1 (and steps x chains < 2^31)");
+    hipSetDevice(ch->ctx->device);
+    FK_HIP(ch->ctx, hipStreamSynchronize(ch->ctx->stream));
+    ch->spl = steps;
+    drop_graphs(ch);
+    return FKGPU_OK;
 }
 
 int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
@@ -1561,15 +1622,13 @@ int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
     if (!ch->kpm && !ch->dense_ok)
         return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
     FK_HIP(ctx, hipEventRecord(ch->ev0, st));
-    for (int s = 0; s < nsteps; ++s) {
-        a.step = s;
-        FK_HIP(ctx, launch_step(ch, a, st));
-    }
+    FK_HIP(ctx, launch_sweep_chunks(ch, a, nsteps, st));
     FK_HIP(ctx, hipEventRecord(ch->ev1, st));
     FK_HIP(ctx, hipEventSynchronize(ch->ev1));
     float ms = 0.f;
     FK_HIP(ctx, hipEventElapsedTime(&ms, ch->ev0, ch->ev1));
     ch->last_ms = ms;
+    if (int r = queue_check(ch)) return r;
     if (trace && nsteps > 0) {
         auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
             return dst ? hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) : hipSuccess;
@@ -1599,15 +1658,12 @@ int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
         if (!ch->kpm && !ch->dense_ok)
             return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
         FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < nsteps; ++s) {
-            a.step = s;
-            hipError_t e = launch_step(ch, a, st);
-            if (e != hipSuccess) {
-                hipGraph_t g;
-                hipStreamEndCapture(st, &g);
-                if (g) hipGraphDestroy(g);
-                return hipfail(ctx, e, "graph capture");
-            }
+        hipError_t e = launch_sweep_chunks(ch, a, nsteps, st);
+        if (e != hipSuccess) {
+            hipGraph_t g = nullptr;
+            hipStreamEndCapture(st, &g);
+            if (g) hipGraphDestroy(g);
+            return hipfail(ctx, e, "graph capture");
         }
         FK_HIP(ctx, hipStreamEndCapture(st, &ch->graph));
         FK_HIP(ctx, hipGraphInstantiate(&ch->gexec, ch->graph, nullptr, nullptr, 0));
@@ -1620,7 +1676,7 @@ int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
     float ms = 0.f;
     FK_HIP(ctx, hipEventElapsedTime(&ms, ch->ev0, ch->ev1));
     ch->last_ms = ms;
-    return FKGPU_OK;
+    return queue_check(ch);
 }
 
 int fkgpu_chains_last_ms(const fkgpu_chains* ch, double* ms) {
@@ -1835,11 +1891,7 @@ int fkgpu_sweep_measure(fkgpu_chains* ch, int nsweeps, int sweep_len) {
         sweep_args a = ch->args;
         a.trace = 0;
         FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipSuccess;
-        for (int s = 0; s < sweep_len && e == hipSuccess; ++s) {
-            a.step = s;
-            e = launch_step(ch, a, st);
-        }
+        hipError_t e = launch_sweep_chunks(ch, a, sweep_len, st);
         if (e == hipSuccess) e = meas_launch(ch, st);
         if (e != hipSuccess) {
             hipGraph_t g = nullptr;
@@ -1858,7 +1910,7 @@ int fkgpu_sweep_measure(fkgpu_chains* ch, int nsweeps, int sweep_len) {
     float ms = 0.f;
     FK_HIP(ctx, hipEventElapsedTime(&ms, ch->ev0, ch->ev1));
     ch->last_ms = ms;
-    return FKGPU_OK;
+    return queue_check(ch);
 }
 
 int fkgpu_chains_energies(fkgpu_chains* ch, int64_t* count, double* energies, double* d2energies,

@@ -101,6 +101,7 @@ def lib():
         L.fkgpu_get_solver.argtypes = [P]
         L.fkgpu_autotune.argtypes = [P, C.c_int, _d]
         L.fkgpu_kpm_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, C.c_int, C.c_uint64, C.c_int, _d, _d, _d]
+        L.fkgpu_chains_set_steps_per_launch.argtypes = [P, C.c_int]
         L.fkgpu_chains_measure_enable.argtypes = [P, C.c_int, C.c_int]
         L.fkgpu_chains_measure.argtypes = [P]
         L.fkgpu_sweep_measure.argtypes = [P, C.c_int, C.c_int]
@@ -332,6 +333,7 @@ class Chains:
         _LIVE.append(weakref.ref(self))
         f = np.ascontiguousarray(np.atleast_2d(f_init), dtype=np.int32)
         self.B, self.N = f.shape
+        self.steps_per_launch = 16  # the library's default
         B = self.B
 
         def vec(x, dt=np.float64):
@@ -377,6 +379,10 @@ class Chains:
                    _p(tr["logw"], _d), _p(tr["logz"], _d), _p(tr["acc"], _i8))
         self.ctx._check(lib().fkgpu_sweep(self._h, nsteps, C.byref(t)))
         return {k: v.reshape(nsteps, self.B) for k, v in tr.items()}
+
+    def set_steps_per_launch(self, steps):
+        self.ctx._check(lib().fkgpu_chains_set_steps_per_launch(self._h, steps))
+        self.steps_per_launch = steps
 
     def set_precision(self, precision):
         self.ctx._check(lib().fkgpu_chains_set_precision(self._h, precision))

@@ -115,11 +115,13 @@ int fk_solver_nmax(int kind);
 size_t fk_solver_ws_doubles(int kind, int N);
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
-// Work queue of a multi-step sweep launch: an item counter, per-chain step marks (B),
-// an error flag set when a wait exceeds wait_ticks of the wall clock.
+// Work queue of a multi-step sweep launch: take and append counters (2), a ring of
+// max_steps * B slots, an error flag set when a wait exceeds wait_ticks of the wall
+// clock.
 struct sweep_queue {
     unsigned* ctr;
-    int* done;
+    int* ring;
+    int max_steps;
     int* err;
     long long wait_ticks;
 };

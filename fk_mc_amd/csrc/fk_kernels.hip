@@ -5,8 +5,8 @@
 //                          matrix resident in VGPRs, one workgroup per config.
 //   k_eig_logz_glob        same for larger N: matrix in an HBM workspace.
 //   k_sweep_q<Solver>      several move steps of every chain in one launch (one
-//                          workgroup per (step, chain) item, taken in order from a
-//                          counter; each chain's steps stay in order)
+//                          workgroup per (step, chain) item, chains taken from a queue
+//                          of ready chains; each chain's steps stay in order)
 //   k_sweep<Solver>        one Metropolis move for every chain of a batch in one
 //                          launch: move choice, proposal, H assembly, eigensolve,
 //                          logZ, accept test, state update
@@ -241,58 +241,69 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
 }
 
 // ---------------------------------------------------------------------------
-// nsub Metropolis move steps of every chain in one launch of nsub * B workgroups.
-// A workgroup takes the next (step, chain) item from a counter when it starts; item
-// (s, c) waits until chain c has finished step s - 1 (published by the workgroup that
-// ran it with an agent-scope release, read with an acquire), so each chain's moves
-// stay in order and consume the same random words as one launch per step, while the
-// solves of all chains and steps fill the CUs without a launch boundary (and its tail
-// of idle CUs) between steps.  Items are taken only by running workgroups and item
-// s*B + c waits only on item (s-1)*B + c, taken earlier, so the smallest unfinished
-// item never waits; each wait is also bounded by wait_ticks of the wall clock, after
-// which err is set and every later workgroup leaves at once.
+// nsub Metropolis move steps of every chain in one launch of nsub * B workgroups,
+// scheduled through a queue of ready chains: ring[0..B) holds every chain (step 0);
+// the workgroup that finishes step s < nsub - 1 of chain c appends (s + 1) * B + c
+// (chain and step in one word, so the slot alone tells the next workgroup both).
+// Every wave's stores of the chain's state reach the L2 before the workgroup's last
+// barrier, and thread 0's release writes the L2 back before it fills the slot.  A
+// workgroup takes
+// the next ring slot when it starts and waits until the slot is filled, so a free CU
+// always runs the chain that became ready first, each chain's steps stay in order and
+// consume the same random words as one launch per step, and the solves of all chains
+// and steps stream through the CUs without a launch boundary between steps.  Slot
+// fills are agent-scope releases (the chain's state was written by a workgroup that
+// may sit on another XCD), slot reads acquires.  Slot q is filled by the B initial
+// entries or by a completion; if every running workgroup waited, all filled slots would
+// be completed items, which makes the fills equal B plus the completions that append,
+// i.e. every chain would have finished its last step and no slot would be left to wait
+// on.  Each wait is also bounded by wait_ticks of the wall clock, after which err is
+// set and every later workgroup leaves at once.
 // ---------------------------------------------------------------------------
 template <class S>
 __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsigned* __restrict__ qctr,
-                                                   int* __restrict__ done, int* __restrict__ err,
+                                                   int* __restrict__ ring, int* __restrict__ err,
                                                    long long wait_ticks) {
     __shared__ typename S::smem sm;
     __shared__ uint8_t f_lds[S::NMAX];
     __shared__ uint32_t mt[fkrng::MT_WORDS];
     __shared__ move_prop prop;
-    __shared__ int acc_s, item_s;
+    __shared__ int acc_s, chain_s, step_s;
     __shared__ sweep_args sa;
     const int tid = threadIdx.x;
     const int N = a.N, B = a.B;
     if (tid == 0) {
         sa = a;
-        int it = -1;
+        int c = -1, st = 0;
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            const unsigned q = atomicAdd(qctr, 1u);
+            const unsigned q = atomicAdd(&qctr[0], 1u);
             if (q < (unsigned)nsub * (unsigned)B) {
-                it = (int)q;
-                const int s = (int)(q / (unsigned)B), c = (int)(q % (unsigned)B);
-                if (s > 0) {
-                    const long long t0 = wall_clock64();
-                    while (__hip_atomic_load(&done[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < s) {
-                        if (wall_clock64() - t0 > wait_ticks) {
-                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            it = -1;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(2);
+                const long long t0 = wall_clock64();
+                int v;
+                while ((v = __hip_atomic_load(&ring[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < 0) {
+                    if (wall_clock64() - t0 > wait_ticks) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
                     }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                if (v >= 0) {
+                    c = v % B;
+                    st = v / B;
                 }
             }
         }
-        item_s = it;
+        chain_s = c;
+        step_s = st;
     }
     __syncthreads();
-    // thread 0's agent-scope acquire and the barrier order every wave's loads after the
-    // previous step's stores; the item is uniform (held in scalar registers)
-    const int item = __builtin_amdgcn_readfirstlane(item_s);
-    if (item < 0) return;
-    const int s = item / B, ch = item % B;
+    // every wave acquires at agent scope after thread 0 saw the slot filled, so its loads
+    // of the chain's state follow the stores of the step before (which may have run on
+    // another XCD); chain and step are uniform (held in scalar registers)
+    const int ch = __builtin_amdgcn_readfirstlane(chain_s);
+    if (ch < 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int s = __builtin_amdgcn_readfirstlane(step_s);
     uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
     for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;
@@ -304,11 +315,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     }
     __syncthreads();
     double lz_new = 0.0;
-    if (prop.solve) {
-        // workspace solvers: one workspace per chain (the grid has nsub * B groups)
-        double* ws = a.ws ? a.ws + (size_t)ch * a.ws_stride : nullptr;
-        lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
-    }
+    if (prop.solve) lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (tid == 0) {
         double logw;
         const int acc = decide_move(prop, prop.sa >= 0 ? f_lds[prop.sa] : 0, lz_new, a.logz[ch], beta, mu_f,
@@ -323,16 +330,29 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
             for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = S::evptr(sm)[i];
     }
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mtg[i] = mt[i];
-    // the barrier orders every wave's stores before thread 0's agent-scope release
+    // every wave waits until its stores have reached the L2 (s_waitcnt vmcnt(0)) before
+    // the barrier; thread 0's agent-scope release then writes the XCD's L2 back before
+    // the slot is filled
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&done[ch], s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0 && s + 1 < nsub) {
+        const unsigned slot = atomicAdd(&qctr[1], 1u);
+        if (slot < (unsigned)nsub * (unsigned)B)  // always true: one append per step but the last
+            __hip_atomic_store(&ring[slot], (s + 1) * B + ch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
-// Clears the item counter and the per-chain step marks before a k_sweep_q launch.
-__global__ void k_sweep_q_reset(int B, unsigned* __restrict__ qctr, int* __restrict__ done) {
+// Queue of a k_sweep_q launch: slots [0, B) hold the chains in order at step 0, the
+// rest are empty; take counter 0, append counter B.
+__global__ void k_sweep_q_reset(int B, int nslots, unsigned* __restrict__ qctr, int* __restrict__ ring) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) qctr[0] = 0u;
-    if (i < B) done[i] = 0;
+    if (i == 0) {
+        qctr[0] = 0u;
+        qctr[1] = (unsigned)B;
+    }
+    if (i < nslots) ring[i] = i < B ? i : -1;
 }
 
 // ---- Metropolis moves with the stochastic-trace logZ (three launches per step) ----
@@ -401,11 +421,11 @@ static hipError_t launch_sweep(const sweep_args& a, int grid, hipStream_t st) {
 }
 template <class S>
 static hipError_t launch_sweep_q(const sweep_args& a, int nsub, const sweep_queue& q, hipStream_t st) {
-    hipLaunchKernelGGL(k_sweep_q_reset, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, q.ctr, q.done);
+    const int nslots = nsub * a.B;
+    hipLaunchKernelGGL(k_sweep_q_reset, dim3((nslots + 255) / 256), dim3(256), 0, st, a.B, nslots, q.ctr, q.ring);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(k_sweep_q<S>, dim3(nsub * a.B), dim3(S::NT), 0, st, a, nsub, q.ctr, q.done, q.err,
-                       q.wait_ticks);
+    hipLaunchKernelGGL(k_sweep_q<S>, dim3(nslots), dim3(S::NT), 0, st, a, nsub, q.ctr, q.ring, q.err, q.wait_ticks);
     return hipGetLastError();
 }
 
@@ -487,16 +507,14 @@ hipError_t fk_launch_kpm_accept(const sweep_args& a, const move_prop* props, con
     return hipGetLastError();
 }
 hipError_t fk_launch_sweep_q(int kind, const sweep_args& a, int nsub, const sweep_queue& q, hipStream_t st) {
-    if (nsub < 1 || (long long)nsub * a.B > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (nsub < 1 || nsub > q.max_steps || (long long)nsub * a.B > 0x7fffffffLL) return hipErrorInvalidValue;
     switch (kind) {
         case 0: return launch_sweep_q<reg_solver<16, 4>>(a, nsub, q, st);
         case 1: return launch_sweep_q<reg_solver<16, 8>>(a, nsub, q, st);
         case 2: return launch_sweep_q<reg_solver<32, 8>>(a, nsub, q, st);
-        case 3: return launch_sweep_q<glob_solver>(a, nsub, q, st);
-        case 4: return launch_sweep_q<blk64>(a, nsub, q, st);
         case 5: return launch_sweep_q<wrow_solver>(a, nsub, q, st);
     }
-    return hipErrorInvalidValue;
+    return hipErrorInvalidValue;  // workspace solvers: one launch per step
 }
 
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st) {

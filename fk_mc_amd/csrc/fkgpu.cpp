@@ -113,7 +113,7 @@ struct fkgpu_chains {
     // several dense FP64 move steps per launch (k_sweep_q work queue)
     int spl = 16;                    // move steps per launch
     unsigned* q_ctr = nullptr;
-    int32_t *q_done = nullptr, *q_err = nullptr;
+    int32_t *q_ring = nullptr, *q_err = nullptr;  // ring: spl x B slots
     long long q_ticks = 0;           // bound of one wait, in wall-clock ticks
     // on-device measurements and the graph of one measured sweep
     chain_meas_bufs* ms = nullptr;
@@ -1088,7 +1088,7 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     void* ptrs[] = {ch->f, ch->nf, ch->logz, ch->spectrum, ch->mt, ch->beta, ch->U, ch->mu_c, ch->mu_f,
                     ch->naccept, ch->nsolve, ch->ws, ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw,
                     ch->t_logz, ch->t_acc, ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef,
-                    ch->k_mom, ch->k_lz, ch->q_ctr, ch->q_done, ch->q_err};
+                    ch->k_mom, ch->k_lz, ch->q_ctr, ch->q_ring, ch->q_err};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ch->ev0) hipEventDestroy(ch->ev0);
@@ -1137,8 +1137,8 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     if (ch->dense_ok && fk_solver_needs_ws(ch->kind)) chk(dalloc(&ch->ws, (size_t)ch->grid * ch->ws_stride));
     chk(hipEventCreate(&ch->ev0));
     chk(hipEventCreate(&ch->ev1));
-    chk(dalloc(&ch->q_ctr, 1));
-    chk(dalloc(&ch->q_done, B));
+    chk(dalloc(&ch->q_ctr, 2));
+    chk(dalloc(&ch->q_ring, (size_t)ch->spl * B));
     chk(dalloc(&ch->q_err, 1));
     {
         int khz = 0;
@@ -1337,10 +1337,12 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
 }
 
 // K move steps of every chain from trace row a.step on: one work-queue launch for the
-// dense FP64 mode, one launch_step per step otherwise.
+// dense FP64 mode with a register-resident solver, one launch_step per step otherwise
+// (the workspace solvers keep their matrices in the L2, which the queue's cross-XCD
+// hand-over writes back and invalidates at every step).
 static hipError_t launch_steps(fkgpu_chains* ch, sweep_args a, int K, hipStream_t st) {
-    if (K > 1 && !ch->fu && !ch->mixed && !ch->kpm) {
-        sweep_queue q{ch->q_ctr, ch->q_done, ch->q_err, ch->q_ticks};
+    if (K > 1 && !ch->fu && !ch->mixed && !ch->kpm && !fk_solver_needs_ws(ch->kind)) {
+        sweep_queue q{ch->q_ctr, ch->q_ring, ch->spl, ch->q_err, ch->q_ticks};
         return fk_launch_sweep_q(ch->kind, a, K, q, st);
     }
     const int step0 = a.step;
@@ -1377,6 +1379,12 @@ int fkgpu_chains_set_steps_per_launch(fkgpu_chains* ch, int steps) {
         return fail(ch->ctx, FKGPU_EINVAL, "steps per launch must be >= 1 (and steps x chains < 2^31)");
     hipSetDevice(ch->ctx->device);
     FK_HIP(ch->ctx, hipStreamSynchronize(ch->ctx->stream));
+    if (steps > ch->spl) {
+        int32_t* ring = nullptr;
+        if (dalloc(&ring, (size_t)steps * ch->B) != hipSuccess) return fail(ch->ctx, FKGPU_ENOMEM, "chain queue");
+        hipFree(ch->q_ring);
+        ch->q_ring = ring;
+    }
     ch->spl = steps;
     drop_graphs(ch);
     return FKGPU_OK;

@@ -116,7 +116,7 @@ def test_graph_replay_matches_eager(fk, orc):
     ctx.close()
 
 
-@pytest.mark.parametrize("D,L,B", [(2, 8, 3), (2, 16, 64), (1, 300, 2), (1, 1030, 2), (2, 6, 1)])
+@pytest.mark.parametrize("D,L,B", [(2, 8, 3), (2, 8, 64), (2, 16, 64), (2, 16, 300), (1, 300, 2), (1, 1030, 2), (2, 6, 1)])
 def test_steps_per_launch_agree(fk, D, L, B):
     """Several move steps per launch (one workgroup per (step, chain) item, each chain's
     steps in order) draw, decide and end exactly as one launch per step: traces and final

@@ -2,6 +2,7 @@
 
   fk_mc_amd/lib/libfkgpu.so    HIP kernels + C ABI (hipcc, gfx950)
   fk_mc_amd/lib/libfkmc_host.so  C++ host layer mirroring the reference classes
+  fk_mc_amd/bin/fk_mc_gpu      the reference's Monte Carlo program on a batch of chains
 
 Every call recompiles (the build check must exercise the compiler).
 """
@@ -53,9 +54,23 @@ def build_host():
     return out
 
 
+def build_driver():
+    src = "host/fk_mc_gpu.cpp"
+    if not os.path.exists(os.path.join(HERE, src)):
+        return None
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    out = os.path.join(HERE, "bin", "fk_mc_gpu")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", tmp, src,
+          "-L", LIB, "-lfkgpu", "-Wl,-rpath,$ORIGIN/../lib"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all():
     build_gpu()
     build_host()
+    build_driver()
 
 
 if __name__ == "__main__":

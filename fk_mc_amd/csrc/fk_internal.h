@@ -115,6 +115,9 @@ int fk_solver_nmax(int kind);
 size_t fk_solver_ws_doubles(int kind, int N);
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
+// randomize_f of every chain from its own generator (nf_start: B counts, 0 = drawn)
+hipError_t fk_launch_randomize_f(int B, int N, uint32_t* mt, uint8_t* f, int32_t* nf, const int32_t* nf_start,
+                                 int32_t* f32, hipStream_t st);
 // Work queue of a multi-step sweep launch: take and append counters (2), a ring of
 // max_steps * B slots, an error flag set when a wait exceeds wait_ticks of the wall
 // clock.

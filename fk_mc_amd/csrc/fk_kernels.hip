@@ -406,6 +406,40 @@ __global__ void __launch_bounds__(KC_NT) k_kpm_accept(sweep_args a, const move_p
     }
 }
 
+// configuration_t::randomize_f(rnd, nf) (src/configuration.cpp:47-56) from each chain's
+// own generator, as fk_mc::initialize draws the starting configuration
+// (include/fk_mc/fk_mc.hxx:46): nf = nf_start, or a uniform draw when nf_start is 0,
+// then rejection-sampled distinct sites.  One workgroup per chain; f32 receives the
+// configuration as int32 for the eigensolve that follows.
+__global__ void __launch_bounds__(KC_NT) k_randomize_f(int B, int N, uint32_t* __restrict__ mtg_all,
+                                                       uint8_t* __restrict__ f, int32_t* __restrict__ nf,
+                                                       const int32_t* __restrict__ nf_start,
+                                                       int32_t* __restrict__ f32) {
+    __shared__ uint8_t f_lds[KC_NMAX];
+    __shared__ uint32_t mt[fkrng::MT_WORDS];
+    const int ch = blockIdx.x, tid = threadIdx.x;
+    uint32_t* mtg = mtg_all + (size_t)ch * fkrng::MT_WORDS;
+    for (int i = tid; i < fkrng::MT_WORDS; i += KC_NT) mt[i] = mtg[i];
+    for (int i = tid; i < N; i += KC_NT) f_lds[i] = 0;
+    __syncthreads();
+    if (tid == 0) {
+        int n = nf_start[ch];
+        if (n == 0) n = fkrng::uniform_int(mt, 0, N - 1);
+        for (int k = 0; k < n; ++k) {
+            int ind = fkrng::uniform_int(mt, 0, N - 1);
+            while (f_lds[ind] == 1) ind = fkrng::uniform_int(mt, 0, N - 1);
+            f_lds[ind] = 1;
+        }
+        nf[ch] = n;
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += KC_NT) {
+        f[(size_t)ch * N + i] = f_lds[i];
+        f32[(size_t)ch * N + i] = f_lds[i];
+    }
+    for (int i = tid; i < fkrng::MT_WORDS; i += KC_NT) mtg[i] = mt[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
 // ---------------------------------------------------------------------------
@@ -504,6 +538,12 @@ hipError_t fk_launch_kpm_propose(const sweep_args& a, move_prop* props, int32_t*
 hipError_t fk_launch_kpm_accept(const sweep_args& a, const move_prop* props, const int32_t* fprop,
                                 const double* lz_new, hipStream_t st, const double* spec_new) {
     hipLaunchKernelGGL(k_kpm_accept, dim3(a.B), dim3(KC_NT), 0, st, a, props, fprop, lz_new, spec_new);
+    return hipGetLastError();
+}
+hipError_t fk_launch_randomize_f(int B, int N, uint32_t* mt, uint8_t* f, int32_t* nf, const int32_t* nf_start,
+                                 int32_t* f32, hipStream_t st) {
+    if (N < 1 || N > KC_NMAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_randomize_f, dim3(B), dim3(KC_NT), 0, st, B, N, mt, f, nf, nf_start, f32);
     return hipGetLastError();
 }
 hipError_t fk_launch_sweep_q(int kind, const sweep_args& a, int nsub, const sweep_queue& q, hipStream_t st) {

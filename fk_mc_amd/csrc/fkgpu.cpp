@@ -1373,6 +1373,41 @@ static int queue_check(fkgpu_chains* ch) {
     return FKGPU_OK;
 }
 
+int fkgpu_chains_randomize_f(fkgpu_chains* ch, const int32_t* nf_start) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    const int B = ch->B, N = ch->N;
+    if (!nf_start) return fail(ctx, FKGPU_EINVAL, "null nf_start");
+    for (int c = 0; c < B; ++c)
+        if (nf_start[c] < 0 || nf_start[c] > N)
+            return fail(ctx, FKGPU_EINVAL, "nf_start must lie in [0, msize]");
+    if (ch->fu || ch->mixed || ch->kpm || !ch->dense_ok)
+        return fail(ctx, FKGPU_EUNSUPPORTED, "randomize_f: call it on a new chain set (FP64 dense mode)");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    int32_t *dnf, *df;
+    int r;
+    if ((r = ensure(ctx, ctx->b_f, (size_t)B * N, &df)) || (r = ensure(ctx, ctx->b_m4, (size_t)B, &dnf))) return r;
+    drop_graphs(ch);
+    FK_HIP(ctx, hipMemcpyAsync(dnf, nf_start, sizeof(int32_t) * B, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, fk_launch_randomize_f(B, N, ch->mt, ch->f, ch->nf, dnf, df, st));
+    eig_args a{};
+    a.B = B;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = df;
+    a.beta = ch->beta;
+    a.U = ch->U;
+    a.mu_c = ch->mu_c;
+    a.evals = ch->spectrum;
+    a.logz = ch->logz;
+    a.ws = ch->ws;
+    a.ws_stride = ch->ws_stride;
+    FK_HIP(ctx, fk_launch_eig(ch->kind, a, ch->grid, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
 int fkgpu_chains_set_steps_per_launch(fkgpu_chains* ch, int steps) {
     if (!ch) return FKGPU_EINVAL;
     if (steps < 1 || (long long)steps * ch->B > 0x7fffffffLL)

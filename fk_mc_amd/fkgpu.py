@@ -102,6 +102,7 @@ def lib():
         L.fkgpu_autotune.argtypes = [P, C.c_int, _d]
         L.fkgpu_kpm_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, C.c_int, C.c_uint64, C.c_int, _d, _d, _d]
         L.fkgpu_chains_set_steps_per_launch.argtypes = [P, C.c_int]
+        L.fkgpu_chains_randomize_f.argtypes = [P, _i32]
         L.fkgpu_chains_measure_enable.argtypes = [P, C.c_int, C.c_int]
         L.fkgpu_chains_measure.argtypes = [P]
         L.fkgpu_sweep_measure.argtypes = [P, C.c_int, C.c_int]
@@ -379,6 +380,10 @@ class Chains:
                    _p(tr["logw"], _d), _p(tr["logz"], _d), _p(tr["acc"], _i8))
         self.ctx._check(lib().fkgpu_sweep(self._h, nsteps, C.byref(t)))
         return {k: v.reshape(nsteps, self.B) for k, v in tr.items()}
+
+    def randomize_f(self, nf_start):
+        nf = np.ascontiguousarray(np.broadcast_to(np.asarray(nf_start, dtype=np.int32), (self.B,)))
+        self.ctx._check(lib().fkgpu_chains_randomize_f(self._h, _p(nf, _i32)))
 
     def set_steps_per_launch(self, steps):
         self.ctx._check(lib().fkgpu_chains_set_steps_per_launch(self._h, steps))

@@ -1,0 +1,406 @@
+// fk_mc_gpu -- the reference's Monte Carlo program (prog/fk_mc_exec.cpp with
+// include/fk_mc/fk_mc.hxx) run as a batch of chains on one GPU.
+//
+// Parameters keep the reference's names and defaults (fk_mc.hxx:176-207,
+// src/mc_metropolis.cpp:11-19, prog/fk_mc_exec.cpp:182-219) and are read as the
+// reference's alps::params reads them: an optional ini file (`name = value` lines) as
+// the first positional argument, then `--name value` or `--name=value`.  The
+// reference runs one chain per MPI rank seeded SEED + rank; here `--chains R` runs R
+// such chains at once (chain r is rank r), and `--lattice` picks the lattice the
+// reference compiles into separate executables (fk_mc_cubic1d, _cubic2d, _cubic3d,
+// _triangular).
+//
+// The run follows fk_mc::initialize and mcbase::run: randomize_f(Nf_start) from each
+// chain's generator, the moves flip / add_remove / reshuffle registered in the
+// reference's order with their weights, ntherm_sweeps unmeasured sweeps, then nsweeps
+// sweeps of sweep_len moves each followed by the measures (energy, spectrum, spectrum
+// and f-occupation histories when measure_history, IPR when measure_ipr), all kept on
+// the device until the end.
+//
+// Output (`output`, a directory; a trailing .h5 is dropped): the observables of
+// observables_t (include/fk_mc/fk_mc.hpp:13-28) under mc_data/ as .npy files with the
+// chain as the leading axis -- energies, d2energies, c_energies [chains][nsweeps],
+// spectrum [chains][N] (running mean), spectrum_history, focc_history, ipr_history
+// [chains][nsweeps][N] -- plus dos.npy (the DOS of the recorded spectra on the
+// reference's real grid, prog/data_save.hxx:67-69,264-345), parameters.txt and
+// stats.txt (means with errors from the spread over chains, and the specific heat
+// beta^2 (<E^2> - <d2E> - <E>^2) / N of prog/data_save.hxx:185-190).
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "fkgpu.h"
+
+namespace {
+
+struct param {
+    std::string value, help;
+};
+
+struct params {
+    std::vector<std::string> order;
+    std::map<std::string, param> p;
+    void define(const std::string& k, const std::string& v, const std::string& h) {
+        order.push_back(k);
+        p[k] = {v, h};
+    }
+    bool has(const std::string& k) const { return p.count(k) != 0; }
+    const std::string& str(const std::string& k) const { return p.at(k).value; }
+    double d(const std::string& k) const { return std::stod(str(k)); }
+    long i(const std::string& k) const { return std::stol(str(k)); }
+    bool b(const std::string& k) const {
+        const std::string& v = str(k);
+        return v == "1" || v == "true" || v == "True" || v == "yes" || v == "on";
+    }
+    std::vector<double> vec(const std::string& k) const {
+        std::string s = str(k);
+        for (char& c : s)
+            if (c == ',' || c == '[' || c == ']') c = ' ';
+        std::istringstream in(s);
+        std::vector<double> out;
+        double x;
+        while (in >> x) out.push_back(x);
+        return out;
+    }
+    void set(const std::string& k, const std::string& v) {
+        if (!has(k)) throw std::runtime_error("unknown parameter '" + k + "' (--help lists them)");
+        p[k].value = v;
+    }
+};
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t\r\n\"'"), b = s.find_last_not_of(" \t\r\n\"'");
+    return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+}
+
+params define_parameters() {
+    params p;
+    // fk_mc<L>::define_parameters (include/fk_mc/fk_mc.hxx:176-207)
+    p.define("beta", "10", "Inverse temperature");
+    p.define("U", "1", "FK U");
+    p.define("mu_c", "0.5", "Chemical potential of c electrons");
+    p.define("mu_f", "0.5", "Chemical potential of f electrons");
+    p.define("mc_flip", "0", "Make flip moves");
+    p.define("mc_add_remove", "1", "Make add/remove moves");
+    p.define("mc_reshuffle", "0", "Make reshuffle moves");
+    p.define("cheb_moves", "false", "Allow moves using Chebyshev sampling (not in the batched program)");
+    p.define("cheb_prefactor", "2.2", "Prefactor for number of Chebyshev polynomials = #ln(Volume)");
+    p.define("measure_history", "true", "Measure the history");
+    p.define("Nf_start", "5", "Starting number of f-electrons");
+    p.define("seed", "", "Seed for random number generator (default: std::random_device)");
+    p.define("measure_ipr", "false", "Measure inverse participation ratio");
+    p.define("measure_eigenfunctions", "false", "Measure eigenfunctions (not in the batched program)");
+    p.define("cond_offset", "0.05", "dos offset from the real axis");
+    p.define("measure_stiffness", "false", "Measure stiffness/conductivity (not in the batched program)");
+    // mc_metropolis::define_parameters (src/mc_metropolis.cpp:11-19)
+    p.define("nsweeps", "1024", "Total number of sweeps (1 sweep = #sweep_len moves + 1 measurement)");
+    p.define("sweep_len", "16", "Number of moves between subsequent measurements");
+    p.define("ntherm_sweeps", "1", "How many sweeps to do before start measuring");
+    p.define("show_output", "true", "Show run progress of mc");
+    // cmdline_params (prog/fk_mc_exec.cpp:182-219)
+    p.define("L", "4", "System linear size");
+    p.define("t", "1", "Hopping");
+    p.define("tp", "1", "Triangular lattice : NNN Hopping");
+    p.define("W", "", "1d : f-f interaction");
+    p.define("output", "output.h5", "directory to write data to (a trailing .h5 is dropped)");
+    p.define("plaintext", "false", "also write the series as text (.dat)");
+    p.define("dos_width", "6", "Width of DOS");
+    p.define("dos_npts", "240", "Number of points for DOS sampling");
+    p.define("dos_offset", "0.05", "DOS offset from real axis");
+    // batched program
+    p.define("lattice", "cubic2d", "cubic1d | cubic2d | cubic3d | triangular");
+    p.define("chains", "1", "Number of chains (chain r = MPI rank r of the reference, seeded seed + r)");
+    p.define("mc_hop", "0", "Make nearest-neighbour exchange moves (no reference counterpart)");
+    p.define("update", "dense", "dense | rank1 (rank-one spectral updates, N <= 1024)");
+    p.define("refresh", "256", "rank1: full re-decomposition every this many moves");
+    p.define("precision", "fp64", "fp64 | mixed");
+    p.define("device", "0", "HIP device");
+    return p;
+}
+
+void read_ini(params& p, const std::string& file) {
+    std::ifstream in(file);
+    if (!in) throw std::runtime_error("cannot read " + file);
+    std::string line;
+    while (std::getline(in, line)) {
+        line = trim(line.substr(0, line.find('#')));
+        if (line.empty() || line[0] == '[') continue;
+        size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        p.set(trim(line.substr(0, eq)), trim(line.substr(eq + 1)));
+    }
+}
+
+// .npy (format 1.0): little-endian float64, C order
+void write_npy(const std::string& path, const double* data, const std::vector<size_t>& shape) {
+    std::string dims;
+    size_t n = 1;
+    for (size_t s : shape) {
+        dims += std::to_string(s) + ", ";
+        n *= s;
+    }
+    if (shape.size() > 1) dims = dims.substr(0, dims.size() - 2);
+    std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': (" + dims + "), }";
+    while ((10 + hdr.size() + 1) % 64) hdr += ' ';
+    hdr += '\n';
+    std::ofstream out(path, std::ios::binary);
+    if (!out) throw std::runtime_error("cannot write " + path);
+    const char magic[] = "\x93NUMPY\x01\x00";
+    out.write(magic, 8);
+    const uint16_t hl = (uint16_t)hdr.size();
+    out.write(reinterpret_cast<const char*>(&hl), 2);
+    out.write(hdr.data(), hdr.size());
+    out.write(reinterpret_cast<const char*>(data), sizeof(double) * n);
+}
+
+void write_txt(const std::string& path, const double* data, size_t rows, size_t cols) {
+    std::ofstream out(path);
+    out.precision(17);
+    for (size_t r = 0; r < rows; ++r) {
+        for (size_t c = 0; c < cols; ++c) out << (c ? " " : "") << data[r * cols + c];
+        out << "\n";
+    }
+}
+
+void mkdirs(const std::string& path) {
+    std::string cur;
+    std::istringstream in(path);
+    std::string part;
+    if (!path.empty() && path[0] == '/') cur = "/";
+    while (std::getline(in, part, '/')) {
+        if (part.empty()) continue;
+        cur += part + "/";
+        if (mkdir(cur.c_str(), 0755) != 0 && errno != EEXIST) throw std::runtime_error("cannot create " + cur);
+    }
+}
+
+struct mean_err {
+    double mean, err;
+};
+// mean over chains of per-chain values, error = spread over chains / sqrt(chains)
+mean_err over_chains(const std::vector<double>& v) {
+    const double n = (double)v.size();
+    double m = 0.0;
+    for (double x : v) m += x;
+    m /= n;
+    double var = 0.0;
+    for (double x : v) var += (x - m) * (x - m);
+    return {m, v.size() > 1 ? std::sqrt(var / (n - 1.0) / n) : NAN};
+}
+
+#define FK_CHECK(call, ctx)                                                                     \
+    do {                                                                                        \
+        int _r = (call);                                                                        \
+        if (_r != FKGPU_OK) throw std::runtime_error(std::string(#call) + ": " + fkgpu_last_error(ctx)); \
+    } while (0)
+
+int run(params& p) {
+    const std::string lat = p.str("lattice");
+    int kind = FKGPU_LATTICE_HYPERCUBIC, D = 2;
+    if (lat == "cubic1d") D = 1;
+    else if (lat == "cubic2d") D = 2;
+    else if (lat == "cubic3d") D = 3;
+    else if (lat == "triangular") kind = FKGPU_LATTICE_TRIANGULAR;
+    else throw std::runtime_error("unknown lattice " + lat);
+    if (p.b("cheb_moves") || p.b("measure_eigenfunctions") || p.b("measure_stiffness"))
+        throw std::runtime_error("cheb_moves, measure_eigenfunctions and measure_stiffness are not part of the "
+                                 "batched program (use the host layer or fkgpu_measure_stiffness)");
+    const int L = (int)p.i("L"), R = (int)p.i("chains");
+    const long nsweeps = p.i("nsweeps"), sweep_len = p.i("sweep_len"), ntherm = p.i("ntherm_sweeps");
+    if (R < 1 || nsweeps < 0 || sweep_len < 1 || ntherm < 0) throw std::runtime_error("invalid chains / sweeps");
+    if (p.str("seed").empty()) p.set("seed", std::to_string((int)std::random_device()()));
+    const long seed = p.i("seed");
+    const double beta = p.d("beta"), U = p.d("U"), mu_c = p.d("mu_c"), mu_f = p.d("mu_f");
+    const bool history = p.b("measure_history"), ipr = p.b("measure_ipr");
+
+    fkgpu_ctx* ctx = nullptr;
+    if (fkgpu_create((int)p.i("device"), &ctx) != FKGPU_OK) throw std::runtime_error("no HIP device");
+    FK_CHECK(fkgpu_set_lattice(ctx, kind, D, L, p.d("t"), kind == FKGPU_LATTICE_TRIANGULAR ? p.d("tp") : 0.0), ctx);
+    const int N = fkgpu_msize(ctx);
+    std::vector<int32_t> f0((size_t)R * N, 0), nfs(R, (int32_t)p.i("Nf_start"));
+    std::vector<double> vb(R, beta), vU(R, U), vmc(R, mu_c), vmf(R, mu_f);
+    std::vector<uint32_t> seeds(R);
+    for (int r = 0; r < R; ++r) seeds[r] = (uint32_t)(seed + r);  // SEED + rank (src/mc_metropolis.cpp:25)
+    fkgpu_chains* ch = nullptr;
+    FK_CHECK(fkgpu_chains_create(ctx, R, f0.data(), vb.data(), vU.data(), vmc.data(), vmf.data(), seeds.data(), &ch),
+             ctx);
+    // fk_mc::initialize: randomize_f from the Metropolis generator, then the moves in the
+    // reference's order (flip, add_remove, reshuffle) with their weights
+    FK_CHECK(fkgpu_chains_randomize_f(ch, nfs.data()), ctx);
+    std::vector<int32_t> kinds;
+    std::vector<double> w;
+    const std::pair<const char*, int> mv[] = {{"mc_flip", FKGPU_MOVE_FLIP},
+                                               {"mc_add_remove", FKGPU_MOVE_ADDREMOVE},
+                                               {"mc_reshuffle", FKGPU_MOVE_RANDOMIZE},
+                                               {"mc_hop", FKGPU_MOVE_HOP}};
+    for (auto& m : mv)
+        if (p.d(m.first) > 2.220446049250313e-16) {
+            kinds.push_back(m.second);
+            w.push_back(p.d(m.first));
+        }
+    if (kinds.empty()) throw std::runtime_error("No registered moves");
+    FK_CHECK(fkgpu_chains_set_moves(ch, (int)kinds.size(), kinds.data(), w.data()), ctx);
+    if (D == 1 && !p.str("W").empty()) {
+        std::vector<double> W = p.vec("W");
+        FK_CHECK(fkgpu_chains_set_ff(ch, (int)W.size(), W.data()), ctx);
+    }
+    if (p.str("precision") == "mixed") FK_CHECK(fkgpu_chains_set_precision(ch, FKGPU_PRECISION_MIXED), ctx);
+    if (p.str("update") == "rank1") FK_CHECK(fkgpu_chains_set_update(ch, FKGPU_UPDATE_RANK1, (int)p.i("refresh")), ctx);
+    int flags = (history ? FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY : 0) | (ipr ? FKGPU_MEASURE_IPR : 0);
+    FK_CHECK(fkgpu_chains_measure_enable(ch, (int)nsweeps, flags), ctx);
+
+    const bool show = p.b("show_output");
+    if (show)
+        std::printf("fk_mc_gpu: %s L=%d N=%d, %d chains, seeds %ld..%ld, %ld + %ld sweeps of %ld moves\n", lat.c_str(),
+                    L, N, R, seed, seed + R - 1, ntherm, nsweeps, sweep_len);
+    auto t0 = std::chrono::steady_clock::now();
+    if (ntherm > 0) FK_CHECK(fkgpu_sweep_graph(ch, (int)(ntherm * sweep_len)), ctx);
+    // measured sweeps in blocks, with a progress line per block
+    const long block = std::max(1L, nsweeps / 10);
+    for (long done = 0; done < nsweeps;) {
+        const long n = std::min(block, nsweeps - done);
+        FK_CHECK(fkgpu_sweep_measure(ch, (int)n, (int)sweep_len), ctx);
+        done += n;
+        if (show) std::printf("  %ld%%\n", 100 * done / std::max(1L, nsweeps)), std::fflush(stdout);
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+    // results
+    int64_t count = 0;
+    const size_t S = (size_t)nsweeps, RN = (size_t)R * N;
+    std::vector<double> e(R * S), d2(R * S), ce(R * S), em(R), d2m(R), spec(RN), focc(RN);
+    FK_CHECK(fkgpu_chains_energies(ch, &count, e.data(), d2.data(), ce.data()), ctx);
+    FK_CHECK(fkgpu_chains_means(ch, &count, em.data(), d2m.data(), spec.data(), focc.data()), ctx);
+    std::vector<double> sh(history ? R * S * N : 0), ih(ipr ? R * S * N : 0);
+    std::vector<int32_t> fh(history ? R * S * N : 0);
+    if (history || ipr)
+        FK_CHECK(fkgpu_chains_histories(ch, &count, history ? sh.data() : nullptr, history ? fh.data() : nullptr,
+                                        ipr ? ih.data() : nullptr),
+                 ctx);
+    std::vector<int64_t> nacc(R);
+    FK_CHECK(fkgpu_chains_state(ch, nullptr, nullptr, nullptr, nullptr, nacc.data(), nullptr), ctx);
+    // DOS of the recorded spectra (or of the mean spectra without histories)
+    const int npts = (int)p.i("dos_npts");
+    const double wd = p.d("dos_width");
+    std::vector<double> grid(npts), dos(npts, 0.0);
+    for (int i = 0; i < npts; ++i) grid[i] = -wd + 2.0 * wd * i / (1.0 * npts);
+    {
+        const std::vector<double>& src = history && S > 0 ? sh : spec;
+        const int ns = (int)(src.size() / N);
+        std::vector<double> d((size_t)ns * npts);
+        if (ns > 0 && npts > 0) {
+            FK_CHECK(fkgpu_dos(ctx, ns, src.data(), npts, grid.data(), p.d("dos_offset"), d.data()), ctx);
+            for (int s = 0; s < ns; ++s)
+                for (int i = 0; i < npts; ++i) dos[i] += d[(size_t)s * npts + i] / ns;
+        }
+    }
+    fkgpu_chains_destroy(ch);
+    fkgpu_destroy(ctx);
+
+    std::string out = p.str("output");
+    if (out.size() > 3 && out.substr(out.size() - 3) == ".h5") out = out.substr(0, out.size() - 3);
+    mkdirs(out + "/mc_data");
+    const std::string md = out + "/mc_data/";
+    write_npy(md + "energies.npy", e.data(), {(size_t)R, S});
+    write_npy(md + "d2energies.npy", d2.data(), {(size_t)R, S});
+    write_npy(md + "c_energies.npy", ce.data(), {(size_t)R, S});
+    write_npy(md + "spectrum.npy", spec.data(), {(size_t)R, (size_t)N});
+    if (history) {
+        write_npy(md + "spectrum_history.npy", sh.data(), {(size_t)R, S, (size_t)N});
+        std::vector<double> fd(fh.begin(), fh.end());
+        write_npy(md + "focc_history.npy", fd.data(), {(size_t)R, S, (size_t)N});
+    }
+    if (ipr) write_npy(md + "ipr_history.npy", ih.data(), {(size_t)R, S, (size_t)N});
+    std::vector<double> dg(2 * npts);
+    for (int i = 0; i < npts; ++i) dg[2 * i] = grid[i], dg[2 * i + 1] = dos[i];
+    write_npy(out + "/dos.npy", dg.data(), {(size_t)npts, 2});
+    if (p.b("plaintext")) {
+        write_txt(out + "/energies.dat", e.data(), R, S);
+        write_txt(out + "/dos.dat", dg.data(), npts, 2);
+    }
+    {
+        std::ofstream pf(out + "/parameters.txt");
+        for (auto& k : p.order) pf << k << " = " << p.str(k) << "\n";
+    }
+    // statistics: per-chain means, errors from the spread over the chains
+    std::vector<double> cv(R), eN(R), accr(R);
+    for (int r = 0; r < R; ++r) {
+        double m = 0, m2 = 0, md2 = 0;
+        for (size_t s = 0; s < S; ++s) {
+            const double x = e[r * S + s];
+            m += x;
+            m2 += x * x;
+            md2 += d2[r * S + s];
+        }
+        m /= std::max<size_t>(S, 1);
+        m2 /= std::max<size_t>(S, 1);
+        md2 /= std::max<size_t>(S, 1);
+        cv[r] = beta * beta * (m2 - md2 - m * m) / N;  // prog/data_save.hxx:185-190
+        eN[r] = em[r] / N;
+        accr[r] = (double)nacc[r] / (double)((ntherm + nsweeps) * sweep_len);
+    }
+    std::vector<double> nfm(R, 0.0);
+    for (int r = 0; r < R; ++r)
+        for (int i = 0; i < N; ++i) nfm[r] += focc[(size_t)r * N + i] / N;
+    std::ofstream st(out + "/stats.txt");
+    st.precision(12);
+    auto line = [&](const char* name, const std::vector<double>& v) {
+        const mean_err me = over_chains(v);
+        st << name << " = " << me.mean << " +- " << me.err << "\n";
+        if (show) std::printf("  %-12s %.10g +- %.3g\n", name, me.mean, me.err);
+    };
+    st << "# means over " << R << " chains x " << count << " measurements; errors from the spread over chains\n";
+    line("energy/N", eN);
+    line("cv", cv);
+    line("nf/N", nfm);
+    line("acceptance", accr);
+    st << "seconds = " << secs << "\nmoves_per_second = " << (double)R * (ntherm + nsweeps) * sweep_len / secs
+       << "\n";
+    if (show) std::printf("  %.3f s, %.4g moves/s, output in %s/\n", secs,
+                          (double)R * (ntherm + nsweeps) * sweep_len / secs, out.c_str());
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    params p = define_parameters();
+    try {
+        for (int a = 1; a < argc; ++a) {
+            std::string s = argv[a];
+            if (s == "--help" || s == "-h") {
+                std::printf("usage: fk_mc_gpu [file.ini] [--name value | --name=value ...]\n");
+                for (auto& k : p.order)
+                    std::printf("  --%-24s %s (default: %s)\n", k.c_str(), p.p[k].help.c_str(), p.p[k].value.c_str());
+                return 0;
+            }
+            if (s.rfind("--", 0) == 0) {
+                s = s.substr(2);
+                size_t eq = s.find('=');
+                if (eq != std::string::npos) p.set(s.substr(0, eq), s.substr(eq + 1));
+                else if (a + 1 < argc) p.set(s, argv[++a]);
+                else p.set(s, "true");
+            } else if (a == 1) {
+                read_ini(p, s);
+            } else {
+                throw std::runtime_error("unexpected argument " + s);
+            }
+        }
+        return run(p);
+    } catch (const std::exception& ex) {
+        std::fprintf(stderr, "fk_mc_gpu: %s\n", ex.what());
+        return 1;
+    }
+}

@@ -177,6 +177,12 @@ typedef struct {
 int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, const double* beta, const double* U,
                         const double* mu_c, const double* mu_f, const uint32_t* seeds, fkgpu_chains** out);
 void fkgpu_chains_destroy(fkgpu_chains* ch);
+/* configuration_t::randomize_f(rng, nf) (src/configuration.cpp:47-56) of every chain,
+ * drawn from the chain's own generator, as fk_mc::initialize draws the starting
+ * configuration from the Metropolis generator (include/fk_mc/fk_mc.hxx:46): nf_start[c]
+ * f-electrons on distinct uniform sites (0: a uniform count first).  Recomputes logZ and
+ * the spectrum.  On a new chain set, before any other option is set. */
+int fkgpu_chains_randomize_f(fkgpu_chains* ch, const int32_t* nf_start);
 /* mc_metropolis::add_move in order (include/fk_mc/mc_metropolis.hpp:176-185):
  * move kinds and their probabilities (up to 4). */
 int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, const double* weights);

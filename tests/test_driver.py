@@ -1,0 +1,119 @@
+"""The batched Monte Carlo program fk_mc_amd/bin/fk_mc_gpu -- the reference's
+fk_mc_<lattice> program (prog/fk_mc_exec.cpp, include/fk_mc/fk_mc.hxx) with its parameter
+names and defaults -- against the host layer's single-chain run of the same seeds
+(fkmc_run_mc: configuration_t::randomize_f, move_addremove, mc_metropolis and the
+reference's measures, one chain per seed as one MPI rank each) and against the oracle's
+calc_ed / measure_energy / measure_ipr of the recorded occupations."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "fk_mc_amd", "bin", "fk_mc_gpu")
+
+
+def _run(tmp_path, *args):
+    out = tmp_path / "run"
+    cmd = [EXE] + [str(a) for a in args] + ["--output", str(out) + ".h5", "--show_output", "false"]
+    print("case driver", " ".join(cmd[1:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return out
+
+
+def _params(out):
+    p = {}
+    for line in open(out / "parameters.txt"):
+        k, _, v = line.partition(" = ")
+        p[k.strip()] = v.strip()
+    return p
+
+
+@pytest.mark.gpu
+def test_driver_matches_host_layer(tmp_path, orc):
+    from fk_mc_amd import host_api as host
+    L, R, seed, beta, U, mu = 4, 3, 7, 2.0, 2.0, 1.0
+    ns, sl, nt = 12, 4, 2
+    N = L * L
+    out = _run(tmp_path, "--lattice", "cubic2d", "--L", L, "--beta", beta, "--U", U, "--mu_c", mu, "--mu_f", mu,
+               "--Nf_start", N // 2, "--nsweeps", ns, "--sweep_len", sl, "--ntherm_sweeps", nt, "--chains", R,
+               "--seed", seed, "--measure_ipr=true")
+    md = out / "mc_data"
+    e, d2, ce = (np.load(md / f"{k}.npy") for k in ("energies", "d2energies", "c_energies"))
+    fh, sh, ih = (np.load(md / f"{k}.npy") for k in ("focc_history", "spectrum_history", "ipr_history"))
+    spec = np.load(md / "spectrum.npy")
+    assert e.shape == (R, ns) and fh.shape == (R, ns, N) and spec.shape == (R, N)
+    m = host.model(0, 2, L, beta, U, mu, mu)
+    checked = 0
+    for r in range(R):
+        # rank r of the reference: seed + r, randomize_f(msize / 2) from the Metropolis generator
+        ref = host.run_mc(m, seed + r, [(0, 1.0)], nsweeps=ns, sweep_len=sl, ntherm=nt)
+        np.testing.assert_array_equal(fh[r], ref["focc"])
+        np.testing.assert_allclose(e[r], ref["energies"], rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(d2[r], ref["d2energies"], rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(ce[r], ref["c_energies"], rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(spec[r], ref["avg_spectrum"], rtol=0, atol=1e-11)
+        for i in range(ns):
+            f = fh[r, i].astype(np.int32)
+            ed = orc.calc_ed(orc.KIND_HYPERCUBIC, 2, L, f, beta, U, mu, tp=0.0, vectors=True)
+            np.testing.assert_allclose(sh[r, i], ed["evals"], rtol=0, atol=1e-10)
+            en = orc.measure_energy(ed["evals"], ed["cexp"], mu, int(f.sum()))
+            np.testing.assert_allclose(e[r, i], en["energy"], rtol=1e-10, atol=1e-10)
+            ipr = orc.measure_ipr(ed["evecs"])
+            gap = np.diff(ed["evals"])
+            iso = np.ones(N, bool)
+            iso[:-1] &= gap > 1e-6
+            iso[1:] &= gap > 1e-6
+            np.testing.assert_allclose(ih[r, i][iso], ipr[iso], rtol=0, atol=1e-9)
+            checked += int(iso.sum())
+    assert checked > 0
+    # DOS of the recorded spectra on the reference's grid (prog/data_save.hxx:67-69)
+    dg = np.load(out / "dos.npy")
+    grid = -6.0 + 12.0 * np.arange(240) / 240.0
+    np.testing.assert_allclose(dg[:, 0], grid, rtol=0, atol=1e-14)
+    eta = 0.05
+    lor = (eta / np.pi) / ((grid[None, :] - sh.reshape(-1)[:, None]) ** 2 + eta ** 2)
+    np.testing.assert_allclose(dg[:, 1], lor.sum(0) / (R * ns * N), rtol=1e-10, atol=1e-13)
+    p = _params(out)
+    assert p["seed"] == str(seed) and p["chains"] == str(R) and p["mc_add_remove"] == "1"
+    stats = open(out / "stats.txt").read()
+    assert "energy/N" in stats and "cv" in stats
+
+
+@pytest.mark.gpu
+def test_driver_moves_ini_and_1d(tmp_path, orc):
+    """An ini file as the first argument, flip + add_remove moves registered in the
+    reference's order, the 1-D f-f term W, no history (energies only)."""
+    from fk_mc_amd import host_api as host
+    ini = tmp_path / "p.ini"
+    ini.write_text("# parameters\nL = 12\nbeta = 3\nU = 2\nmu_c = 1\nmu_f = 1\nmc_flip = 0.3\n"
+                   "mc_add_remove = 0.7\nW = 0.0, 0.5\nNf_start = 6\nmeasure_history = false\n")
+    out = _run(tmp_path, ini, "--lattice", "cubic1d", "--nsweeps", 8, "--sweep_len", 5, "--ntherm_sweeps", 1,
+               "--chains", 2, "--seed", 11)
+    md = out / "mc_data"
+    e = np.load(md / "energies.npy")
+    assert not (md / "spectrum_history.npy").exists()
+    m = host.model(0, 1, 12, 3.0, 2.0, 1.0, 1.0, W=[0.0, 0.5])
+    for r in range(2):
+        # moves in the reference's registration order: flip, then add_remove
+        ref = host.run_mc(m, 11 + r, [(1, 0.3), (0, 0.7)], nsweeps=8, sweep_len=5, ntherm=1)
+        np.testing.assert_allclose(e[r], ref["energies"], rtol=1e-10, atol=1e-10)
+
+
+def test_driver_parameters_without_device(tmp_path):
+    """Parameter handling ends before any device call: runs without a GPU."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built")
+    r = subprocess.run([EXE, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    for name, default in (("beta", "10"), ("U", "1"), ("mu_c", "0.5"), ("mc_add_remove", "1"), ("Nf_start", "5"),
+                          ("nsweeps", "1024"), ("sweep_len", "16"), ("ntherm_sweeps", "1"), ("L", "4"),
+                          ("dos_npts", "240")):
+        assert f"--{name} " in r.stdout and f"(default: {default})" in r.stdout.split(f"--{name} ")[1].split("\n")[0]
+    r = subprocess.run([EXE, "--no_such_parameter", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "unknown parameter" in r.stderr
+    r = subprocess.run([EXE, "--cheb_moves", "true", "--output", str(tmp_path / "x")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0

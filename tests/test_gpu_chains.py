@@ -146,3 +146,30 @@ def test_steps_per_launch_agree(fk, D, L, B):
         for k in ("f", "nf", "logZ", "spectrum", "naccept", "nsolve"):
             assert np.array_equal(st[k], res[0][1][k]), k
     ctx.close()
+
+
+def test_randomize_f_from_chain_generator(fk, orc):
+    """fkgpu_chains_randomize_f draws configuration_t::randomize_f from each chain's own
+    generator (src/configuration.cpp:47-56): the oracle's randomize_f of the same seed,
+    including the drawn count for nf_start = 0; then the chain continues exactly like an
+    oracle chain that started from that configuration with the advanced generator."""
+    print("case randomize_f", flush=True)
+    D, L, B = 2, 6, 4
+    N = L * L
+    seeds = np.array([5, 6, 7, 8], np.uint32)
+    nf = np.array([0, 1, N // 2, N], np.int32)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, D, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, np.zeros((B, N), np.int32), 2.0, 3.0, 1.5, 1.5, seeds)
+    ch.randomize_f(nf)
+    st = ch.state()
+    for c in range(B):
+        ref = orc.randomize_f(N, int(seeds[c]), int(nf[c]))
+        np.testing.assert_array_equal(st["f"][c], ref)
+        assert st["nf"][c] == ref.sum()
+        ed = orc.calc_ed(orc.KIND_HYPERCUBIC, D, L, ref, 2.0, 3.0, 1.5, tp=0.0)
+        np.testing.assert_allclose(st["spectrum"][c], ed["evals"], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(st["logZ"][c], ed["logZ"], rtol=1e-12)
+    with pytest.raises(fk.FkgpuError):
+        ch.randomize_f(N + 1)
+    ch.close()
+    ctx.close()

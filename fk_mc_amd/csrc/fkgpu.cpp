@@ -1408,6 +1408,17 @@ int fkgpu_chains_randomize_f(fkgpu_chains* ch, const int32_t* nf_start) {
     return FKGPU_OK;
 }
 
+int fkgpu_chains_set_queue_timeout(fkgpu_chains* ch, double seconds) {
+    if (!ch) return FKGPU_EINVAL;
+    if (!(seconds >= 0.0) || seconds > 3600.0) return fail(ch->ctx, FKGPU_EINVAL, "timeout must lie in [0, 3600] s");
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ch->ctx->device) != hipSuccess || khz <= 0)
+        khz = 100000;
+    ch->q_ticks = (long long)(seconds * 1000.0 * khz);
+    drop_graphs(ch);  // the bound is a kernel argument
+    return FKGPU_OK;
+}
+
 int fkgpu_chains_set_steps_per_launch(fkgpu_chains* ch, int steps) {
     if (!ch) return FKGPU_EINVAL;
     if (steps < 1 || (long long)steps * ch->B > 0x7fffffffLL)

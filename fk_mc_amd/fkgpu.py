@@ -103,6 +103,7 @@ def lib():
         L.fkgpu_kpm_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, C.c_int, C.c_uint64, C.c_int, _d, _d, _d]
         L.fkgpu_chains_set_steps_per_launch.argtypes = [P, C.c_int]
         L.fkgpu_chains_randomize_f.argtypes = [P, _i32]
+        L.fkgpu_chains_set_queue_timeout.argtypes = [P, C.c_double]
         L.fkgpu_chains_measure_enable.argtypes = [P, C.c_int, C.c_int]
         L.fkgpu_chains_measure.argtypes = [P]
         L.fkgpu_sweep_measure.argtypes = [P, C.c_int, C.c_int]
@@ -384,6 +385,9 @@ class Chains:
     def randomize_f(self, nf_start):
         nf = np.ascontiguousarray(np.broadcast_to(np.asarray(nf_start, dtype=np.int32), (self.B,)))
         self.ctx._check(lib().fkgpu_chains_randomize_f(self._h, _p(nf, _i32)))
+
+    def set_queue_timeout(self, seconds):
+        self.ctx._check(lib().fkgpu_chains_set_queue_timeout(self._h, seconds))
 
     def set_steps_per_launch(self, steps):
         self.ctx._check(lib().fkgpu_chains_set_steps_per_launch(self._h, steps))

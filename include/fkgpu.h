@@ -227,6 +227,10 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
  * decides exactly as with one launch per step, while the solves of all chains and steps
  * fill the CUs without a launch boundary between steps. */
 int fkgpu_chains_set_steps_per_launch(fkgpu_chains* ch, int steps);
+/* Bound of one wait in that queue (default 30 s; no move step comes near it).  When a
+ * wait exceeds it the launch ends early and the sweep call returns FKGPU_EHIP; the chain
+ * set's state is then undefined (destroy it). */
+int fkgpu_chains_set_queue_timeout(fkgpu_chains* ch, double seconds);
 /* Run nsteps single moves on every chain.  trace may be null. */
 int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace);
 /* Replay nsteps move steps from a captured HIP graph (no trace). */

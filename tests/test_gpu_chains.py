@@ -173,3 +173,27 @@ def test_randomize_f_from_chain_generator(fk, orc):
         ch.randomize_f(N + 1)
     ch.close()
     ctx.close()
+
+
+def test_queue_wait_bound_reports_error(fk):
+    """The move-step queue's bounded wait, forced from the host: with a zero bound a
+    workgroup that finds its slot not yet filled gives up at once, every later one leaves,
+    the launch ends normally and the call reports the error (and keeps reporting it)."""
+    print("case queue wait bound", flush=True)
+    L = 8
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, np.ones((1, L * L), np.int32), 2.0, 2.0, 1.0, 1.0, np.array([3], np.uint32))
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
+    ch.set_queue_timeout(0.0)
+    # one chain, 16 steps in one launch: steps 1..15 wait for their predecessors
+    with pytest.raises(fk.FkgpuError):
+        ch.sweep(16)
+    with pytest.raises(fk.FkgpuError):
+        ch.sweep_graph(16)
+    ch.close()
+    # a fresh chain set with the default bound runs normally
+    ch = fk.Chains(ctx, np.ones((1, L * L), np.int32), 2.0, 2.0, 1.0, 1.0, np.array([3], np.uint32))
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
+    ch.sweep(16)
+    ch.close()
+    ctx.close()

@@ -315,7 +315,11 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     }
     __syncthreads();
     double lz_new = 0.0;
-    if (prop.solve) lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (prop.solve) {
+        // workspace solvers: one workspace per chain (a chain's steps never overlap)
+        double* ws = a.ws ? a.ws + (size_t)ch * a.ws_stride : nullptr;
+        lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
+    }
     if (tid == 0) {
         double logw;
         const int acc = decide_move(prop, prop.sa >= 0 ? f_lds[prop.sa] : 0, lz_new, a.logz[ch], beta, mu_f,
@@ -552,9 +556,11 @@ hipError_t fk_launch_sweep_q(int kind, const sweep_args& a, int nsub, const swee
         case 0: return launch_sweep_q<reg_solver<16, 4>>(a, nsub, q, st);
         case 1: return launch_sweep_q<reg_solver<16, 8>>(a, nsub, q, st);
         case 2: return launch_sweep_q<reg_solver<32, 8>>(a, nsub, q, st);
+        case 3: return launch_sweep_q<glob_solver>(a, nsub, q, st);
+        case 4: return launch_sweep_q<blk64>(a, nsub, q, st);
         case 5: return launch_sweep_q<wrow_solver>(a, nsub, q, st);
     }
-    return hipErrorInvalidValue;  // workspace solvers: one launch per step
+    return hipErrorInvalidValue;
 }
 
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st) {

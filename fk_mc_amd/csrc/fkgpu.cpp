@@ -1337,11 +1337,9 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
 }
 
 // K move steps of every chain from trace row a.step on: one work-queue launch for the
-// dense FP64 mode with a register-resident solver, one launch_step per step otherwise
-// (the workspace solvers keep their matrices in the L2, which the queue's cross-XCD
-// hand-over writes back and invalidates at every step).
+// dense FP64 mode, one launch_step per step otherwise.
 static hipError_t launch_steps(fkgpu_chains* ch, sweep_args a, int K, hipStream_t st) {
-    if (K > 1 && !ch->fu && !ch->mixed && !ch->kpm && !fk_solver_needs_ws(ch->kind)) {
+    if (K > 1 && !ch->fu && !ch->mixed && !ch->kpm) {
         sweep_queue q{ch->q_ctr, ch->q_ring, ch->spl, ch->q_err, ch->q_ticks};
         return fk_launch_sweep_q(ch->kind, a, K, q, st);
     }

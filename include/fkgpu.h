@@ -220,9 +220,8 @@ int fkgpu_chains_fallbacks(const fkgpu_chains* ch, int64_t* n);
  * re-estimated; each move step then takes three launches (propose, trace, accept).
  * Required for lattices above the dense solver's size (N > 2048). */
 int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jackson);
-/* Move steps of every chain fused into one launch in the FP64 dense mode with the
- * register-resident solvers, N <= 256 (no reference counterpart; default 16, 1 = one
- * launch per step).  One workgroup per (step, chain) item, chains taken from a queue of
+/* Move steps of every chain fused into one launch in the FP64 dense mode (no reference
+ * counterpart; default 16, 1 = one launch per step).  One workgroup per (step, chain) item, chains taken from a queue of
  * ready chains; a chain's step starts after its previous step, so every chain draws and
  * decides exactly as with one launch per step, while the solves of all chains and steps
  * fill the CUs without a launch boundary between steps. */

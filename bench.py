@@ -24,6 +24,10 @@ Printed: one JSON line with the contract fields plus
                 the same chains with rank-one spectral updates / mixed precision,
                 and cubic3d L=8 (N=512) mixed against FP64, each with its dOmega
                 against FP64 dense chains on the same seeds;
+  adjoint_triangular
+                triangular L = 12..24 adjoint gradients (fkgpu_adjoint), configs/s and
+                TFLOP/s at (10/3) N^3 flop per configuration, gradients against central
+                differences;
   kpm_cubic2d_L64
                 cubic2d L=64 (N=4096) stochastic-trace Chebyshev chains, with
                 every 100th proposal's dOmega against a dense FP64 eigensolve;
@@ -76,6 +80,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=8, help="mixed-precision timing steps (0: skip)")
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
+    ap.add_argument("--no-c4", action="store_true", help="skip the triangular adjoint workload")
     ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
     ap.add_argument("--measure-sweeps", type=int, default=4,
                     help="measured sweeps of 16 moves with on-device measures (0: skip)")
@@ -165,6 +170,10 @@ def main():
     if args.mixed_steps > 0 and not args.no_c3:
         c3 = c3_block(fk, max(2, args.mixed_steps // 2))
 
+    c4 = None
+    if not args.no_c4:
+        c4 = c4_block(fk)
+
     c5 = None
     if args.c5_steps > 0:
         c5 = c5_block(fk, args.c5_steps, args.c5_check_steps)
@@ -197,6 +206,7 @@ def main():
         "measured_sweeps": measured,
         "mixed": mixed,
         "mixed_cubic3d_L8": c3,
+        "adjoint_triangular": c4,
         "kpm_cubic2d_L64": c5,
         "cpu_baseline": cpu,
     }
@@ -342,6 +352,82 @@ def c3_block(fk, steps):
            "fp64_solver_variant": ctx.get_solver()}
     out.update(compare_traces(t64, tmx, beta))
     ctx.close()
+    return out
+
+
+def _omega_diff(a, b, beta):
+    """Omega(a) - Omega(b) for spectra a, b, summed eigenvalue by eigenvalue with
+    -log1p(e^{-beta x})/beta differences formed through expm1 (differencing the totals
+    would lose ~1e-13 absolute, 1e-7 after division by 2h)."""
+    a, b = np.asarray(a), np.asarray(b)
+    d = np.empty_like(a)
+    pos = b >= 0
+    x = np.exp(-beta * b[pos])
+    d[pos] = -np.log1p(x * np.expm1(-beta * (a[pos] - b[pos])) / (1 + x)) / beta
+    y = np.exp(beta * b[~pos])
+    d[~pos] = (a[~pos] - b[~pos]) - np.log1p(y * np.expm1(beta * (a[~pos] - b[~pos])) / (1 + y)) / beta
+    return d.sum(axis=-1)
+
+
+def c4_block(fk, reps=3, per_size=32, nfd=8):
+    """Adjoint workload: triangular t = tp = 1, U = 3, beta = 5, mu_c = 1.5, 32 seeded
+    configurations each of L = 12, 16, 20, 24 (N = 144 ... 576), one context (and HIP
+    stream) per size.  Each size's batch is one fkgpu_adjoint call (calc_ed with
+    eigenvectors, rho = V f(Lambda) V^T on the diagonal and the bonds, the gradients) on
+    host buffers; the four sizes are issued from four host threads so their launches
+    overlap on the device (a 32-configuration batch fills an eighth of the CUs).  Timed on
+    the host clock; (10/3) N^3 flop per configuration.  dOmega/dU and dOmega/dt against
+    central differences (h = 1e-6) of FP64 spectra on the first configurations of each size."""
+    from concurrent.futures import ThreadPoolExecutor
+    beta, U, mu, h = 5.0, 3.0, 1.5, 1e-6
+    sizes = (12, 16, 20, 24)
+    out = {"sizes": {}, "configs": len(sizes) * per_size}
+    ctxs, fs, err = {}, {}, 0.0
+    for L in sizes:
+        fs[L] = (np.random.default_rng(400 + L).random((per_size, L * L)) < 0.5).astype(np.int32)
+        ctxs[L] = fk.Context(0).set_lattice(fk.LATTICE_TRIANGULAR, 2, L, 1.0, 1.0)
+        ctxs[L].adjoint(fs[L], beta, U, mu)  # warm-up (workspace allocation)
+
+    def one(L):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            a = ctxs[L].adjoint(fs[L], beta, U, mu)
+        return a, (time.perf_counter() - t0) / reps
+
+    seq = {L: one(L) for L in sizes}  # each size alone
+    with ThreadPoolExecutor(len(sizes)) as ex:
+        t0 = time.perf_counter()
+        res = dict(zip(sizes, ex.map(one, sizes)))
+        wall = (time.perf_counter() - t0) / reps
+    flop_total = 0.0
+    for L in sizes:
+        N = L * L
+        a, _ = res[L]
+        f = fs[L][:nfd]
+        fdU = _omega_diff(ctxs[L].eig_logz(f, beta, U + h, mu)["evals"],
+                          ctxs[L].eig_logz(f, beta, U - h, mu)["evals"], beta) / (2 * h)
+        spec = {}
+        for t in (1.0 + h, 1.0 - h):
+            c2 = fk.Context(0).set_lattice(fk.LATTICE_TRIANGULAR, 2, L, t, 1.0)
+            spec[t] = c2.eig_logz(f, beta, U, mu)["evals"]
+            c2.close()
+        fdt = _omega_diff(spec[1.0 + h], spec[1.0 - h], beta) / (2 * h)
+        e = max(np.abs(a["dOmega_dU"][:nfd] - fdU).max(), np.abs(a["dOmega_dt"][:nfd] - fdt).max())
+        flop = per_size * (10.0 / 3.0) * N ** 3
+        sec = seq[L][1]
+        out["sizes"][f"L{L}"] = {"N": N, "ms_per_batch_alone": round(sec * 1e3, 3),
+                                 "configs_per_s_alone": round(per_size / sec, 1),
+                                 "grad_max_abs_err_vs_fd": float(e)}
+        flop_total += flop
+        err = max(err, e)
+        ctxs[L].close()
+    out["ms_all_sizes_concurrent"] = round(wall * 1e3, 3)
+    out["configs_per_s"] = round(len(sizes) * per_size / wall, 1)
+    out["grad_max_abs_err_vs_fd"] = float(err)
+    out["roofline"] = {"bound": "mfma", "achieved": round(flop_total / wall / 1e12, 4), "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(flop_total / wall / 1e12 / FP64_PEAK_TFLOPS, 5),
+                       "work": "(10/3) N^3 FP64 flop per configuration, host-clock time of the four "
+                               "concurrent batched calls"}
     return out
 
 

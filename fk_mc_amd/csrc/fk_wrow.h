@@ -5,9 +5,11 @@
 // (b = 0..3).  Thread (w, l) keeps A[w + 16a][l + 64b] for b <= a/4 in VGPRs
 // (40 doubles): every 64 x 64 diagonal block is stored in full, the blocks below
 // it once.  For a Householder step (Golub & Van Loan 8.3.1):
-//   * a lane's column values v_j, w_j (j = l + 64b) are its own registers, the
-//     row values v_i, w_i (i = w + 16a, uniform over the wave) are read from the
-//     lane that holds them with v_readlane into scalar registers;
+//   * a lane's column values v_j, w_j (j = l + 64b) are its own registers; the
+//     row values v_i (i = w + 16a, uniform over the wave) are read from the lane
+//     that holds them with v_readlane for the column sums, and for the rank-2
+//     update v_i and p_i come from LDS by broadcast reads (w_i formed as the lanes
+//     form w_j), which issues fewer VALU instructions than 4 readlanes per row;
 //   * the row sums of p = A v are reduced across the 64 lanes by a
 //     reduce-scatter on v_permlane32_swap / v_permlane16_swap / DPP (no LDS); the
 //     column sums of the blocks below the diagonal go through LDS (16 partials
@@ -71,6 +73,7 @@ struct wrow_solver {
         double x[2][NP];
         double rs[NP];
         double p[NP];
+        double v[NP];          // Householder vector of the current step
         double d[NP], e[NP];
         double red[NWV + 8];
         union {
@@ -187,6 +190,7 @@ struct wrow_solver {
                 const double pi = (i > k && i < N) ? tau * s : 0.0;
                 if (q4 == 0) sm.p[i] = pi;
                 const double vi = (i == k + 1) ? 1.0 : ((i > k + 1 && i < N) ? xk[i] * scal : 0.0);
+                if (q4 == 0) sm.v[i] = vi;
                 const double pv = wave_sum(q4 == 0 ? pi * vi : 0.0);
                 if (l == 0) sm.red[tid / WAVE] = pv;
             }
@@ -198,15 +202,16 @@ struct wrow_solver {
                 const double coef = uniform(0.5 * tau * pvt);
                 double wc[NBC];
 #pragma unroll
-                for (int b = 0; b < NBC; ++b) wc[b] = sm.p[l + 64 * b] - coef * vc[b];
+                for (int b = 0; b < NBC; ++b) wc[b] = fma(-coef, vc[b], sm.p[l + 64 * b]);
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     if (g < bmin) continue;
 #pragma unroll
                     for (int a = 4 * g; a < 4 * g + 4; ++a) {
-                        const int src = (w + 16 * a) & 63;
-                        const double vra = readlane_d(vc[g], src);
-                        const double wra = readlane_d(wc[g], src);
+                        // row values by broadcast LDS reads, w_i formed as the lanes form w_j
+                        const int i = w + 16 * a;
+                        const double vra = sm.v[i];
+                        const double wra = fma(-coef, vra, sm.p[i]);
 #pragma unroll
                         for (int b = 0; b <= g; ++b) A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
                     }

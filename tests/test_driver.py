@@ -117,3 +117,22 @@ def test_driver_parameters_without_device(tmp_path):
     r = subprocess.run([EXE, "--cheb_moves", "true", "--output", str(tmp_path / "x")], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0
+
+
+@pytest.mark.gpu
+def test_driver_update_modes_agree(tmp_path):
+    """--update rank1 and --precision mixed run the same chains as the dense FP64 default:
+    the same occupations and energies within 1e-10 (decisions can only part at near-ties)."""
+    common = ["--lattice", "cubic2d", "--L", 6, "--beta", 4, "--U", 2, "--mu_c", 1, "--mu_f", 1, "--Nf_start", 18,
+              "--mc_add_remove", 0.5, "--mc_hop", 0.5, "--nsweeps", 10, "--sweep_len", 8, "--ntherm_sweeps", 1,
+              "--chains", 8, "--seed", 21]
+    runs = {}
+    for name, extra in (("dense", []), ("rank1", ["--update", "rank1", "--refresh", 16]),
+                        ("mixed", ["--precision", "mixed"])):
+        d = tmp_path / name
+        d.mkdir()
+        out = _run(d, *(common + extra))
+        runs[name] = {k: np.load(out / "mc_data" / f"{k}.npy") for k in ("energies", "focc_history")}
+    for name in ("rank1", "mixed"):
+        np.testing.assert_array_equal(runs[name]["focc_history"], runs["dense"]["focc_history"])
+        np.testing.assert_allclose(runs[name]["energies"], runs["dense"]["energies"], rtol=1e-10, atol=1e-10)

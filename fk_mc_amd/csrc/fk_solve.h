@@ -379,8 +379,8 @@ __device__ __forceinline__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], t
 // H = sum_k 1/(s - l_k)^2 = -dG/ds, the inputs of Laguerre's iteration for the
 // real-rooted polynomial det(T - s) (cubic convergence, monotone towards the
 // adjacent root in the chosen direction).
-//   1. every thread evaluates the count at one of NT equally spaced shifts in
-//      the Gershgorin interval (multisection);
+//   1. every thread evaluates the count at MS of MS*NT equally spaced shifts in
+//      the Gershgorin interval (multisection; two shifts share one pass);
 //   2. thread j takes the cell holding eigenvalue j, bisects until the cell
 //      holds that eigenvalue alone, then takes Laguerre steps towards it
 //      (the Sturm count says which side it is on); a step that leaves the
@@ -388,7 +388,7 @@ __device__ __forceinline__ void tridiagonalize(double (&A)[NB * (NB + 1) / 2], t
 //   3. once a step is tiny, one two-shift Sturm pass checks that l_j lies in
 //      (s - t, s + t]; if not, the bracket is updated and the iteration goes on
 //      from the window's edge on the eigenvalue's side.
-// de: LDS, >= 2*N doubles; cnt: LDS, >= NT+2 ints; scratch >= 3*NT/WAVE.
+// de: LDS, >= 2*N doubles; cnt: LDS, >= MS*NT+2 ints; scratch >= 3*NT/WAVE.
 // ---------------------------------------------------------------------------
 struct sturm_res {
     int count;
@@ -468,9 +468,11 @@ __device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N
     return c;
 }
 
-template <int NT>
+template <int NT, int MS = 1>
 __device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e, double* de_buf, int* cnt, int N,
                                                 double* ev_out, double* scratch) {
+    static_assert(MS == 1 || MS == 2, "one or two multisection shifts per thread");
+    constexpr int NC = MS * NT;  // interior multisection points
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1), wid = tid / WAVE;
     double2* de = reinterpret_cast<double2*>(de_buf);
@@ -504,19 +506,26 @@ __device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e
     const double atol = 2.0 * eps * tnorm;
     const double vtol = 8.0 * eps * fmax(tnorm, 1.0);   // verification half-window
     const double small = 1e-6 * fmax(tnorm, 1.0);        // Laguerre step below which to verify
-    const double h = (hi - lo) / (double)(NT + 1);
-    // 1. multisection: cnt[m] = count at lo + m*h, m = 0..NT+1
-    cnt[tid + 1] = sturm_count(de, N, lo + (double)(tid + 1) * h, pivmin);
-    if (tid == 0) { cnt[0] = 0; cnt[NT + 1] = N; }
+    const double h = (hi - lo) / (double)(NC + 1);
+    // 1. multisection: cnt[m] = count at lo + m*h, m = 0..NC+1
+    if (MS == 1) {
+        cnt[tid + 1] = sturm_count(de, N, lo + (double)(tid + 1) * h, pivmin);
+    } else {
+        int c1, c2;
+        sturm_count2(de, N, lo + (double)(2 * tid + 1) * h, lo + (double)(2 * tid + 2) * h, pivmin, c1, c2);
+        cnt[2 * tid + 1] = c1;
+        cnt[2 * tid + 2] = c2;
+    }
+    if (tid == 0) { cnt[0] = 0; cnt[NC + 1] = N; }
     __syncthreads();
     const double nd = (double)N;
     for (int j = tid; j < N; j += NT) {
-        int ml = 0, mh = NT + 1;  // invariant: cnt[ml] <= j < cnt[mh]
+        int ml = 0, mh = NC + 1;  // invariant: cnt[ml] <= j < cnt[mh]
         while (mh - ml > 1) {
             int mid = (ml + mh) >> 1;
             if (cnt[mid] > j) mh = mid; else ml = mid;
         }
-        double l = lo + (double)ml * h, u = (mh == NT + 1) ? hi : lo + (double)mh * h;
+        double l = lo + (double)ml * h, u = (mh == NC + 1) ? hi : lo + (double)mh * h;
         int cl = cnt[ml], cu = cnt[mh];
         double s = 0.5 * (l + u);
         double res = s;

@@ -66,7 +66,7 @@ struct wrow_solver {
     struct eig_area {
         double de[2 * NP];
         double ev[NP];
-        int cnt[NT + 2];
+        int cnt[2 * NT + 2];
         double scratch[3 * NWV + 8];
     };
     struct smem {
@@ -265,7 +265,7 @@ struct wrow_solver {
                 A[sl(a, b)] = v;
             }
         tridiagonalize(A, sm, N, vout, tauout);
-        tridiag_eigvals<NT>(sm.d, sm.e, sm.u.ea.de, sm.u.ea.cnt, N, sm.u.ea.ev, sm.u.ea.scratch);
+        tridiag_eigvals<NT, 2>(sm.d, sm.e, sm.u.ea.de, sm.u.ea.cnt, N, sm.u.ea.ev, sm.u.ea.scratch);
         return logz_block<NT>(sm.u.ea.ev, N, beta, cexp, cfermi, sm.u.ea.scratch);
     }
 };

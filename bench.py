@@ -16,6 +16,9 @@ Printed: one JSON line with the contract fields plus
                 stream) against MI355X's 78.6 TFLOP/s FP64 peak;
   plain         the single-chain drop-in path (calc_hamiltonian + calc_ed
                 through the C ABI with host buffers, one call per move);
+  single_chain_cubic2d_L8
+                one chain of 10^4 addremove moves at L=8 (N=64) on the device, and the
+                plain drop-in at that size;
   measured_sweeps
                 sweeps of 16 moves each followed by the on-device measures
                 (energy, spectrum mean, spectrum / occupation / IPR histories) of
@@ -81,6 +84,7 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=8, help="mixed-precision timing steps (0: skip)")
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
     ap.add_argument("--no-c4", action="store_true", help="skip the triangular adjoint workload")
+    ap.add_argument("--no-c1", action="store_true", help="skip the single-chain cubic2d L=8 timing")
     ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
     ap.add_argument("--measure-sweeps", type=int, default=4,
                     help="measured sweeps of 16 moves with on-device measures (0: skip)")
@@ -157,6 +161,10 @@ def main():
         plain = {"moves_per_s": round(plain_rate, 2), "ms_per_move": round((tp1 - tp0) * 1e3 / args.plain_moves, 4),
                  "batched_over_plain": round(value / plain_rate, 1)}
 
+    c1 = None
+    if not args.no_c1:
+        c1 = c1_block(fk)
+
     rank1 = None
     if args.rank1_steps > 0:
         rank1 = rank1_block(fk, ctx, f, beta, U, seeds, args.rank1_steps, plain, L)
@@ -202,6 +210,7 @@ def main():
         "accept_rate": round(acc / moves, 4),
         "roofline": roofline,
         "plain": plain,
+        "single_chain_cubic2d_L8": c1,
         "rank1_update": rank1,
         "measured_sweeps": measured,
         "mixed": mixed,
@@ -353,6 +362,35 @@ def c3_block(fk, steps):
     out.update(compare_traces(t64, tmx, beta))
     ctx.close()
     return out
+
+
+def c1_block(fk, nmoves=10000, nplain=2000):
+    """The single-chain configuration: cubic2d L=8 (N=64), U=1, beta=1, mu_c = mu_f = 0.5,
+    one chain of move_addremove only, 10^4 moves (1 chain x 10^4 steps in 625 launches of
+    16 steps, replayed from a graph), and the plain drop-in (calc_hamiltonian + calc_ed
+    through the C ABI, host buffers, one call per move) at the same size."""
+    L, N = 8, 64
+    f = (np.random.default_rng(8).random((1, N)) < 0.5).astype(np.int32)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, f, 1.0, 1.0, 0.5, 0.5, np.array([1], np.uint32))
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
+    ch.sweep_graph(nmoves)  # capture + warm-up
+    ch.sweep_graph(nmoves)
+    ms = ch.last_ms()
+    acc = int(ch.state()["naccept"][0])
+    ch.close()
+    fp = f.copy()
+    rng = np.random.default_rng(9)
+    ctx.eig_logz(fp, 1.0, 1.0, 0.5)
+    t0 = time.perf_counter()
+    for _ in range(nplain):
+        fp[0, rng.integers(N)] ^= 1
+        ctx.eig_logz(fp, 1.0, 1.0, 0.5)
+    plain = nplain / (time.perf_counter() - t0)
+    ctx.close()
+    return {"N": N, "moves": nmoves, "chain_moves_per_s": round(nmoves / ms * 1e3, 1),
+            "chain_us_per_move": round(ms * 1e3 / nmoves, 2), "accept_rate": round(acc / (2 * nmoves), 4),
+            "plain_moves_per_s": round(plain, 1)}
 
 
 def _omega_diff(a, b, beta):

@@ -19,6 +19,7 @@
 #include "fk_rng.h"
 #include "fk_solve.h"
 #include "fk_moves.h"
+#include "fk_wave.h"
 #include "fk_wrow.h"
 
 namespace fkk {
@@ -474,8 +475,10 @@ using namespace fkk;
 // Solver kinds: 0-2 register-resident one-stage, block-cyclic P x P grid (N <= 64 /
 // 128 / 256), 3 unblocked HBM workspace (N <= 2048), 4 blocked FP64 with MFMA
 // trailing updates (N <= 1024), 5 register-resident one-stage, wave-row layout
-// (N <= 256); 9 blocked FP32 (first stage of the mixed-precision path, N <= 1024).
+// (N <= 256), 6 register-resident one-wave (N <= 64); 9 blocked FP32 (first stage of
+// the mixed-precision path, N <= 1024).
 using fkw::wrow_solver;
+using fkq::wave_solver;
 typedef blocked_solver<double, 512> blk64;
 typedef blocked_solver<float, 512> blk32;
 
@@ -495,6 +498,7 @@ int fk_solver_nmax(int kind) {
         case 3: return glob_solver::NMAX;
         case 4: return blk64::NMAX;
         case 5: return wrow_solver::NMAX;
+        case 6: return wave_solver::NMAX;
         case 9: return blk32::NMAX;
     }
     return 0;
@@ -507,6 +511,7 @@ int fk_solver_threads(int kind) {
         case 3: return glob_solver::NT;
         case 4: return blk64::NT;
         case 5: return wrow_solver::NT;
+        case 6: return wave_solver::NT;
         case 9: return blk32::NT;
     }
     return 0;
@@ -529,6 +534,7 @@ hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) 
         case 3: return launch_eig<glob_solver>(a, grid, st);
         case 4: return launch_eig<blk64>(a, grid, st);
         case 5: return launch_eig<wrow_solver>(a, grid, st);
+        case 6: return launch_eig<wave_solver>(a, grid, st);
         case 9: return launch_eig<blk32>(a, grid, st);
     }
     return hipErrorInvalidValue;
@@ -559,6 +565,7 @@ hipError_t fk_launch_sweep_q(int kind, const sweep_args& a, int nsub, const swee
         case 3: return launch_sweep_q<glob_solver>(a, nsub, q, st);
         case 4: return launch_sweep_q<blk64>(a, nsub, q, st);
         case 5: return launch_sweep_q<wrow_solver>(a, nsub, q, st);
+        case 6: return launch_sweep_q<wave_solver>(a, nsub, q, st);
     }
     return hipErrorInvalidValue;
 }
@@ -571,6 +578,7 @@ hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t 
         case 3: return launch_sweep<glob_solver>(a, grid, st);
         case 4: return launch_sweep<blk64>(a, grid, st);
         case 5: return launch_sweep<wrow_solver>(a, grid, st);
+        case 6: return launch_sweep<wave_solver>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }

@@ -395,7 +395,7 @@ struct sturm_res {
     double G, H;
 };
 
-__device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ de, int N, double sigma,
+__device__ __forceinline__ sturm_res sturm_laguerre_q(const double2* __restrict__ de, int N, double sigma,
                                                     double pivmin) {
     double q = de[0].x - sigma;
     if (fabs(q) < pivmin) q = -pivmin;
@@ -429,7 +429,7 @@ __device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ 
 }
 
 // counts at two shifts in one pass (independent chains interleave)
-__device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int N, double s1, double s2,
+__device__ __forceinline__ void sturm_count2_q(const double2* __restrict__ de, int N, double s1, double s2,
                                              double pivmin, int& c1, int& c2) {
     double q1 = de[0].x - s1, q2 = de[0].x - s2;
     if (fabs(q1) < pivmin) q1 = -pivmin;
@@ -447,7 +447,7 @@ __device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int
     }
 }
 
-__device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N, double s, double pivmin) {
+__device__ __forceinline__ int sturm_count_q(const double2* __restrict__ de, int N, double s, double pivmin) {
     double q = de[0].x - s;
     if (fabs(q) < pivmin) q = -pivmin;
     int c = (q < 0.0);
@@ -466,6 +466,107 @@ __device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N
         c += (q < 0.0);
     }
     return c;
+}
+
+
+// The same passes in leading-minor form: P_i = det(T[0..i] - s), P_{-1} = 1,
+//   P_i = (d_i - s) P_{i-1} - e_{i-1}^2 P_{i-2},
+// so q_i = P_i / P_{i-1}, the count is the number of sign changes of P, and
+// G = P'/P, H = G^2 - P''/P (derivatives in s, carried by the differentiated
+// recurrence).  A step is one dependent FMA instead of a reciprocal, and its rounding
+// is that of the ratio form (relative perturbations of d_i - s and of e^2, one each),
+// so the counts are those of a nearby matrix as before.  The carried values are
+// rescaled by a power of two every 4 steps (exact); a pivot |q_i| < pivmin or a value
+// out of range sends the whole pass to the ratio form above.
+constexpr int SPF = 4;  // steps between rescalings
+
+__device__ __forceinline__ void rescale2(double& a1, double& a2, int ex) {
+    a1 = __builtin_amdgcn_ldexp(a1, -ex);
+    a2 = __builtin_amdgcn_ldexp(a2, -ex);
+}
+
+__device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ de, int N, double sigma,
+                                                    double pivmin) {
+    double p1 = 1.0, p2 = 0.0, d1 = 0.0, d2 = 0.0, h1 = 0.0, h2 = 0.0;
+    int cnt = 0;
+    bool ok = true;
+    auto step = [&](double2 t) {
+        const double c = t.x - sigma;
+        const double p = fma(c, p1, -t.y * p2);
+        const double dp = fma(c, d1, -fma(t.y, d2, p1));
+        const double hp = fma(c, h1, -fma(t.y, h2, 2.0 * d1));
+        ok &= fabs(p) >= pivmin * fabs(p1);  // false for NaN too
+        cnt += (p < 0.0) != (p1 < 0.0);
+        p2 = p1; p1 = p;
+        d2 = d1; d1 = dp;
+        h2 = h1; h1 = hp;
+    };
+    int i = 0;
+    for (; i + SPF <= N; i += SPF) {
+        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
+        step(t0); step(t1); step(t2); step(t3);
+        const int ex = __builtin_amdgcn_frexp_exp(p1);
+        rescale2(p1, p2, ex);
+        rescale2(d1, d2, ex);
+        rescale2(h1, h2, ex);
+    }
+    for (; i < N; ++i) step(de[i]);
+    const double G = d1 / p1;
+    const double H = fma(G, G, -h1 / p1);
+    if (!ok || !isfinite(G) || !isfinite(H)) return sturm_laguerre_q(de, N, sigma, pivmin);
+    return {cnt, G, H};
+}
+
+__device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N, double s, double pivmin) {
+    double p1 = 1.0, p2 = 0.0;
+    int c = 0;
+    bool ok = true;
+    auto step = [&](double2 t) {
+        const double p = fma(t.x - s, p1, -t.y * p2);
+        ok &= fabs(p) >= pivmin * fabs(p1);
+        c += (p < 0.0) != (p1 < 0.0);
+        p2 = p1; p1 = p;
+    };
+    int i = 0;
+    for (; i + SPF <= N; i += SPF) {
+        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
+        step(t0); step(t1); step(t2); step(t3);
+        rescale2(p1, p2, __builtin_amdgcn_frexp_exp(p1));
+    }
+    for (; i < N; ++i) step(de[i]);
+    if (!ok || !isfinite(p1)) return sturm_count_q(de, N, s, pivmin);
+    return c;
+}
+
+// counts at two shifts in one pass (independent chains interleave)
+__device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int N, double s1, double s2,
+                                             double pivmin, int& c1, int& c2) {
+    double a1 = 1.0, a2 = 0.0, b1 = 1.0, b2 = 0.0;
+    int ca = 0, cb = 0;
+    bool ok = true;
+    auto step = [&](double2 t) {
+        const double pa = fma(t.x - s1, a1, -t.y * a2);
+        const double pb = fma(t.x - s2, b1, -t.y * b2);
+        ok &= (fabs(pa) >= pivmin * fabs(a1)) & (fabs(pb) >= pivmin * fabs(b1));
+        ca += (pa < 0.0) != (a1 < 0.0);
+        cb += (pb < 0.0) != (b1 < 0.0);
+        a2 = a1; a1 = pa;
+        b2 = b1; b1 = pb;
+    };
+    int i = 0;
+    for (; i + SPF <= N; i += SPF) {
+        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
+        step(t0); step(t1); step(t2); step(t3);
+        rescale2(a1, a2, __builtin_amdgcn_frexp_exp(a1));
+        rescale2(b1, b2, __builtin_amdgcn_frexp_exp(b1));
+    }
+    for (; i < N; ++i) step(de[i]);
+    if (!ok || !isfinite(a1) || !isfinite(b1)) {
+        sturm_count2_q(de, N, s1, s2, pivmin, c1, c2);
+        return;
+    }
+    c1 = ca;
+    c2 = cb;
 }
 
 template <int NT, int MS = 1>

@@ -69,9 +69,10 @@ int fkgpu_msize(const fkgpu_ctx* ctx);
  *   3                       unblocked, matrix in an HBM workspace, N <= 2048
  *   4                       blocked (panel + MFMA rank-2nb trailing update), HBM workspace, N <= 1024
  *   5                       register-resident one-stage, 16 waves x 64 lanes (wave = row residue,
- *                           lane = column residue; row values by v_readlane), N <= 256 */
+ *                           lane = column residue; row values by v_readlane), N <= 256
+ *   6                       register-resident one-stage, one wave (lane = column), N <= 64 */
 #define FKGPU_SOLVER_AUTO (-1)
-#define FKGPU_SOLVER_VARIANTS 6
+#define FKGPU_SOLVER_VARIANTS 7
 int fkgpu_set_solver(fkgpu_ctx* ctx, int variant);
 int fkgpu_get_solver(const fkgpu_ctx* ctx); /* the variant used for the current lattice */
 /* Autotuner: times every variant that fits the lattice's N on `batch` synthetic

@@ -1,5 +1,5 @@
 """Dense eigensolver variants (register-resident one-stage, unblocked HBM,
-blocked with MFMA trailing updates) against the oracle, and the autotuner."""
+blocked with MFMA trailing updates, one-wave) against the oracle, and the autotuner."""
 import numpy as np
 import pytest
 
@@ -7,8 +7,9 @@ from test_gpu_chains import compare_chain
 
 pytestmark = pytest.mark.gpu
 
-VARIANT_NMAX = [64, 128, 256, 2048, 1024, 256]
-CASES = [(0, 2, 8), (0, 2, 16), (1, 2, 18), (0, 3, 7), (0, 3, 8), (1, 2, 24), (0, 2, 32)]
+VARIANT_NMAX = [64, 128, 256, 2048, 1024, 256, 64]
+CASES = [(0, 2, 8), (0, 2, 16), (1, 2, 18), (0, 3, 7), (0, 3, 8), (1, 2, 24), (0, 2, 32), (0, 1, 3), (0, 1, 5),
+         (0, 1, 64), (1, 2, 7)]
 
 
 @pytest.mark.parametrize("kind,D,L", CASES, ids=lambda c: str(c))

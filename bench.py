@@ -57,7 +57,8 @@ FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_BYTES = {5: 124.2e6}
 KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",
                 2: "fkk::k_sweep_q<reg_solver<32,8>>", 3: "fkk::k_sweep_q<glob_solver>",
-                4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver>"}
+                4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver>",
+                6: "fkk::k_sweep_q<wave_solver>"}
 
 
 def workload(L=16, B=1024, seed=20240601):

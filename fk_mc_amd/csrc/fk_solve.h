@@ -478,7 +478,7 @@ __device__ __forceinline__ int sturm_count_q(const double2* __restrict__ de, int
 // so the counts are those of a nearby matrix as before.  The carried values are
 // rescaled by a power of two every 4 steps (exact); a pivot |q_i| < pivmin or a value
 // out of range sends the whole pass to the ratio form above.
-constexpr int SPF = 4;  // steps between rescalings
+constexpr int SPF = 8;  // steps between rescalings
 
 __device__ __forceinline__ void rescale2(double& a1, double& a2, int ex) {
     a1 = __builtin_amdgcn_ldexp(a1, -ex);
@@ -503,8 +503,11 @@ __device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ 
     };
     int i = 0;
     for (; i + SPF <= N; i += SPF) {
-        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
-        step(t0); step(t1); step(t2); step(t3);
+        double2 t[SPF];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) t[q] = de[i + q];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) step(t[q]);
         const int ex = __builtin_amdgcn_frexp_exp(p1);
         rescale2(p1, p2, ex);
         rescale2(d1, d2, ex);
@@ -529,8 +532,11 @@ __device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N
     };
     int i = 0;
     for (; i + SPF <= N; i += SPF) {
-        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
-        step(t0); step(t1); step(t2); step(t3);
+        double2 t[SPF];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) t[q] = de[i + q];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) step(t[q]);
         rescale2(p1, p2, __builtin_amdgcn_frexp_exp(p1));
     }
     for (; i < N; ++i) step(de[i]);
@@ -555,8 +561,11 @@ __device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int
     };
     int i = 0;
     for (; i + SPF <= N; i += SPF) {
-        const double2 t0 = de[i], t1 = de[i + 1], t2 = de[i + 2], t3 = de[i + 3];
-        step(t0); step(t1); step(t2); step(t3);
+        double2 t[SPF];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) t[q] = de[i + q];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) step(t[q]);
         rescale2(a1, a2, __builtin_amdgcn_frexp_exp(a1));
         rescale2(b1, b2, __builtin_amdgcn_frexp_exp(b1));
     }

@@ -174,8 +174,13 @@ bool finite_all(const double* p, int n) {
 // N x N workspace per workgroup, so its grid is one batch of resident groups
 // and the kernel is launched repeatedly over slices of the batch.
 // Solver kind for the context's lattice: the forced / autotuned one when it fits N.
-int pick_kind(const fkgpu_ctx* ctx) {
+// Solver variant for a batch of `batch` matrices (0: unknown): the forced or autotuned one,
+// else by size; for N <= 64 a small batch (a single chain, the plain drop-in) takes the
+// one-wave solver, whose solve has the lowest latency, and larger batches the 16 x 16
+// block-cyclic one, which has the higher throughput.
+int pick_kind(const fkgpu_ctx* ctx, int batch = 0) {
     if (ctx->solver >= 0 && ctx->N <= fk_solver_nmax(ctx->solver)) return ctx->solver;
+    if (ctx->N <= 64 && batch > 0 && batch < 128) return 6;
     return fk_solver_kind(ctx->N);
 }
 
@@ -404,7 +409,7 @@ static int eig_pairs_dev(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
     FK_HIP(ctx, hipMemcpyAsync(o.beta, beta, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(o.U, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(o.mu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
-    const int kind = pick_kind(ctx);
+    const int kind = pick_kind(ctx, batch);
     const int grid = grid_for(ctx, kind, batch);
     eig_args a{};
     a.B = batch;
@@ -679,7 +684,7 @@ int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const d
         return fail(ctx, FKGPU_EINVAL, "null argument or empty batch");
     if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "dense eigensolver implemented for N <= 2048");
     const int N = ctx->N;
-    const int kind = pick_kind(ctx);
+    const int kind = pick_kind(ctx, batch);
     const int grid = grid_for(ctx, kind, batch);
     eig_args a{};
     a.B = batch;
@@ -739,7 +744,7 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     FK_HIP(ctx, hipMemcpyAsync(dbeta, beta, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dU, U, sizeof(double) * batch, hipMemcpyHostToDevice, st));
     FK_HIP(ctx, hipMemcpyAsync(dmu, mu_c, sizeof(double) * batch, hipMemcpyHostToDevice, st));
-    const int kind = pick_kind(ctx);
+    const int kind = pick_kind(ctx, batch);
     const int grid = grid_for(ctx, kind, batch);
     eig_args a{};
     a.B = batch;
@@ -1117,7 +1122,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     ctx->refs += 1;
     ch->B = B;
     ch->N = N;
-    ch->kind = pick_kind(ctx);
+    ch->kind = pick_kind(ctx, B);
     ch->dense_ok = ch->kind >= 0;
     ch->grid = grid_for(ctx, ch->kind, B);
     hipError_t e = hipSuccess;

@@ -54,7 +54,7 @@ FP32_PEAK_TFLOPS = 157.3
 # workload (cubic2d L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 +
 # WRITE_SIZE x 1.00 (scales from a 256 MiB copy, bench.py --copy-calibration); keyed by
 # solver variant
-TRAFFIC_BYTES = {5: 124.2e6}
+TRAFFIC_BYTES = {5: 124.5e6}
 KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",
                 2: "fkk::k_sweep_q<reg_solver<32,8>>", 3: "fkk::k_sweep_q<glob_solver>",
                 4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver>",

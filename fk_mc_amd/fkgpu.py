@@ -336,6 +336,7 @@ class Chains:
         f = np.ascontiguousarray(np.atleast_2d(f_init), dtype=np.int32)
         self.B, self.N = f.shape
         self.steps_per_launch = 16  # the library's default
+        self._mcap, self._mflags = 0, 0  # measurement records (measure_enable)
         B = self.B
 
         def vec(x, dt=np.float64):

@@ -83,8 +83,16 @@ struct chain_meas_args {
     double* spec_hist;             // B x cap x N or null
     uint8_t* focc_hist;            // B x cap x N or null
     double* ipr_hist;              // B x cap x N or null
+    // stiffness and conductivity of this measurement (null: not measured)
+    int nw;
+    const double* stiff;           // B
+    const double* cond;            // B x nw
+    double *stiff_hist, *stiff_sum;  // B x cap, B
+    double *cond_hist, *cond_sum;    // B x cap x nw, B x nw
 };
 hipError_t fk_launch_chain_measure(const chain_meas_args& a, long long* count, hipStream_t st);
+// fermi[b][k] = 1 / (1 + exp(beta_b ev[b][k])) (ed_cache::cached_fermi, src/configuration.cpp:238)
+hipError_t fk_launch_fermi(int B, int N, const double* ev, const double* beta, double* fermi, hipStream_t st);
 
 // One proposed move (kept between the launches of a stochastic-trace move step).
 struct move_prop {

@@ -17,8 +17,9 @@
 //                    device: measure_energy (src/measures/energy.cpp:6-26),
 //                    measure_spectrum (src/measures/spectrum.cpp:13-21),
 //                    measure_spectrum_history (src/measures/spectrum_history.cpp:13-19),
-//                    measure_focc (src/measures/focc_history.cpp:8-13) and the IPR
-//                    history of measure_ipr.
+//                    measure_focc (src/measures/focc_history.cpp:8-13), the IPR
+//                    history of measure_ipr and the stiffness / conductivity history
+//                    of measure_stiffness.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -288,6 +289,19 @@ __global__ void __launch_bounds__(CM_T) k_chain_measure(chain_meas_args a) {
         a.focc_sum[row + k] += fi;
         if (rec && a.focc_hist) a.focc_hist[hrow + k] = fi;
     }
+    if (a.stiff) {
+        const size_t crow = (size_t)b * a.nw;
+        const size_t chrow = ((size_t)b * a.cap + (size_t)(rec ? m : 0)) * a.nw;
+        for (int q = tid; q < a.nw; q += CM_T) {
+            const double c = a.cond[crow + q];
+            a.cond_sum[crow + q] += c;
+            if (rec) a.cond_hist[chrow + q] = c;
+        }
+        if (tid == 0) {
+            a.stiff_sum[b] += a.stiff[b];
+            if (rec) a.stiff_hist[(size_t)b * a.cap + m] = a.stiff[b];
+        }
+    }
     ec = wave_sum(ec);
     d2 = wave_sum(d2);
     const int w = tid / WAVE;
@@ -314,6 +328,13 @@ __global__ void __launch_bounds__(CM_T) k_chain_measure(chain_meas_args a) {
         a.esum[b] += e;
         a.d2sum[b] += d2;
     }
+}
+
+__global__ void k_fermi(int B, int N, const double* __restrict__ ev, const double* __restrict__ beta,
+                        double* __restrict__ fermi) {
+    const int b = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || k >= N) return;
+    fermi[(size_t)b * N + k] = 1.0 / (1.0 + exp(beta[b] * ev[(size_t)b * N + k]));
 }
 
 // count += 1, after every chain has recorded the measurement.  One thread.
@@ -373,5 +394,10 @@ hipError_t fk_launch_chain_measure(const chain_meas_args& a, long long* count, h
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     hipLaunchKernelGGL(k_chain_measure_tick, dim3(1), dim3(64), 0, st, count);
+    return hipGetLastError();
+}
+
+hipError_t fk_launch_fermi(int B, int N, const double* ev, const double* beta, double* fermi, hipStream_t st) {
+    hipLaunchKernelGGL(k_fermi, dim3((N + 255) / 256, B), dim3(256), 0, st, B, N, ev, beta, fermi);
     return hipGetLastError();
 }

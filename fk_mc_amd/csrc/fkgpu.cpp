@@ -54,8 +54,16 @@ struct chain_meas_bufs {
     double* spec_hist = nullptr;                         // B x cap x N
     uint8_t* focc_hist = nullptr;                        // B x cap x N
     double *ipr = nullptr, *ipr_hist = nullptr;          // B x N, B x cap x N
+    // stiffness / conductivity on nw frequencies with Lorentzian width `offset`
+    int nw = 0;
+    double offset = 0.05;
+    double *wgrid = nullptr, *fermi = nullptr, *JV = nullptr, *mJ = nullptr, *tdiag = nullptr;
+    double *stiff = nullptr, *cond = nullptr;                   // B, B x nw (this measurement)
+    double *stiff_hist = nullptr, *cond_hist = nullptr;         // B x cap, B x cap x nw
+    double *stiff_sum = nullptr, *cond_sum = nullptr;           // B, B x nw
     // decomposition of the current configurations, for chain sets that keep no
-    // eigenvalues (stochastic trace) or no eigenvectors (IPR outside the rank-one mode)
+    // eigenvalues (stochastic trace) or no eigenvectors (IPR / stiffness outside the
+    // rank-one mode)
     int32_t* f32 = nullptr;
     double *ev = nullptr, *lz = nullptr, *vh = nullptr, *tau = nullptr, *d = nullptr, *eo = nullptr;
     double *V = nullptr, *scr = nullptr;
@@ -64,7 +72,9 @@ struct chain_meas_bufs {
 static void meas_free(chain_meas_bufs* m) {
     if (!m) return;
     void* ps[] = {m->count, m->e, m->d2e, m->ce, m->esum, m->d2sum, m->spec_mean, m->focc_sum, m->spec_hist,
-                  m->focc_hist, m->ipr, m->ipr_hist, m->f32, m->ev, m->lz, m->vh, m->tau, m->d, m->eo, m->V, m->scr};
+                  m->focc_hist, m->ipr, m->ipr_hist, m->f32, m->ev, m->lz, m->vh, m->tau, m->d, m->eo, m->V, m->scr,
+                  m->wgrid, m->fermi, m->JV, m->mJ, m->tdiag, m->stiff, m->cond, m->stiff_hist, m->cond_hist,
+                  m->stiff_sum, m->cond_sum};
     for (void* p : ps)
         if (p) hipFree(p);
     delete m;
@@ -117,6 +127,8 @@ struct fkgpu_chains {
     long long q_ticks = 0;           // bound of one wait, in wall-clock ticks
     // on-device measurements and the graph of one measured sweep
     chain_meas_bufs* ms = nullptr;
+    std::vector<double> m_wgrid{0.0};  // conductivity frequencies of the stiffness measure
+    double m_offset = 0.05;            // their Lorentzian width (cond_offset)
     hipGraphExec_t mexec = nullptr;
     hipGraph_t mgraph = nullptr;
     int mgraph_len = 0;
@@ -1773,7 +1785,7 @@ static int meas_prepare(fkgpu_chains* ch) {
     if (!m) return fail(ctx, FKGPU_EINVAL, "measurements are not enabled on this chain set");
     if (!ch->kpm && !ch->dense_ok)
         return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
-    const bool ipr = m->flags & FKGPU_MEASURE_IPR;
+    const bool ipr = m->flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS);
     if (!ch->kpm && !(ipr && !ch->fu)) return FKGPU_OK;
     if (!ch->dense_ok) return fail(ctx, FKGPU_EUNSUPPORTED, "measurements need the dense eigensolver (N <= 2048)");
     const int B = ch->B, N = ch->N;
@@ -1800,9 +1812,11 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
     fkgpu_ctx* ctx = ch->ctx;
     chain_meas_bufs* m = ch->ms;
     const int B = ch->B, N = ch->N;
-    const bool ipr = m->flags & FKGPU_MEASURE_IPR;
+    const bool vec = m->flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS);
+    const bool ipr = vec;  // eigenvectors needed
     const double* spec = ch->spectrum;
     const double* V = (ipr && ch->fu) ? ch->fu_V : nullptr;
+    const double* specV = ch->spectrum;  // the eigenvalues that belong to V
     hipError_t e;
     if (ch->kpm || (ipr && !ch->fu)) {
         if ((e = fk_launch_f8to32(B * N, ch->f, m->f32, st)) != hipSuccess) return e;
@@ -1829,10 +1843,20 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
             e = fk_launch_evecs(B, N, m->d, m->eo, m->ev, m->vh, m->tau, m->V, m->scr, st);
             if (e != hipSuccess) return e;
             V = m->V;
+            specV = m->ev;
         }
         if (ch->kpm) spec = m->ev;
     }
-    if (V && (e = fk_launch_ipr(B, N, V, m->ipr, st)) != hipSuccess) return e;
+    if (V && (m->flags & FKGPU_MEASURE_IPR) && (e = fk_launch_ipr(B, N, V, m->ipr, st)) != hipSuccess) return e;
+    if (V && (m->flags & FKGPU_MEASURE_STIFFNESS)) {
+        // measure_stiffness::accumulate on the chains' eigenpairs, direction 0, T_x hopping 1
+        if ((e = fk_launch_fermi(B, N, specV, ch->beta, m->fermi, st)) != hipSuccess) return e;
+        int stride0 = 1;
+        for (int d = 1; d < ctx->lat_ndim; ++d) stride0 *= ctx->lat_L;
+        e = fk_launch_stiffness(B, N, stride0, ctx->lat_L, 1.0, V, specV, m->fermi, m->nw, m->wgrid, m->offset, m->JV,
+                                m->mJ, m->tdiag, m->stiff, m->cond, st);
+        if (e != hipSuccess) return e;
+    }
     chain_meas_args a{};
     a.B = B;
     a.N = N;
@@ -1844,7 +1868,15 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
     a.spectrum = spec;
     a.beta = ch->beta;
     a.mu_f = ch->mu_f;
-    a.ipr = V ? m->ipr : nullptr;
+    a.ipr = (V && (m->flags & FKGPU_MEASURE_IPR)) ? m->ipr : nullptr;
+    const bool stf = V && (m->flags & FKGPU_MEASURE_STIFFNESS);
+    a.nw = m->nw;
+    a.stiff = stf ? m->stiff : nullptr;
+    a.cond = m->cond;
+    a.stiff_hist = m->stiff_hist;
+    a.stiff_sum = m->stiff_sum;
+    a.cond_hist = m->cond_hist;
+    a.cond_sum = m->cond_sum;
     a.count = m->count;
     a.e = m->e;
     a.d2e = m->d2e;
@@ -1871,10 +1903,13 @@ static int meas_count(fkgpu_chains* ch, long long* count) {
 int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
-    const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | FKGPU_MEASURE_IPR;
+    const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | FKGPU_MEASURE_IPR |
+                    FKGPU_MEASURE_STIFFNESS;
     if (capacity < 0 || (flags & ~all)) return fail(ctx, FKGPU_EINVAL, "invalid capacity or measurement flags");
-    if ((flags & FKGPU_MEASURE_IPR) && (!ch->dense_ok || ch->N > 1024))
-        return fail(ctx, FKGPU_EUNSUPPORTED, "the IPR measure needs eigenvectors, implemented for N <= 1024");
+    if ((flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS)) && (!ch->dense_ok || ch->N > 1024))
+        return fail(ctx, FKGPU_EUNSUPPORTED, "IPR and stiffness need eigenvectors, implemented for N <= 1024");
+    if ((flags & FKGPU_MEASURE_STIFFNESS) && (ctx->lat_kind != FKGPU_LATTICE_HYPERCUBIC || ctx->lat_ndim < 2))
+        return fail(ctx, FKGPU_EUNSUPPORTED, "stiffness needs a hypercubic lattice of dimension >= 2");
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     FK_HIP(ctx, hipStreamSynchronize(st));
@@ -1902,6 +1937,23 @@ int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
         chk(dalloc(&m->ipr, BN));
         chk(dalloc(&m->ipr_hist, BcN));
     }
+    const int nw = (int)ch->m_wgrid.size();
+    m->nw = nw;
+    m->offset = ch->m_offset;
+    if (flags & FKGPU_MEASURE_STIFFNESS) {
+        const size_t BNN = BN * N;
+        chk(dalloc(&m->wgrid, nw));
+        chk(dalloc(&m->fermi, BN));
+        chk(dalloc(&m->JV, BNN));
+        chk(dalloc(&m->mJ, BNN));
+        chk(dalloc(&m->tdiag, BN));
+        chk(dalloc(&m->stiff, B));
+        chk(dalloc(&m->cond, (size_t)B * nw));
+        chk(dalloc(&m->stiff_hist, Bc));
+        chk(dalloc(&m->cond_hist, Bc * nw));
+        chk(dalloc(&m->stiff_sum, B));
+        chk(dalloc(&m->cond_sum, (size_t)B * nw));
+    }
     if (e != hipSuccess) {
         meas_free(m);
         return fail(ctx, FKGPU_ENOMEM, std::string("measurement records: ") + hipGetErrorString(e));
@@ -1911,12 +1963,58 @@ int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
     chk(hipMemsetAsync(m->d2sum, 0, sizeof(double) * B, st));
     chk(hipMemsetAsync(m->spec_mean, 0, sizeof(double) * BN, st));
     chk(hipMemsetAsync(m->focc_sum, 0, sizeof(long long) * BN, st));
+    if (flags & FKGPU_MEASURE_STIFFNESS) {
+        chk(hipMemcpyAsync(m->wgrid, ch->m_wgrid.data(), sizeof(double) * nw, hipMemcpyHostToDevice, st));
+        chk(hipMemsetAsync(m->stiff_sum, 0, sizeof(double) * B, st));
+        chk(hipMemsetAsync(m->cond_sum, 0, sizeof(double) * B * nw, st));
+    }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
         meas_free(m);
         return hipfail(ctx, e, "measurement records");
     }
     ch->ms = m;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_measure_conductivity(fkgpu_chains* ch, int nw, const double* wgrid, double offset) {
+    if (!ch) return FKGPU_EINVAL;
+    if (nw < 1 || !wgrid || !finite_all(wgrid, nw) || !(offset > 0.0) || !std::isfinite(offset))
+        return fail(ch->ctx, FKGPU_EINVAL, "conductivity grid: nw >= 1 finite frequencies and a positive width");
+    ch->m_wgrid.assign(wgrid, wgrid + nw);
+    ch->m_offset = offset;
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_stiffness(fkgpu_chains* ch, int64_t* count, double* stiffness, double* cond, double* stiffness_mean,
+                           double* cond_mean) {
+    if (!ch) return FKGPU_EINVAL;
+    long long c = 0;
+    int r = meas_count(ch, &c);
+    if (r) return r;
+    fkgpu_ctx* ctx = ch->ctx;
+    chain_meas_bufs* m = ch->ms;
+    if (!(m->flags & FKGPU_MEASURE_STIFFNESS)) return fail(ctx, FKGPU_EINVAL, "stiffness was not enabled");
+    if (count) *count = c;
+    const int B = ch->B, nw = m->nw;
+    const size_t n = (size_t)std::min<long long>(c, m->cap);
+    if (n > 0) {
+        if (stiffness)
+            FK_HIP(ctx, hipMemcpy2D(stiffness, sizeof(double) * n, m->stiff_hist, sizeof(double) * m->cap,
+                                    sizeof(double) * n, B, hipMemcpyDeviceToHost));
+        if (cond)
+            FK_HIP(ctx, hipMemcpy2D(cond, sizeof(double) * n * nw, m->cond_hist, sizeof(double) * m->cap * nw,
+                                    sizeof(double) * n * nw, B, hipMemcpyDeviceToHost));
+    }
+    const double z = c > 0 ? (double)c : 1.0;
+    if (stiffness_mean) {
+        FK_HIP(ctx, hipMemcpy(stiffness_mean, m->stiff_sum, sizeof(double) * B, hipMemcpyDeviceToHost));
+        for (int b = 0; b < B; ++b) stiffness_mean[b] /= z;
+    }
+    if (cond_mean) {
+        FK_HIP(ctx, hipMemcpy(cond_mean, m->cond_sum, sizeof(double) * B * nw, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < (size_t)B * nw; ++i) cond_mean[i] /= z;
+    }
     return FKGPU_OK;
 }
 

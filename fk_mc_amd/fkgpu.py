@@ -19,7 +19,7 @@ MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
 SOLVER_AUTO, SOLVER_VARIANTS = -1, 7
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 UPDATE_DENSE, UPDATE_RANK1 = 0, 1
-MEASURE_SPECTRUM_HISTORY, MEASURE_FOCC_HISTORY, MEASURE_IPR = 1, 2, 4
+MEASURE_SPECTRUM_HISTORY, MEASURE_FOCC_HISTORY, MEASURE_IPR, MEASURE_STIFFNESS = 1, 2, 4, 8
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -110,6 +110,8 @@ def lib():
         L.fkgpu_chains_energies.argtypes = [P, _i64, _d, _d, _d]
         L.fkgpu_chains_means.argtypes = [P, _i64, _d, _d, _d, _d]
         L.fkgpu_chains_histories.argtypes = [P, _i64, _d, _i32, _d]
+        L.fkgpu_chains_measure_conductivity.argtypes = [P, C.c_int, _d, C.c_double]
+        L.fkgpu_chains_stiffness.argtypes = [P, _i64, _d, _d, _d, _d]
         _LIB = L
     return _LIB
 
@@ -430,6 +432,11 @@ class Chains:
         self.ctx._check(lib().fkgpu_chains_measure_enable(self._h, capacity, flags))
         self._mcap, self._mflags = capacity, flags
 
+    def measure_conductivity(self, wgrid, offset=0.05):
+        w = np.ascontiguousarray(wgrid, dtype=np.float64)
+        self.ctx._check(lib().fkgpu_chains_measure_conductivity(self._h, len(w), _p(w, _d), offset))
+        self._nw = len(w)
+
     def measure(self):
         self.ctx._check(lib().fkgpu_chains_measure(self._h))
 
@@ -456,6 +463,17 @@ class Chains:
         sp, fo = np.zeros((B, N)), np.zeros((B, N))
         self.ctx._check(lib().fkgpu_chains_means(self._h, _p(c, _i64), _p(e, _d), _p(d2, _d), _p(sp, _d), _p(fo, _d)))
         return dict(count=int(c[0]), energy=e, d2energy=d2, spectrum=sp, focc=fo)
+
+    def stiffness(self):
+        """(count, stiffness [B, n], cond [B, n, nw], stiffness_mean [B], cond_mean [B, nw])."""
+        B, nw = self.B, getattr(self, "_nw", 1)
+        n = min(self._mcount(), self._mcap)
+        st, cd = np.zeros((B, n)), np.zeros((B, n, nw))
+        sm, cm = np.zeros(B), np.zeros((B, nw))
+        c = np.zeros(1, np.int64)
+        self.ctx._check(lib().fkgpu_chains_stiffness(self._h, _p(c, _i64), _p(st, _d), _p(cd, _d), _p(sm, _d),
+                                                     _p(cm, _d)))
+        return int(c[0]), st, cd, sm, cm
 
     def histories(self):
         """Enabled histories, each [B, min(count, capacity), N] (None where not enabled)."""

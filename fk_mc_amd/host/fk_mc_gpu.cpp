@@ -14,14 +14,16 @@
 // chain's generator, the moves flip / add_remove / reshuffle registered in the
 // reference's order with their weights, ntherm_sweeps unmeasured sweeps, then nsweeps
 // sweeps of sweep_len moves each followed by the measures (energy, spectrum, spectrum
-// and f-occupation histories when measure_history, IPR when measure_ipr), all kept on
+// and f-occupation histories when measure_history, IPR when measure_ipr, stiffness and
+// conductivity when measure_stiffness), all kept on
 // the device until the end.
 //
 // Output (`output`, a directory; a trailing .h5 is dropped): the observables of
 // observables_t (include/fk_mc/fk_mc.hpp:13-28) under mc_data/ as .npy files with the
 // chain as the leading axis -- energies, d2energies, c_energies [chains][nsweeps],
 // spectrum [chains][N] (running mean), spectrum_history, focc_history, ipr_history
-// [chains][nsweeps][N] -- plus dos.npy (the DOS of the recorded spectra on the
+// [chains][nsweeps][N], stiffness [chains][nsweeps] and cond_history [chains][nsweeps][nw]
+// with measure_stiffness on the reference's conductivity grid -- plus dos.npy (the DOS of the recorded spectra on the
 // reference's real grid, prog/data_save.hxx:67-69,264-345), parameters.txt and
 // stats.txt (means with errors from the spread over chains, and the specific heat
 // beta^2 (<E^2> - <d2E> - <E>^2) / N of prog/data_save.hxx:185-190).
@@ -103,7 +105,7 @@ params define_parameters() {
     p.define("measure_ipr", "false", "Measure inverse participation ratio");
     p.define("measure_eigenfunctions", "false", "Measure eigenfunctions (not in the batched program)");
     p.define("cond_offset", "0.05", "dos offset from the real axis");
-    p.define("measure_stiffness", "false", "Measure stiffness/conductivity (not in the batched program)");
+    p.define("measure_stiffness", "false", "Measure stiffness/conductivity");
     // mc_metropolis::define_parameters (src/mc_metropolis.cpp:11-19)
     p.define("nsweeps", "1024", "Total number of sweeps (1 sweep = #sweep_len moves + 1 measurement)");
     p.define("sweep_len", "16", "Number of moves between subsequent measurements");
@@ -119,6 +121,7 @@ params define_parameters() {
     p.define("dos_width", "6", "Width of DOS");
     p.define("dos_npts", "240", "Number of points for DOS sampling");
     p.define("dos_offset", "0.05", "DOS offset from real axis");
+    p.define("cond_npoints", "150", "number of points to sample conductivity");
     // batched program
     p.define("lattice", "cubic2d", "cubic1d | cubic2d | cubic3d | triangular");
     p.define("chains", "1", "Number of chains (chain r = MPI rank r of the reference, seeded seed + r)");
@@ -214,9 +217,9 @@ int run(params& p) {
     else if (lat == "cubic3d") D = 3;
     else if (lat == "triangular") kind = FKGPU_LATTICE_TRIANGULAR;
     else throw std::runtime_error("unknown lattice " + lat);
-    if (p.b("cheb_moves") || p.b("measure_eigenfunctions") || p.b("measure_stiffness"))
-        throw std::runtime_error("cheb_moves, measure_eigenfunctions and measure_stiffness are not part of the "
-                                 "batched program (use the host layer or fkgpu_measure_stiffness)");
+    if (p.b("cheb_moves") || p.b("measure_eigenfunctions"))
+        throw std::runtime_error("cheb_moves and measure_eigenfunctions are not part of the batched program "
+                                 "(use the host layer)");
     const int L = (int)p.i("L"), R = (int)p.i("chains");
     const long nsweeps = p.i("nsweeps"), sweep_len = p.i("sweep_len"), ntherm = p.i("ntherm_sweeps");
     if (R < 1 || nsweeps < 0 || sweep_len < 1 || ntherm < 0) throw std::runtime_error("invalid chains / sweeps");
@@ -258,7 +261,22 @@ int run(params& p) {
     }
     if (p.str("precision") == "mixed") FK_CHECK(fkgpu_chains_set_precision(ch, FKGPU_PRECISION_MIXED), ctx);
     if (p.str("update") == "rank1") FK_CHECK(fkgpu_chains_set_update(ch, FKGPU_UPDATE_RANK1, (int)p.i("refresh")), ctx);
-    int flags = (history ? FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY : 0) | (ipr ? FKGPU_MEASURE_IPR : 0);
+    // conductivity grid of prog/fk_mc_exec.cpp:127-139: 2 cond_npoints + 1 frequencies, log-spaced
+    // from wmax e^-15 to wmax = max(8, 2U) on either side of 0
+    const bool stiff = p.b("measure_stiffness");
+    std::vector<double> wgrid;
+    if (stiff) {
+        const int nw = (int)p.i("cond_npoints");
+        const double wmax = std::max(8.0, 2.0 * U);
+        std::vector<double> w2(nw);
+        for (int i = 0; i < nw; ++i) w2[i] = wmax * std::exp(nw > 1 ? -15.0 + 15.0 * i / (nw - 1.0) : 0.0);
+        for (int i = nw - 1; i >= 0; --i) wgrid.push_back(-w2[i]);
+        wgrid.push_back(0.0);
+        for (int i = 0; i < nw; ++i) wgrid.push_back(w2[i]);
+        FK_CHECK(fkgpu_chains_measure_conductivity(ch, (int)wgrid.size(), wgrid.data(), p.d("cond_offset")), ctx);
+    }
+    int flags = (history ? FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY : 0) | (ipr ? FKGPU_MEASURE_IPR : 0) |
+                (stiff ? FKGPU_MEASURE_STIFFNESS : 0);
     FK_CHECK(fkgpu_chains_measure_enable(ch, (int)nsweeps, flags), ctx);
 
     const bool show = p.b("show_output");
@@ -289,6 +307,9 @@ int run(params& p) {
         FK_CHECK(fkgpu_chains_histories(ch, &count, history ? sh.data() : nullptr, history ? fh.data() : nullptr,
                                         ipr ? ih.data() : nullptr),
                  ctx);
+    const size_t NW = wgrid.size();
+    std::vector<double> stf(stiff ? R * S : 0), cnd(stiff ? R * S * NW : 0), stfm(stiff ? R : 0);
+    if (stiff) FK_CHECK(fkgpu_chains_stiffness(ch, &count, stf.data(), cnd.data(), stfm.data(), nullptr), ctx);
     std::vector<int64_t> nacc(R);
     FK_CHECK(fkgpu_chains_state(ch, nullptr, nullptr, nullptr, nullptr, nacc.data(), nullptr), ctx);
     // DOS of the recorded spectra (or of the mean spectra without histories)
@@ -323,6 +344,11 @@ int run(params& p) {
         write_npy(md + "focc_history.npy", fd.data(), {(size_t)R, S, (size_t)N});
     }
     if (ipr) write_npy(md + "ipr_history.npy", ih.data(), {(size_t)R, S, (size_t)N});
+    if (stiff) {
+        write_npy(md + "stiffness.npy", stf.data(), {(size_t)R, S});
+        write_npy(md + "cond_history.npy", cnd.data(), {(size_t)R, S, NW});
+        write_npy(out + "/wgrid_conductivity.npy", wgrid.data(), {NW});
+    }
     std::vector<double> dg(2 * npts);
     for (int i = 0; i < npts; ++i) dg[2 * i] = grid[i], dg[2 * i + 1] = dos[i];
     write_npy(out + "/dos.npy", dg.data(), {(size_t)npts, 2});
@@ -366,6 +392,7 @@ int run(params& p) {
     line("cv", cv);
     line("nf/N", nfm);
     line("acceptance", accr);
+    if (stiff) line("stiffness", stfm);
     st << "seconds = " << secs << "\nmoves_per_second = " << (double)R * (ntherm + nsweeps) * sweep_len / secs
        << "\n";
     if (show) std::printf("  %.3f s, %.4g moves/s, output in %s/\n", secs,

@@ -251,6 +251,7 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
  *   measure_spectrum_history::accumulate  src/measures/spectrum_history.cpp:13-19
  *   measure_focc::accumulate              src/measures/focc_history.cpp:8-13
  *   measure_ipr::accumulate               include/fk_mc/measures/ipr.hpp:39-56
+ *   measure_stiffness::accumulate         include/fk_mc/measures/stiffness.hpp:129-187
  * The spectrum is the chain's own (the rank-one, mixed and FP64 modes keep it); chain
  * sets on the stochastic trace diagonalise their configurations at each measurement,
  * as the reference's calc_ed(false) does, and the IPR takes the eigenvectors of the
@@ -261,9 +262,14 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
 #define FKGPU_MEASURE_SPECTRUM_HISTORY 1
 #define FKGPU_MEASURE_FOCC_HISTORY 2
 #define FKGPU_MEASURE_IPR 4
+#define FKGPU_MEASURE_STIFFNESS 8 /* hypercubic lattices, D >= 2, N <= 1024 */
 /* Allocate the records and clear them (measurement count 0).  Replaces the
  * observables the reference's measures are registered on (include/fk_mc/fk_mc.hxx:97-123). */
 int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags);
+/* Frequencies and Lorentzian width of the conductivity that the stiffness measure
+ * records (measure_stiffness's wgrid and offset, include/fk_mc/measures/stiffness.hpp:69-127;
+ * default one frequency 0.0, width 0.05).  Takes effect at fkgpu_chains_measure_enable. */
+int fkgpu_chains_measure_conductivity(fkgpu_chains* ch, int nw, const double* wgrid, double offset);
 /* One measurement of every chain now (measure_*::accumulate). */
 int fkgpu_chains_measure(fkgpu_chains* ch);
 /* nsweeps x (sweep_len move steps, then one measurement), replayed from one captured
@@ -282,6 +288,11 @@ int fkgpu_chains_means(fkgpu_chains* ch, int64_t* count, double* energy, double*
 /* Histories of the first n = min(count, capacity) measurements, [nchains][n][N]
  * (nullable outputs; each must have been enabled): spectra, occupations, IPRs. */
 int fkgpu_chains_histories(fkgpu_chains* ch, int64_t* count, double* spectrum, int32_t* focc, double* ipr);
+/* Stiffness records (nullable outputs): stiffness [nchains][n] = (T + V)/N, cond
+ * [nchains][n][nw] on the conductivity grid, and their means over all measurements
+ * ([nchains], [nchains][nw]); n = min(count, capacity). */
+int fkgpu_chains_stiffness(fkgpu_chains* ch, int64_t* count, double* stiffness, double* cond, double* stiffness_mean,
+                           double* cond_mean);
 
 /* Diagnostic (host only, no device needed; no reference counterpart): draws
  * from the library's port of std::mt19937(seed) and the libstdc++

@@ -136,3 +136,29 @@ def test_driver_update_modes_agree(tmp_path):
     for name in ("rank1", "mixed"):
         np.testing.assert_array_equal(runs[name]["focc_history"], runs["dense"]["focc_history"])
         np.testing.assert_allclose(runs[name]["energies"], runs["dense"]["energies"], rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_driver_stiffness(tmp_path):
+    """--measure_stiffness: the stiffness and conductivity records on the reference's
+    conductivity grid (prog/fk_mc_exec.cpp:127-139) equal fkgpu_measure_stiffness of the
+    recorded occupations."""
+    import fk_mc_amd as fk
+    L, U, beta, mu = 6, 2.0, 5.0, 1.0
+    out = _run(tmp_path, "--lattice", "cubic2d", "--L", L, "--beta", beta, "--U", U, "--mu_c", mu, "--mu_f", mu,
+               "--Nf_start", 18, "--nsweeps", 3, "--sweep_len", 6, "--ntherm_sweeps", 1, "--chains", 2,
+               "--seed", 5, "--measure_stiffness", "true", "--cond_npoints", 4, "--cond_offset", 0.05)
+    md = out / "mc_data"
+    st, cd = np.load(md / "stiffness.npy"), np.load(md / "cond_history.npy")
+    fh = np.load(md / "focc_history.npy").astype(np.int32)
+    w = np.load(out / "wgrid_conductivity.npy")
+    w2 = 8.0 * np.exp(np.linspace(-15.0, 0.0, 4))
+    np.testing.assert_allclose(w, np.concatenate([-w2[::-1], [0.0], w2]), rtol=1e-15)
+    assert st.shape == (2, 3) and cd.shape == (2, 3, 9)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    for r in range(2):
+        ref_st, ref_cd = ctx.measure_stiffness(fh[r], beta, U, mu, w, 0.05)
+        np.testing.assert_allclose(st[r], ref_st, rtol=1e-9, atol=1e-11)
+        np.testing.assert_allclose(cd[r], ref_cd, rtol=1e-9, atol=1e-11)
+    ctx.close()
+    assert "stiffness" in open(out / "stats.txt").read()

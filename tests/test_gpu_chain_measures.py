@@ -10,7 +10,9 @@ fkgpu_sweep_measure and the readers) against the oracle:
   recorded occupations;
 * measure_ipr (include/fk_mc/measures/ipr.hpp:39-56) on the oracle's eigenvectors of the
   recorded occupations, for eigenvalues separated from their neighbours (the IPR of a
-  degenerate eigenspace depends on the basis chosen inside it).
+  degenerate eigenspace depends on the basis chosen inside it);
+* measure_stiffness (include/fk_mc/measures/stiffness.hpp:129-187) against the
+  single-configuration entry point and the oracle.
 
 Measuring must not change the chains: a measured chain set ends bit-identical to an
 unmeasured one with the same seeds.
@@ -220,8 +222,12 @@ def test_measure_errors(fk):
     with pytest.raises(fk.FkgpuError):
         ch.measure_enable(-1, 0)
     with pytest.raises(fk.FkgpuError):
-        ch.measure_enable(4, 8)
+        ch.measure_enable(4, 16)
+    with pytest.raises(fk.FkgpuError):
+        ch.measure_conductivity([0.0], -1.0)
     ch.measure_enable(4, 1)
+    with pytest.raises(fk.FkgpuError):
+        ch.stiffness()  # not enabled
     ch.measure()
     ch._mflags = 2  # ask for a history that was not enabled
     with pytest.raises(fk.FkgpuError):
@@ -234,5 +240,45 @@ def test_measure_errors(fk):
     ch = fk.Chains(ctx, np.zeros((1, 1100), np.int32), 1.0, 1.0, 0.5, 0.5, np.array([1], np.uint32))
     with pytest.raises(fk.FkgpuError):
         ch.measure_enable(2, 4)  # eigenvectors implemented for N <= 1024
+    ch.close()
+    ctx.close()
+    ctx, ch, _, _ = _chains(fk, 1, 12, 2, 8, 1.0, 1.0, [(fk.MOVE_ADDREMOVE, 1.0)])
+    with pytest.raises(fk.FkgpuError):
+        ch.measure_enable(2, 8)  # stiffness: hypercubic lattices of dimension >= 2
+    ch.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["dense", "rank1"])
+def test_measures_stiffness(fk, orc, mode):
+    """Stiffness and conductivity records (measure_stiffness::accumulate,
+    include/fk_mc/measures/stiffness.hpp:129-187) equal the single-configuration entry point
+    fkgpu_measure_stiffness on the recorded occupations, and the oracle on one of them; from
+    an FP64 decomposition per measurement (dense chains) or the resident eigenvectors
+    (rank-one chains).  Means are the means of the records."""
+    print(f"case chain stiffness {mode}", flush=True)
+    D, L, B = 2, 6, 4
+    wgrid = np.array([0.0, 0.3, 1.0, 2.5])
+    moves = [(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)]
+    ctx, ch, beta, U = _chains(fk, D, L, B, 9, [1.0, 3.0, 10.0, 30.0], 2.0, moves)
+    if mode == "rank1":
+        ch.set_update(fk.UPDATE_RANK1, 16)
+    ch.measure_conductivity(wgrid, 0.05)
+    ch.measure_enable(4, 2 | 8)
+    ch.sweep_measure(4, 6)
+    count, st, cd, sm, cm = ch.stiffness()
+    h = ch.histories()
+    assert count == 4 and st.shape == (B, 4) and cd.shape == (B, 4, len(wgrid))
+    for c in range(B):
+        ref_st, ref_cd = ctx.measure_stiffness(h["focc"][c], beta[c], U[c], U[c] / 2, wgrid, 0.05)
+        np.testing.assert_allclose(st[c], ref_st, rtol=1e-9, atol=1e-11)
+        np.testing.assert_allclose(cd[c], ref_cd, rtol=1e-9, atol=1e-11)
+        np.testing.assert_allclose(sm[c], st[c].mean(), rtol=1e-13)
+        np.testing.assert_allclose(cm[c], cd[c].mean(axis=0), rtol=1e-13)
+    f = h["focc"][0, -1].astype(np.int32)
+    ed = orc.calc_ed(orc.KIND_HYPERCUBIC, D, L, f, beta[0], U[0], U[0] / 2, tp=0.0, vectors=True)
+    o = orc.measure_stiffness(D, L, ed["evecs"], ed["evals"], ed["cfermi"], wgrid, 0.05)
+    np.testing.assert_allclose(st[0, -1], o["stiffness"], rtol=1e-9, atol=1e-11)
+    np.testing.assert_allclose(cd[0, -1], o["cond"], rtol=1e-9, atol=1e-11)
     ch.close()
     ctx.close()

@@ -89,6 +89,9 @@ struct chain_meas_args {
     const double* cond;            // B x nw
     double *stiff_hist, *stiff_sum;  // B x cap, B
     double *cond_hist, *cond_sum;    // B x cap x nw, B x nw
+    // eigenvectors of this measurement (null: not recorded)
+    const double* V;               // B x N x N
+    double* evec_hist;             // B x cap x N x N
 };
 hipError_t fk_launch_chain_measure(const chain_meas_args& a, long long* count, hipStream_t st);
 // fermi[b][k] = 1 / (1 + exp(beta_b ev[b][k])) (ed_cache::cached_fermi, src/configuration.cpp:238)
@@ -149,6 +152,14 @@ hipError_t fk_launch_stiffness(int B, int N, int stride0, int L0, double t, cons
                                double* mJ, double* tdiag, double* stiff, double* cond, hipStream_t st);
 hipError_t fk_launch_dos(int S, int N, const double* spec, int nw, const double* wgrid, double eta, double* out,
                          hipStream_t st);
+// Green's function / local DOS over the records of a chain set (n per chain, capacity cap):
+// part buffers hold splits x N^2 (each of re, im) or splits x nw x N doubles
+int fk_green_splits(int nrec, int tiles);
+hipError_t fk_launch_green(int n, int cap, int nchains, int N, const double* V, const double* ev, double w,
+                           double eta, int splits, double* part_re, double* part_im, double* g_re, double* g_im,
+                           hipStream_t st);
+hipError_t fk_launch_ldos(int n, int cap, int nchains, int N, const double* V, const double* ev, int nw,
+                          const double* wgrid, double eta, int splits, double* part, double* ldos, hipStream_t st);
 hipError_t fk_launch_adjoint(int B, int N, const double* V, const double* fermi, const int32_t* f,
                              const double* beta, const int32_t* nbr, const int32_t* z, int zmax, int nt_bonds,
                              double* dens, double* bond_t, double* bond_tp, double* out, double* rho_full,

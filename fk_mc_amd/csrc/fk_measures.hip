@@ -18,8 +18,8 @@
 //                    measure_spectrum (src/measures/spectrum.cpp:13-21),
 //                    measure_spectrum_history (src/measures/spectrum_history.cpp:13-19),
 //                    measure_focc (src/measures/focc_history.cpp:8-13), the IPR
-//                    history of measure_ipr and the stiffness / conductivity history
-//                    of measure_stiffness.
+//                    history of measure_ipr, the stiffness / conductivity history of
+//                    measure_stiffness and the eigenvectors of measure_eigenfunctions.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -288,6 +288,12 @@ __global__ void __launch_bounds__(CM_T) k_chain_measure(chain_meas_args a) {
         const uint8_t fi = a.f[row + k];
         a.focc_sum[row + k] += fi;
         if (rec && a.focc_hist) a.focc_hist[hrow + k] = fi;
+    }
+    if (a.V && rec) {
+        const size_t NN = (size_t)N * N;
+        const double* src = a.V + (size_t)b * NN;
+        double* dst = a.evec_hist + ((size_t)b * a.cap + (size_t)m) * NN;
+        for (size_t q = tid; q < NN; q += CM_T) dst[q] = src[q];
     }
     if (a.stiff) {
         const size_t crow = (size_t)b * a.nw;

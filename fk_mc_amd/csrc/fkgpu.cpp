@@ -61,6 +61,7 @@ struct chain_meas_bufs {
     double *stiff = nullptr, *cond = nullptr;                   // B, B x nw (this measurement)
     double *stiff_hist = nullptr, *cond_hist = nullptr;         // B x cap, B x cap x nw
     double *stiff_sum = nullptr, *cond_sum = nullptr;           // B, B x nw
+    double* evec_hist = nullptr;                                // B x cap x N x N
     // decomposition of the current configurations, for chain sets that keep no
     // eigenvalues (stochastic trace) or no eigenvectors (IPR / stiffness outside the
     // rank-one mode)
@@ -74,7 +75,7 @@ static void meas_free(chain_meas_bufs* m) {
     void* ps[] = {m->count, m->e, m->d2e, m->ce, m->esum, m->d2sum, m->spec_mean, m->focc_sum, m->spec_hist,
                   m->focc_hist, m->ipr, m->ipr_hist, m->f32, m->ev, m->lz, m->vh, m->tau, m->d, m->eo, m->V, m->scr,
                   m->wgrid, m->fermi, m->JV, m->mJ, m->tdiag, m->stiff, m->cond, m->stiff_hist, m->cond_hist,
-                  m->stiff_sum, m->cond_sum};
+                  m->stiff_sum, m->cond_sum, m->evec_hist};
     for (void* p : ps)
         if (p) hipFree(p);
     delete m;
@@ -1785,7 +1786,7 @@ static int meas_prepare(fkgpu_chains* ch) {
     if (!m) return fail(ctx, FKGPU_EINVAL, "measurements are not enabled on this chain set");
     if (!ch->kpm && !ch->dense_ok)
         return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
-    const bool ipr = m->flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS);
+    const bool ipr = m->flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS | FKGPU_MEASURE_EIGENFUNCTIONS);
     if (!ch->kpm && !(ipr && !ch->fu)) return FKGPU_OK;
     if (!ch->dense_ok) return fail(ctx, FKGPU_EUNSUPPORTED, "measurements need the dense eigensolver (N <= 2048)");
     const int B = ch->B, N = ch->N;
@@ -1812,7 +1813,7 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
     fkgpu_ctx* ctx = ch->ctx;
     chain_meas_bufs* m = ch->ms;
     const int B = ch->B, N = ch->N;
-    const bool vec = m->flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS);
+    const bool vec = m->flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS | FKGPU_MEASURE_EIGENFUNCTIONS);
     const bool ipr = vec;  // eigenvectors needed
     const double* spec = ch->spectrum;
     const double* V = (ipr && ch->fu) ? ch->fu_V : nullptr;
@@ -1877,6 +1878,8 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
     a.stiff_sum = m->stiff_sum;
     a.cond_hist = m->cond_hist;
     a.cond_sum = m->cond_sum;
+    a.V = (V && (m->flags & FKGPU_MEASURE_EIGENFUNCTIONS)) ? V : nullptr;
+    a.evec_hist = m->evec_hist;
     a.count = m->count;
     a.e = m->e;
     a.d2e = m->d2e;
@@ -1903,11 +1906,11 @@ static int meas_count(fkgpu_chains* ch, long long* count) {
 int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
-    const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | FKGPU_MEASURE_IPR |
-                    FKGPU_MEASURE_STIFFNESS;
+    const int vecs = FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS | FKGPU_MEASURE_EIGENFUNCTIONS;
+    const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | vecs;
     if (capacity < 0 || (flags & ~all)) return fail(ctx, FKGPU_EINVAL, "invalid capacity or measurement flags");
-    if ((flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS)) && (!ch->dense_ok || ch->N > 1024))
-        return fail(ctx, FKGPU_EUNSUPPORTED, "IPR and stiffness need eigenvectors, implemented for N <= 1024");
+    if ((flags & vecs) && (!ch->dense_ok || ch->N > 1024))
+        return fail(ctx, FKGPU_EUNSUPPORTED, "IPR, stiffness and eigenfunctions need eigenvectors, implemented for N <= 1024");
     if ((flags & FKGPU_MEASURE_STIFFNESS) && (ctx->lat_kind != FKGPU_LATTICE_HYPERCUBIC || ctx->lat_ndim < 2))
         return fail(ctx, FKGPU_EUNSUPPORTED, "stiffness needs a hypercubic lattice of dimension >= 2");
     hipSetDevice(ctx->device);
@@ -1937,6 +1940,7 @@ int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
         chk(dalloc(&m->ipr, BN));
         chk(dalloc(&m->ipr_hist, BcN));
     }
+    if (flags & FKGPU_MEASURE_EIGENFUNCTIONS) chk(dalloc(&m->evec_hist, BcN * N));
     const int nw = (int)ch->m_wgrid.size();
     m->nw = nw;
     m->offset = ch->m_offset;
@@ -2015,6 +2019,91 @@ int fkgpu_chains_stiffness(fkgpu_chains* ch, int64_t* count, double* stiffness, 
         FK_HIP(ctx, hipMemcpy(cond_mean, m->cond_sum, sizeof(double) * B * nw, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < (size_t)B * nw; ++i) cond_mean[i] /= z;
     }
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_eigenfunctions(fkgpu_chains* ch, int64_t* count, double* evecs) {
+    if (!ch) return FKGPU_EINVAL;
+    long long c = 0;
+    int r = meas_count(ch, &c);
+    if (r) return r;
+    fkgpu_ctx* ctx = ch->ctx;
+    chain_meas_bufs* m = ch->ms;
+    if (!(m->flags & FKGPU_MEASURE_EIGENFUNCTIONS)) return fail(ctx, FKGPU_EINVAL, "eigenfunctions were not enabled");
+    if (count) *count = c;
+    const size_t n = (size_t)std::min<long long>(c, m->cap), NN = (size_t)ch->N * ch->N;
+    if (n > 0 && evecs)
+        FK_HIP(ctx, hipMemcpy2D(evecs, sizeof(double) * n * NN, m->evec_hist, sizeof(double) * m->cap * NN,
+                                sizeof(double) * n * NN, ch->B, hipMemcpyDeviceToHost));
+    return FKGPU_OK;
+}
+
+// The records averaged by the Green's-function readers: every recorded measurement of
+// every chain (eigenvectors and spectrum history both enabled).
+static int green_records(fkgpu_chains* ch, int* n) {
+    long long c = 0;
+    int r = meas_count(ch, &c);
+    if (r) return r;
+    chain_meas_bufs* m = ch->ms;
+    const int need = FKGPU_MEASURE_EIGENFUNCTIONS | FKGPU_MEASURE_SPECTRUM_HISTORY;
+    if ((m->flags & need) != need)
+        return fail(ch->ctx, FKGPU_EINVAL, "the Green's function needs the eigenfunction and spectrum histories");
+    *n = (int)std::min<long long>(c, m->cap);
+    if (*n < 1) return fail(ch->ctx, FKGPU_EINVAL, "no recorded measurements");
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_green(fkgpu_chains* ch, double w, double offset, double* g_re, double* g_im) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    if (!(offset > 0.0) || !std::isfinite(w) || !g_re || !g_im)
+        return fail(ctx, FKGPU_EINVAL, "invalid Green's function arguments");
+    int n = 0, r = green_records(ch, &n);
+    if (r) return r;
+    chain_meas_bufs* m = ch->ms;
+    const int N = ch->N, tb = (N + 63) / 64;
+    const int splits = fk_green_splits(n * ch->B, tb * tb);
+    const size_t NN = (size_t)N * N;
+    double* buf = nullptr;
+    if (dalloc(&buf, (2 * (size_t)splits + 2) * NN) != hipSuccess)
+        return fail(ctx, FKGPU_ENOMEM, "Green's function buffers");
+    hipStream_t st = ctx->stream;
+    hipError_t e = fk_launch_green(n, m->cap, ch->B, N, m->evec_hist, m->spec_hist, w, offset, splits, buf,
+                                   buf + splits * NN, buf + 2 * splits * NN, buf + (2 * splits + 1) * NN, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(g_re, buf + 2 * splits * NN, sizeof(double) * NN, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g_im, buf + (2 * splits + 1) * NN, sizeof(double) * NN, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(buf);
+    FK_HIP(ctx, e);
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_local_dos(fkgpu_chains* ch, int nw, const double* wgrid, double offset, double* ldos) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    if (nw < 1 || !wgrid || !ldos || !(offset > 0.0)) return fail(ctx, FKGPU_EINVAL, "invalid local DOS arguments");
+    for (int i = 0; i < nw; ++i)
+        if (!std::isfinite(wgrid[i])) return fail(ctx, FKGPU_EINVAL, "invalid local DOS arguments");
+    int n = 0, r = green_records(ch, &n);
+    if (r) return r;
+    chain_meas_bufs* m = ch->ms;
+    const int N = ch->N;
+    const int splits = fk_green_splits(n * ch->B, ((N + 63) / 64) * ((nw + 63) / 64));
+    const size_t len = (size_t)nw * N;
+    double* buf = nullptr;
+    if (dalloc(&buf, ((size_t)splits + 1) * len + nw) != hipSuccess)
+        return fail(ctx, FKGPU_ENOMEM, "local DOS buffers");
+    double* wg = buf + (splits + 1) * len;
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipMemcpyAsync(wg, wgrid, sizeof(double) * nw, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = fk_launch_ldos(n, m->cap, ch->B, N, m->evec_hist, m->spec_hist, nw, wg, offset, splits, buf,
+                           buf + splits * len, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ldos, buf + splits * len, sizeof(double) * len, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(buf);
+    FK_HIP(ctx, e);
     return FKGPU_OK;
 }
 

@@ -20,6 +20,7 @@ SOLVER_AUTO, SOLVER_VARIANTS = -1, 7
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 UPDATE_DENSE, UPDATE_RANK1 = 0, 1
 MEASURE_SPECTRUM_HISTORY, MEASURE_FOCC_HISTORY, MEASURE_IPR, MEASURE_STIFFNESS = 1, 2, 4, 8
+MEASURE_EIGENFUNCTIONS = 16
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -112,6 +113,9 @@ def lib():
         L.fkgpu_chains_histories.argtypes = [P, _i64, _d, _i32, _d]
         L.fkgpu_chains_measure_conductivity.argtypes = [P, C.c_int, _d, C.c_double]
         L.fkgpu_chains_stiffness.argtypes = [P, _i64, _d, _d, _d, _d]
+        L.fkgpu_chains_eigenfunctions.argtypes = [P, _i64, _d]
+        L.fkgpu_chains_green.argtypes = [P, C.c_double, C.c_double, _d, _d]
+        L.fkgpu_chains_local_dos.argtypes = [P, C.c_int, _d, C.c_double, _d]
         _LIB = L
     return _LIB
 
@@ -474,6 +478,28 @@ class Chains:
         self.ctx._check(lib().fkgpu_chains_stiffness(self._h, _p(c, _i64), _p(st, _d), _p(cd, _d), _p(sm, _d),
                                                      _p(cm, _d)))
         return int(c[0]), st, cd, sm, cm
+
+    def eigenfunctions(self):
+        """(count, evecs [B, n, N, N])."""
+        n = min(self._mcount(), self._mcap)
+        V = np.zeros((self.B, n, self.N, self.N))
+        c = np.zeros(1, np.int64)
+        self.ctx._check(lib().fkgpu_chains_eigenfunctions(self._h, _p(c, _i64), _p(V, _d)))
+        return int(c[0]), V
+
+    def green(self, w, offset):
+        """G(w + i offset) averaged over every record of every chain: (re [N, N], im [N, N])."""
+        re = np.zeros((self.N, self.N))
+        im = np.zeros((self.N, self.N))
+        self.ctx._check(lib().fkgpu_chains_green(self._h, float(w), float(offset), _p(re, _d), _p(im, _d)))
+        return re, im
+
+    def local_dos(self, wgrid, offset):
+        """Local densities of states [len(wgrid), N] averaged over every record of every chain."""
+        w = np.ascontiguousarray(wgrid, dtype=np.float64).ravel()
+        out = np.zeros((len(w), self.N))
+        self.ctx._check(lib().fkgpu_chains_local_dos(self._h, len(w), _p(w, _d), float(offset), _p(out, _d)))
+        return out
 
     def histories(self):
         """Enabled histories, each [B, min(count, capacity), N] (None where not enabled)."""

@@ -15,15 +15,18 @@
 // reference's order with their weights, ntherm_sweeps unmeasured sweeps, then nsweeps
 // sweeps of sweep_len moves each followed by the measures (energy, spectrum, spectrum
 // and f-occupation histories when measure_history, IPR when measure_ipr, stiffness and
-// conductivity when measure_stiffness), all kept on
-// the device until the end.
+// conductivity when measure_stiffness, eigenvectors when measure_eigenfunctions), all kept
+// on the device until the end.
 //
 // Output (`output`, a directory; a trailing .h5 is dropped): the observables of
 // observables_t (include/fk_mc/fk_mc.hpp:13-28) under mc_data/ as .npy files with the
 // chain as the leading axis -- energies, d2energies, c_energies [chains][nsweeps],
 // spectrum [chains][N] (running mean), spectrum_history, focc_history, ipr_history
 // [chains][nsweeps][N], stiffness [chains][nsweeps] and cond_history [chains][nsweeps][nw]
-// with measure_stiffness on the reference's conductivity grid -- plus dos.npy (the DOS of the recorded spectra on the
+// with measure_stiffness on the reference's conductivity grid, eig_history
+// [chains][nsweeps][N][N] with save_eigenfunctions -- plus, with measure_eigenfunctions, the
+// outputs of save_gwr under stats/ (tdos_gwr, tdos<w>, gr_full_w<w>_re/_im, gr_w<w>, gk_w<w>
+// at w = 0 + i dos_offset) -- plus dos.npy (the DOS of the recorded spectra on the
 // reference's real grid, prog/data_save.hxx:67-69,264-345), parameters.txt and
 // stats.txt (means with errors from the spread over chains, and the specific heat
 // beta^2 (<E^2> - <d2E> - <E>^2) / N of prog/data_save.hxx:185-190).
@@ -103,7 +106,7 @@ params define_parameters() {
     p.define("Nf_start", "5", "Starting number of f-electrons");
     p.define("seed", "", "Seed for random number generator (default: std::random_device)");
     p.define("measure_ipr", "false", "Measure inverse participation ratio");
-    p.define("measure_eigenfunctions", "false", "Measure eigenfunctions (not in the batched program)");
+    p.define("measure_eigenfunctions", "false", "Measure eigenfunctions");
     p.define("cond_offset", "0.05", "dos offset from the real axis");
     p.define("measure_stiffness", "false", "Measure stiffness/conductivity");
     // mc_metropolis::define_parameters (src/mc_metropolis.cpp:11-19)
@@ -122,6 +125,7 @@ params define_parameters() {
     p.define("dos_npts", "240", "Number of points for DOS sampling");
     p.define("dos_offset", "0.05", "DOS offset from real axis");
     p.define("cond_npoints", "150", "number of points to sample conductivity");
+    p.define("save_eigenfunctions", "false", "Store eigenfunctions?");
     // batched program
     p.define("lattice", "cubic2d", "cubic1d | cubic2d | cubic3d | triangular");
     p.define("chains", "1", "Number of chains (chain r = MPI rank r of the reference, seeded seed + r)");
@@ -209,6 +213,88 @@ mean_err over_chains(const std::vector<double>& v) {
         if (_r != FKGPU_OK) throw std::runtime_error(std::string(#call) + ": " + fkgpu_last_error(ctx)); \
     } while (0)
 
+// std::to_string(float) with trailing zeros removed, the frequency label of save_gwr's
+// dataset names (prog/data_save.hxx:568-572)
+std::string wlabel(double x) {
+    std::string s = std::to_string((float)x);
+    return s.erase(s.find_last_not_of('0') + 1);
+}
+
+// one row of tdos_vals (prog/data_save.hxx:623-642) from the local DOS of the sites; the
+// reference leaves the error columns at zero
+std::vector<double> tdos_row(double w, double off, const double* ld, int N) {
+    double dos = 0.0, lg = 0.0;
+    for (int i = 0; i < N; ++i) dos += ld[i], lg += std::log(ld[i]);
+    dos /= N;
+    const double geom = std::exp(lg / N);
+    return {w, off, geom, 0.0, dos, 0.0, geom / dos, 0.0};
+}
+
+void save_gwr(const std::string& dir, int D, int L, int N, double off, const std::vector<double>& gre,
+              const std::vector<double>& gim, const std::vector<double>& grid, const std::vector<double>& ldos,
+              bool plaintext) {
+    mkdirs(dir);
+    const std::string ws = wlabel(0.0) + "_" + wlabel(off);
+    std::vector<double> ld0(N);
+    for (int i = 0; i < N; ++i) ld0[i] = gim[(size_t)i * N + i] / -M_PI;
+    const std::vector<double> t0 = tdos_row(0.0, off, ld0.data(), N);
+    write_npy(dir + "tdos" + ws + ".npy", t0.data(), {8});
+    write_npy(dir + "gr_full_w" + ws + "_re.npy", gre.data(), {(size_t)N, (size_t)N});
+    write_npy(dir + "gr_full_w" + ws + "_im.npy", gim.data(), {(size_t)N, (size_t)N});
+    if (plaintext) {
+        write_txt(dir + "gr_full_w" + ws + "_re.dat", gre.data(), N, N);
+        write_txt(dir + "gr_full_w" + ws + "_im.dat", gim.data(), N, N);
+    }
+    if (D >= 2) {
+        // G(r_j - r_i) over the first two lattice axes (the site index runs fastest along
+        // the last axis), then its 2-D discrete Fourier transform, forward, unnormalised
+        std::vector<double> r2re((size_t)L * L, 0.0), r2im((size_t)L * L, 0.0);
+        int per = 1;
+        for (int d = 2; d < D; ++d) per *= L;
+        auto pos01 = [&](int i, int& a, int& b) { a = i / (L * per); b = (i / per) % L; };
+        for (int i = 0; i < N; ++i) {
+            int ai, bi;
+            pos01(i, ai, bi);
+            for (int j = 0; j < N; ++j) {
+                int aj, bj;
+                pos01(j, aj, bj);
+                const size_t q = (size_t)((L + aj - ai) % L) * L + (L + bj - bi) % L;
+                r2re[q] += gre[(size_t)i * N + j] / N;
+                r2im[q] += gim[(size_t)i * N + j] / N;
+            }
+        }
+        std::vector<double> kre((size_t)L * L, 0.0), kim((size_t)L * L, 0.0);
+        for (int k0 = 0; k0 < L; ++k0)
+            for (int k1 = 0; k1 < L; ++k1) {
+                double sr = 0.0, si = 0.0;
+                for (int p0 = 0; p0 < L; ++p0)
+                    for (int p1 = 0; p1 < L; ++p1) {
+                        const double ph = -2.0 * M_PI * (double)((k0 * p0 + k1 * p1) % L) / L;
+                        const double c = std::cos(ph), s = std::sin(ph);
+                        const double xr = r2re[(size_t)p0 * L + p1], xi = r2im[(size_t)p0 * L + p1];
+                        sr += xr * c - xi * s;
+                        si += xr * s + xi * c;
+                    }
+                kre[(size_t)k0 * L + k1] = sr;
+                kim[(size_t)k0 * L + k1] = si;
+            }
+        write_npy(dir + "gr_w" + ws + "_re.npy", r2re.data(), {(size_t)L, (size_t)L});
+        write_npy(dir + "gr_w" + ws + "_im.npy", r2im.data(), {(size_t)L, (size_t)L});
+        write_npy(dir + "gk_w" + ws + "_re.npy", kre.data(), {(size_t)L, (size_t)L});
+        write_npy(dir + "gk_w" + ws + "_im.npy", kim.data(), {(size_t)L, (size_t)L});
+    }
+    const int npts = (int)grid.size();
+    std::vector<double> tv;
+    for (int q = 0; q < npts && !ldos.empty(); ++q) {
+        const std::vector<double> r = tdos_row(grid[q], off, ldos.data() + (size_t)q * N, N);
+        tv.insert(tv.end(), r.begin(), r.end());
+    }
+    if (!tv.empty()) {
+        write_npy(dir + "tdos_gwr.npy", tv.data(), {(size_t)npts, 8});
+        if (plaintext) write_txt(dir + "tdos_gwr.dat", tv.data(), npts, 8);
+    }
+}
+
 int run(params& p) {
     const std::string lat = p.str("lattice");
     int kind = FKGPU_LATTICE_HYPERCUBIC, D = 2;
@@ -217,16 +303,17 @@ int run(params& p) {
     else if (lat == "cubic3d") D = 3;
     else if (lat == "triangular") kind = FKGPU_LATTICE_TRIANGULAR;
     else throw std::runtime_error("unknown lattice " + lat);
-    if (p.b("cheb_moves") || p.b("measure_eigenfunctions"))
-        throw std::runtime_error("cheb_moves and measure_eigenfunctions are not part of the batched program "
-                                 "(use the host layer)");
+    if (p.b("cheb_moves"))
+        throw std::runtime_error("cheb_moves is not part of the batched program (use the host layer)");
     const int L = (int)p.i("L"), R = (int)p.i("chains");
     const long nsweeps = p.i("nsweeps"), sweep_len = p.i("sweep_len"), ntherm = p.i("ntherm_sweeps");
     if (R < 1 || nsweeps < 0 || sweep_len < 1 || ntherm < 0) throw std::runtime_error("invalid chains / sweeps");
     if (p.str("seed").empty()) p.set("seed", std::to_string((int)std::random_device()()));
     const long seed = p.i("seed");
     const double beta = p.d("beta"), U = p.d("U"), mu_c = p.d("mu_c"), mu_f = p.d("mu_f");
-    const bool history = p.b("measure_history"), ipr = p.b("measure_ipr");
+    const bool ipr = p.b("measure_ipr"), eig = p.b("measure_eigenfunctions");
+    // prog/fk_mc_exec.cpp:62: the IPR and the eigenfunctions switch the histories on
+    const bool history = p.b("measure_history") || ipr || eig;
 
     fkgpu_ctx* ctx = nullptr;
     if (fkgpu_create((int)p.i("device"), &ctx) != FKGPU_OK) throw std::runtime_error("no HIP device");
@@ -276,7 +363,7 @@ int run(params& p) {
         FK_CHECK(fkgpu_chains_measure_conductivity(ch, (int)wgrid.size(), wgrid.data(), p.d("cond_offset")), ctx);
     }
     int flags = (history ? FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY : 0) | (ipr ? FKGPU_MEASURE_IPR : 0) |
-                (stiff ? FKGPU_MEASURE_STIFFNESS : 0);
+                (stiff ? FKGPU_MEASURE_STIFFNESS : 0) | (eig ? FKGPU_MEASURE_EIGENFUNCTIONS : 0);
     FK_CHECK(fkgpu_chains_measure_enable(ch, (int)nsweeps, flags), ctx);
 
     const bool show = p.b("show_output");
@@ -327,6 +414,18 @@ int run(params& p) {
                 for (int i = 0; i < npts; ++i) dos[i] += d[(size_t)s * npts + i] / ns;
         }
     }
+    // data_saver::save_gwr (prog/data_save.hxx:72-79,535-708) on the recorded eigenvectors of
+    // every chain: the full G at w = 0 with the DOS and typical DOS there, and the DOS and
+    // typical DOS on the DOS grid
+    const double doff = p.d("dos_offset");
+    std::vector<double> evh(eig && p.b("save_eigenfunctions") ? R * S * N * N : 0);
+    if (!evh.empty()) FK_CHECK(fkgpu_chains_eigenfunctions(ch, &count, evh.data()), ctx);
+    const bool gwr = eig && count > 0 && S > 0;
+    std::vector<double> g0re(gwr ? (size_t)N * N : 0), g0im(g0re.size()), ldos(gwr ? (size_t)npts * N : 0);
+    if (gwr) {
+        FK_CHECK(fkgpu_chains_green(ch, 0.0, doff, g0re.data(), g0im.data()), ctx);
+        if (npts > 0) FK_CHECK(fkgpu_chains_local_dos(ch, npts, grid.data(), doff, ldos.data()), ctx);
+    }
     fkgpu_chains_destroy(ch);
     fkgpu_destroy(ctx);
 
@@ -344,6 +443,8 @@ int run(params& p) {
         write_npy(md + "focc_history.npy", fd.data(), {(size_t)R, S, (size_t)N});
     }
     if (ipr) write_npy(md + "ipr_history.npy", ih.data(), {(size_t)R, S, (size_t)N});
+    if (!evh.empty()) write_npy(md + "eig_history.npy", evh.data(), {(size_t)R, S, (size_t)N, (size_t)N});
+    if (gwr) save_gwr(out + "/stats/", D, L, N, doff, g0re, g0im, grid, ldos, p.b("plaintext"));
     if (stiff) {
         write_npy(md + "stiffness.npy", stf.data(), {(size_t)R, S});
         write_npy(md + "cond_history.npy", cnd.data(), {(size_t)R, S, NW});

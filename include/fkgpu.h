@@ -252,6 +252,8 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
  *   measure_focc::accumulate              src/measures/focc_history.cpp:8-13
  *   measure_ipr::accumulate               include/fk_mc/measures/ipr.hpp:39-56
  *   measure_stiffness::accumulate         include/fk_mc/measures/stiffness.hpp:129-187
+ *   measure_eigenfunctions::accumulate    src/measures/eigenfunctions.cpp:12-18
+ *   data_saver::save_gwr (G(w,r), typical DOS)  prog/data_save.hxx:535-708
  * The spectrum is the chain's own (the rank-one, mixed and FP64 modes keep it); chain
  * sets on the stochastic trace diagonalise their configurations at each measurement,
  * as the reference's calc_ed(false) does, and the IPR takes the eigenvectors of the
@@ -263,6 +265,7 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
 #define FKGPU_MEASURE_FOCC_HISTORY 2
 #define FKGPU_MEASURE_IPR 4
 #define FKGPU_MEASURE_STIFFNESS 8 /* hypercubic lattices, D >= 2, N <= 1024 */
+#define FKGPU_MEASURE_EIGENFUNCTIONS 16 /* N <= 1024; capacity x N^2 doubles per chain */
 /* Allocate the records and clear them (measurement count 0).  Replaces the
  * observables the reference's measures are registered on (include/fk_mc/fk_mc.hxx:97-123). */
 int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags);
@@ -293,6 +296,22 @@ int fkgpu_chains_histories(fkgpu_chains* ch, int64_t* count, double* spectrum, i
  * ([nchains], [nchains][nw]); n = min(count, capacity). */
 int fkgpu_chains_stiffness(fkgpu_chains* ch, int64_t* count, double* stiffness, double* cond, double* stiffness_mean,
                            double* cond_mean);
+/* Eigenvector records [nchains][n][N][N] (evecs[i*N + k] of a record is component i of
+ * eigenvector k, eigenvalues ascending), n = min(count, capacity).  Signs, and bases inside
+ * degenerate eigenspaces, are those of this library's solvers, not Eigen's. */
+int fkgpu_chains_eigenfunctions(fkgpu_chains* ch, int64_t* count, double* evecs);
+
+/* Lattice Green's function G(w + i offset)_{ij} = (1/n) sum_r sum_k V_r[i][k] V_r[j][k] /
+ * (w + i offset - e_rk), averaged over every recorded measurement r of every chain
+ * (n = nchains x min(count, capacity)), as gwr_re / gwr_im of data_saver::save_gwr
+ * (prog/data_save.hxx:535-621).  Needs FKGPU_MEASURE_EIGENFUNCTIONS and
+ * FKGPU_MEASURE_SPECTRUM_HISTORY; offset > 0.  g_re, g_im: N x N, row-major. */
+int fkgpu_chains_green(fkgpu_chains* ch, double w, double offset, double* g_re, double* g_im);
+
+/* Local densities of states rho_i(w) = -Im G(w + i offset)_{ii} / pi over the same records,
+ * for nw frequencies: ldos[q * N + i] (the gw_diag of prog/data_save.hxx:625; the DOS of
+ * save_gwr is their site mean and the typical DOS exp(mean_i log rho_i)). */
+int fkgpu_chains_local_dos(fkgpu_chains* ch, int nw, const double* wgrid, double offset, double* ldos);
 
 /* Diagnostic (host only, no device needed; no reference counterpart): draws
  * from the library's port of std::mt19937(seed) and the libstdc++

@@ -162,3 +162,54 @@ def test_driver_stiffness(tmp_path):
         np.testing.assert_allclose(cd[r], ref_cd, rtol=1e-9, atol=1e-11)
     ctx.close()
     assert "stiffness" in open(out / "stats.txt").read()
+
+
+@pytest.mark.gpu
+def test_driver_eigenfunctions_and_gwr(tmp_path, orc):
+    """--measure_eigenfunctions (switching the histories on, prog/fk_mc_exec.cpp:62) with
+    --save_eigenfunctions: eig_history, and save_gwr's outputs (prog/data_save.hxx:535-708)
+    -- G(0 + i dos_offset) as the mean resolvent of the recorded configurations, its
+    translation average and numpy's 2-D FFT of it, and the DOS / typical DOS rows from the
+    recorded spectra."""
+    L, U, beta, mu, R, ns, off = 4, 2.0, 3.0, 1.0, 2, 3, 0.1
+    N = L * L
+    out = _run(tmp_path, "--lattice", "cubic2d", "--L", L, "--beta", beta, "--U", U, "--mu_c", mu, "--mu_f", mu,
+               "--Nf_start", 8, "--nsweeps", ns, "--sweep_len", 4, "--ntherm_sweeps", 1, "--chains", R,
+               "--seed", 3, "--measure_history", "false", "--measure_eigenfunctions", "true",
+               "--save_eigenfunctions", "true", "--dos_offset", off, "--dos_npts", 20, "--dos_width", 4)
+    md, sd = out / "mc_data", out / "stats"
+    V = np.load(md / "eig_history.npy")
+    fh = np.load(md / "focc_history.npy")  # histories on despite --measure_history false
+    sh = np.load(md / "spectrum_history.npy")
+    assert V.shape == (R, ns, N, N)
+    hop = orc.lattice_hop(orc.KIND_HYPERCUBIC, 2, L, 1.0, 0.0)[0]
+    G = np.zeros((N, N), complex)
+    for r in range(R):
+        for m in range(ns):
+            H = hop + np.diag(-mu + U * fh[r, m])
+            np.testing.assert_allclose(V[r, m].T @ V[r, m], np.eye(N), atol=1e-10)
+            np.testing.assert_allclose(H @ V[r, m], V[r, m] * sh[r, m][None, :], atol=1e-9)
+            G += np.linalg.inv(1j * off * np.eye(N) - H)
+    G /= R * ns
+    ws = "0._0.1"
+    gre, gim = np.load(sd / f"gr_full_w{ws}_re.npy"), np.load(sd / f"gr_full_w{ws}_im.npy")
+    np.testing.assert_allclose(gre, G.real, atol=1e-10 * np.abs(G).max())
+    np.testing.assert_allclose(gim, G.imag, atol=1e-10 * np.abs(G).max())
+    r2 = np.load(sd / f"gr_w{ws}_re.npy") + 1j * np.load(sd / f"gr_w{ws}_im.npy")
+    gk = np.load(sd / f"gk_w{ws}_re.npy") + 1j * np.load(sd / f"gk_w{ws}_im.npy")
+    assert abs(r2[0, 0] - np.trace(G) / N) <= 1e-12 * np.abs(G).max()
+    assert abs(r2.sum() - G.sum() / N) <= 1e-11 * np.abs(G).sum()
+    np.testing.assert_allclose(gk, np.fft.fft2(r2), atol=1e-11 * np.abs(gk).max())
+    ld0 = -np.diag(G).imag / np.pi
+    t0 = np.load(sd / f"tdos{ws}.npy")
+    np.testing.assert_allclose(t0[[0, 1, 3, 5, 7]], [0.0, off, 0.0, 0.0, 0.0])
+    np.testing.assert_allclose(t0[4], ld0.mean(), rtol=1e-10)
+    np.testing.assert_allclose(t0[2], np.exp(np.log(ld0).mean()), rtol=1e-10)
+    tv = np.load(sd / "tdos_gwr.npy")
+    grid = -4.0 + 8.0 * np.arange(20) / 20
+    assert tv.shape == (20, 8)
+    np.testing.assert_allclose(tv[:, 0], grid, rtol=1e-15)
+    lor = (off / np.pi) / ((grid[:, None] - sh.reshape(-1)[None, :]) ** 2 + off * off)
+    np.testing.assert_allclose(tv[:, 4], lor.sum(axis=1) / (R * ns * N), rtol=1e-10)
+    assert (tv[:, 2] <= tv[:, 4] * (1 + 1e-12)).all()  # geometric mean <= arithmetic mean
+    np.testing.assert_allclose(tv[:, 6], tv[:, 2] / tv[:, 4], rtol=1e-14)

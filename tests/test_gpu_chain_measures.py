@@ -12,7 +12,12 @@ fkgpu_sweep_measure and the readers) against the oracle:
   recorded occupations, for eigenvalues separated from their neighbours (the IPR of a
   degenerate eigenspace depends on the basis chosen inside it);
 * measure_stiffness (include/fk_mc/measures/stiffness.hpp:129-187) against the
-  single-configuration entry point and the oracle.
+  single-configuration entry point and the oracle;
+* measure_eigenfunctions (src/measures/eigenfunctions.cpp:12-18): each recorded basis is
+  orthonormal, diagonalises the Hamiltonian of the recorded occupations with the recorded
+  spectrum, and spans the oracle's eigenvector for every eigenvalue separated from its
+  neighbours (signs and bases inside degenerate eigenspaces are free: the reference's
+  Eigen bits are parity unpinned).
 
 Measuring must not change the chains: a measured chain set ends bit-identical to an
 unmeasured one with the same seeds.
@@ -222,12 +227,14 @@ def test_measure_errors(fk):
     with pytest.raises(fk.FkgpuError):
         ch.measure_enable(-1, 0)
     with pytest.raises(fk.FkgpuError):
-        ch.measure_enable(4, 16)
+        ch.measure_enable(4, 32)
     with pytest.raises(fk.FkgpuError):
         ch.measure_conductivity([0.0], -1.0)
     ch.measure_enable(4, 1)
     with pytest.raises(fk.FkgpuError):
         ch.stiffness()  # not enabled
+    with pytest.raises(fk.FkgpuError):
+        ch.eigenfunctions()  # not enabled
     ch.measure()
     ch._mflags = 2  # ask for a history that was not enabled
     with pytest.raises(fk.FkgpuError):
@@ -280,5 +287,45 @@ def test_measures_stiffness(fk, orc, mode):
     o = orc.measure_stiffness(D, L, ed["evecs"], ed["evals"], ed["cfermi"], wgrid, 0.05)
     np.testing.assert_allclose(st[0, -1], o["stiffness"], rtol=1e-9, atol=1e-11)
     np.testing.assert_allclose(cd[0, -1], o["cond"], rtol=1e-9, atol=1e-11)
+    ch.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["dense", "rank1"])
+def test_measures_eigenfunctions(fk, orc, mode):
+    """Eigenvector records (measure_eigenfunctions::accumulate,
+    src/measures/eigenfunctions.cpp:12-18) from an FP64 decomposition per measurement
+    (dense chains) or the resident eigenvectors (rank-one chains)."""
+    print(f"case chain eigenfunctions {mode}", flush=True)
+    D, L, B = 2, 6, 3
+    N = L ** D
+    moves = [(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)]
+    ctx, ch, beta, U = _chains(fk, D, L, B, 11, [2.0, 5.0, 20.0], [1.0, 3.0, 6.0], moves)
+    if mode == "rank1":
+        ch.set_update(fk.UPDATE_RANK1, 16)
+    ch.measure_enable(3, 1 | 2 | 16)
+    ch.sweep_measure(3, 5)
+    count, V = ch.eigenfunctions()
+    h = ch.histories()
+    assert count == 3 and V.shape == (B, 3, N, N)
+    hop = orc.lattice_hop(orc.KIND_HYPERCUBIC, D, L, 1.0, 0.0)[0]
+    checked = 0
+    for c in range(B):
+        for m in range(3):
+            v = V[c, m]
+            ev = h["spectrum"][c, m]
+            H = hop + np.diag(-U[c] / 2 + U[c] * h["focc"][c, m].astype(float))
+            np.testing.assert_allclose(v.T @ v, np.eye(N), atol=1e-10)
+            np.testing.assert_allclose(H @ v, v * ev[None, :], atol=1e-9)
+            ed = orc.calc_ed(orc.KIND_HYPERCUBIC, D, L, h["focc"][c, m], beta[c], U[c], U[c] / 2, tp=0.0,
+                             vectors=True)
+            gap = np.diff(ed["evals"])
+            iso = np.ones(N, bool)
+            iso[:-1] &= gap > 1e-6
+            iso[1:] &= gap > 1e-6
+            ov = np.abs(np.sum(v * ed["evecs"], axis=0))
+            assert np.abs(ov[iso] - 1.0).max(initial=0.0) <= 1e-9, (c, m)
+            checked += int(iso.sum())
+    assert checked > 0
     ch.close()
     ctx.close()

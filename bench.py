@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
     ap.add_argument("--no-c4", action="store_true", help="skip the triangular adjoint workload")
     ap.add_argument("--no-c1", action="store_true", help="skip the single-chain cubic2d L=8 timing")
+    ap.add_argument("--no-green", action="store_true",
+                    help="skip the G(w,r) / local-DOS timing over recorded eigenvectors")
     ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
     ap.add_argument("--measure-sweeps", type=int, default=4,
                     help="measured sweeps of 16 moves with on-device measures (0: skip)")
@@ -172,6 +174,9 @@ def main():
     measured = None
     if args.measure_sweeps > 0:
         measured = measured_block(fk, ctx, f, beta, U, seeds, args.measure_sweeps)
+    green = None
+    if not args.no_green:
+        green = green_block(fk, ctx, f, beta, U, seeds)
     mixed = None
     if args.mixed_steps > 0:
         mixed = mixed_block(fk, ctx, f, beta, U, seeds, args.mixed_steps, plain)
@@ -214,6 +219,7 @@ def main():
         "single_chain_cubic2d_L8": c1,
         "rank1_update": rank1,
         "measured_sweeps": measured,
+        "green_records": green,
         "mixed": mixed,
         "mixed_cubic3d_L8": c3,
         "adjoint_triangular": c4,
@@ -324,6 +330,38 @@ def measured_block(fk, ctx, f, beta, U, seeds, sweeps, sweep_len=16):
                      "measure_overhead": round(res[True] / res[False] - 1.0, 4),
                      "measurements": res["count"]}
     return out
+
+
+def green_block(fk, ctx, f, beta, U, seeds, chains=256, records=4, npts=240):
+    """save_gwr's device part over recorded eigenvectors: 256 chains x 4 measurements of the
+    headline lattice (1024 records of N x N), G(0 + 0.05i) and the local DOS on the
+    reference's 240-point DOS grid.  Kernel time from HIP events; algorithmic FP64 work
+    4 n N^3 (G, real and imaginary parts) and 2 n N^2 npts (local DOS)."""
+    B = chains
+    ch = fk.Chains(ctx, f[:B], beta[:B], U[:B], U[:B] / 2, U[:B] / 2, seeds[:B])
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    ch.measure_enable(records, fk.MEASURE_SPECTRUM_HISTORY | fk.MEASURE_FOCC_HISTORY | fk.MEASURE_EIGENFUNCTIONS)
+    ch.sweep_measure(records, 1)
+    N = ch.N
+    n = B * records
+    grid = -6.0 + 12.0 * np.arange(npts) / npts
+    t = {}
+    for name, call in (("green", lambda: ch.green(0.0, 0.05)), ("local_dos", lambda: ch.local_dos(grid, 0.05))):
+        call()
+        best = None
+        for _ in range(3):
+            call()
+            ms = ch.last_ms()
+            best = ms if best is None else min(best, ms)
+        t[name] = best
+    ch.close()
+    fg = 4.0 * n * N ** 3
+    fl = 2.0 * n * N * N * npts
+    return {"records": n, "N": N, "npts": npts,
+            "green_ms": round(t["green"], 4), "green_tflops": round(fg / t["green"] * 1e-9, 3),
+            "green_frac_fp64": round(fg / t["green"] * 1e-9 / 78.6, 4),
+            "local_dos_ms": round(t["local_dos"], 4), "local_dos_tflops": round(fl / t["local_dos"] * 1e-9, 3),
+            "local_dos_frac_fp64": round(fl / t["local_dos"] * 1e-9 / 78.6, 4)}
 
 
 def mixed_block(fk, ctx, f, beta, U, seeds, steps, plain):

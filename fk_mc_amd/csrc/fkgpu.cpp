@@ -2068,14 +2068,20 @@ int fkgpu_chains_green(fkgpu_chains* ch, double w, double offset, double* g_re, 
     if (dalloc(&buf, (2 * (size_t)splits + 2) * NN) != hipSuccess)
         return fail(ctx, FKGPU_ENOMEM, "Green's function buffers");
     hipStream_t st = ctx->stream;
-    hipError_t e = fk_launch_green(n, m->cap, ch->B, N, m->evec_hist, m->spec_hist, w, offset, splits, buf,
-                                   buf + splits * NN, buf + 2 * splits * NN, buf + (2 * splits + 1) * NN, st);
+    hipError_t e = hipEventRecord(ch->ev0, st);
+    if (e == hipSuccess)
+        e = fk_launch_green(n, m->cap, ch->B, N, m->evec_hist, m->spec_hist, w, offset, splits, buf, buf + splits * NN,
+                            buf + 2 * splits * NN, buf + (2 * splits + 1) * NN, st);
+    if (e == hipSuccess) e = hipEventRecord(ch->ev1, st);
     if (e == hipSuccess) e = hipMemcpyAsync(g_re, buf + 2 * splits * NN, sizeof(double) * NN, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(g_im, buf + (2 * splits + 1) * NN, sizeof(double) * NN, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ch->ev0, ch->ev1);
     hipFree(buf);
     FK_HIP(ctx, e);
+    ch->last_ms = ms;
     return FKGPU_OK;
 }
 
@@ -2097,13 +2103,18 @@ int fkgpu_chains_local_dos(fkgpu_chains* ch, int nw, const double* wgrid, double
     double* wg = buf + (splits + 1) * len;
     hipStream_t st = ctx->stream;
     hipError_t e = hipMemcpyAsync(wg, wgrid, sizeof(double) * nw, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(ch->ev0, st);
     if (e == hipSuccess)
         e = fk_launch_ldos(n, m->cap, ch->B, N, m->evec_hist, m->spec_hist, nw, wg, offset, splits, buf,
                            buf + splits * len, st);
+    if (e == hipSuccess) e = hipEventRecord(ch->ev1, st);
     if (e == hipSuccess) e = hipMemcpyAsync(ldos, buf + splits * len, sizeof(double) * len, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ch->ev0, ch->ev1);
     hipFree(buf);
     FK_HIP(ctx, e);
+    ch->last_ms = ms;
     return FKGPU_OK;
 }
 

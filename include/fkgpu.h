@@ -235,7 +235,9 @@ int fkgpu_chains_set_queue_timeout(fkgpu_chains* ch, double seconds);
 int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace);
 /* Replay nsteps move steps from a captured HIP graph (no trace). */
 int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps);
-/* Device time of the last fkgpu_sweep/fkgpu_sweep_graph call (HIP events on the context stream). */
+/* Device time of the last fkgpu_sweep / fkgpu_sweep_graph / fkgpu_sweep_measure call, or of the
+ * kernels of the last fkgpu_chains_green / fkgpu_chains_local_dos call (HIP events on the
+ * context stream). */
 int fkgpu_chains_last_ms(const fkgpu_chains* ch, double* ms);
 /* Current state of every chain (nullable outputs): f[nchains*N], nf, logZ,
  * spectrum[nchains*N] (of the current configuration), accepted and solved counts. */

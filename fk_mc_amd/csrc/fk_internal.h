@@ -152,6 +152,8 @@ hipError_t fk_launch_stiffness(int B, int N, int stride0, int L0, double t, cons
                                double* mJ, double* tdiag, double* stiff, double* cond, hipStream_t st);
 hipError_t fk_launch_dos(int S, int N, const double* spec, int nw, const double* wgrid, double eta, double* out,
                          hipStream_t st);
+hipError_t fk_launch_ipr_w(int S, int N, const double* spec, const double* ipr, int nw, const double* wgrid,
+                           double eta, double* out, hipStream_t st);
 // Green's function / local DOS over the records of a chain set (n per chain, capacity cap):
 // part buffers hold splits x N^2 (each of re, im) or splits x nw x N doubles
 int fk_green_splits(int nrec, int tiles);

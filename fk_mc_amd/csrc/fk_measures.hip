@@ -164,6 +164,28 @@ __global__ void k_dos(int S, int N, const double* __restrict__ spec, int nw, con
     out[(size_t)s * nw + w] = acc / (double)N;
 }
 
+// ipr_f of save_ipr (prog/data_save.hxx:490-501) for spectrum s at frequency w: the
+// Lorentzian-weighted mean sum_k L_k ipr_k^4 / sum_k L_k, L_k = eta / ((w - e_k)^2 + eta^2);
+// ipr_k^4 undoes the 4th root of the lpNorm<4> the records hold.
+__global__ void k_ipr_w(int S, int N, const double* __restrict__ spec, const double* __restrict__ ipr, int nw,
+                        const double* __restrict__ wgrid, double eta, double* __restrict__ out) {
+    const int s = blockIdx.y;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || w >= nw) return;
+    const double om = wgrid[w];
+    const double* e = spec + (size_t)s * N;
+    const double* r = ipr + (size_t)s * N;
+    double nom = 0.0, den = 0.0;
+    for (int k = 0; k < N; ++k) {
+        const double x = om - e[k];
+        const double l = eta / (x * x + eta * eta);
+        const double r2 = r[k] * r[k];
+        den += l;
+        nom += l * (r2 * r2);
+    }
+    out[(size_t)s * nw + w] = nom / den;
+}
+
 // One thread per site i: rho_ii and the bond sums of row i.  Per-matrix
 // reductions of the per-site values are done by k_adjoint_reduce.
 __global__ void k_adjoint_sites(int B, int N, const double* __restrict__ V, const double* __restrict__ fermi,
@@ -370,10 +392,29 @@ hipError_t fk_launch_stiffness(int B, int N, int stride0, int L0, double t, cons
     return hipGetLastError();
 }
 
+// spectra in chunks of at most 65535 (the grid's y extent)
 hipError_t fk_launch_dos(int S, int N, const double* spec, int nw, const double* wgrid, double eta, double* out,
                          hipStream_t st) {
-    hipLaunchKernelGGL(k_dos, dim3((nw + 255) / 256, S), dim3(256), 0, st, S, N, spec, nw, wgrid, eta, out);
-    return hipGetLastError();
+    for (int s0 = 0; s0 < S; s0 += 65535) {
+        const int n = S - s0 < 65535 ? S - s0 : 65535;
+        hipLaunchKernelGGL(k_dos, dim3((nw + 255) / 256, n), dim3(256), 0, st, n, N, spec + (size_t)s0 * N, nw, wgrid,
+                           eta, out + (size_t)s0 * nw);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t fk_launch_ipr_w(int S, int N, const double* spec, const double* ipr, int nw, const double* wgrid,
+                           double eta, double* out, hipStream_t st) {
+    for (int s0 = 0; s0 < S; s0 += 65535) {
+        const int n = S - s0 < 65535 ? S - s0 : 65535;
+        hipLaunchKernelGGL(k_ipr_w, dim3((nw + 255) / 256, n), dim3(256), 0, st, n, N, spec + (size_t)s0 * N,
+                           ipr + (size_t)s0 * N, nw, wgrid, eta, out + (size_t)s0 * nw);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t fk_launch_adjoint(int B, int N, const double* V, const double* fermi, const int32_t* f,

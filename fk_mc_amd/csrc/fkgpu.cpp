@@ -862,6 +862,30 @@ int fkgpu_dos(fkgpu_ctx* ctx, int nspectra, const double* spectra, int nw, const
     return FKGPU_OK;
 }
 
+int fkgpu_ipr_w(fkgpu_ctx* ctx, int nspectra, const double* spectra, const double* ipr, int nw, const double* wgrid,
+                double eta, double* out) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
+    if (nspectra <= 0 || nw <= 0 || !spectra || !ipr || !wgrid || !out || !(eta > 0.0))
+        return fail(ctx, FKGPU_EINVAL, "invalid IPR(w) arguments");
+    hipSetDevice(ctx->device);
+    const int N = ctx->N;
+    const size_t SN = (size_t)nspectra * N;
+    double *sp, *wg, *o, *ip;
+    int r;
+    if ((r = ensure(ctx, ctx->b_m1, SN, &sp)) || (r = ensure(ctx, ctx->b_m3, SN, &ip)) ||
+        (r = ensure(ctx, ctx->b_w, nw, &wg)) || (r = ensure(ctx, ctx->b_m2, (size_t)nspectra * nw, &o)))
+        return r;
+    hipStream_t st = ctx->stream;
+    FK_HIP(ctx, hipMemcpyAsync(sp, spectra, sizeof(double) * SN, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(ip, ipr, sizeof(double) * SN, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, hipMemcpyAsync(wg, wgrid, sizeof(double) * nw, hipMemcpyHostToDevice, st));
+    FK_HIP(ctx, fk_launch_ipr_w(nspectra, N, sp, ip, nw, wg, eta, o, st));
+    FK_HIP(ctx, hipMemcpyAsync(out, o, sizeof(double) * nspectra * nw, hipMemcpyDeviceToHost, st));
+    FK_HIP(ctx, hipStreamSynchronize(st));
+    return FKGPU_OK;
+}
+
 int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* beta, const double* U,
                   const double* mu_c, double* logZ, double* dens, double* grads, double* rho) {
     int r = check_params(ctx, batch, f, beta, U, mu_c);

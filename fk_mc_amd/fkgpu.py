@@ -91,6 +91,7 @@ def lib():
         L.fkgpu_measure_ipr.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d]
         L.fkgpu_measure_stiffness.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, _d, C.c_double, _d, _d]
         L.fkgpu_dos.argtypes = [P, C.c_int, _d, C.c_int, _d, C.c_double, _d]
+        L.fkgpu_ipr_w.argtypes = [P, C.c_int, _d, _d, C.c_int, _d, C.c_double, _d]
         L.fkgpu_adjoint.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _d, _d]
         L.fkgpu_cheb_logz.argtypes = [P, _i32, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _d, _d, _d]
         L.fkgpu_eig_logz_mixed.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _d, _i64]
@@ -240,6 +241,18 @@ def _dos(self, spectra, wgrid, eta=0.05):
     return out
 
 
+def _ipr_w(self, spectra, ipr, wgrid, eta=0.05):
+    sp = np.ascontiguousarray(np.atleast_2d(spectra), dtype=np.float64)
+    ip = np.ascontiguousarray(np.atleast_2d(ipr), dtype=np.float64)
+    if ip.shape != sp.shape:
+        raise ValueError("spectra and ipr shapes differ")
+    wg = np.ascontiguousarray(wgrid, dtype=np.float64)
+    out = np.zeros((sp.shape[0], len(wg)))
+    self._check(lib().fkgpu_ipr_w(self._h, sp.shape[0], _p(sp, _d), _p(ip, _d), len(wg), _p(wg, _d), eta,
+                                  _p(out, _d)))
+    return out
+
+
 def _adjoint(self, f, beta, U, mu_c, full=False):
     f, B, b, u, m = _batch(f, beta, U, mu_c)
     lz = np.zeros(B)
@@ -327,6 +340,7 @@ Context.kpm_logz = _kpm_logz
 Context.measure_ipr = _measure_ipr
 Context.measure_stiffness = _measure_stiffness
 Context.dos = _dos
+Context.ipr_w = _ipr_w
 Context.adjoint = _adjoint
 
 

@@ -6,6 +6,7 @@
 #include <memory>
 #include <string>
 
+#include "binning.hpp"
 #include "fkmc/mc_metropolis.hpp"
 #include "fkmc/measures.hpp"
 #include "fkmc/moves.hpp"
@@ -212,6 +213,42 @@ int fkmc_measure_config(const fkmc_model* m, const int32_t* f, double* energy, d
         }
         return 0;
     });
+}
+
+int fkmc_binning(int n, const double* x, int depth, double* table) {
+    if (n < 1 || !x || depth < 0 || depth > 30 || !table) {
+        g_err = "invalid binning arguments";
+        return FKGPU_EINVAL;
+    }
+    namespace b = fkmc::binning;
+    const auto st = b::accumulate(std::vector<double>(x, x + n), depth);
+    const auto cl = b::cor_length(st);
+    for (size_t i = 0; i < st.size(); ++i) {
+        double* r = table + 5 * i;
+        r[0] = st[i].size, r[1] = st[i].mean, r[2] = st[i].disp, r[3] = st[i].sqerror, r[4] = cl[i];
+    }
+    return b::estimate_bin(st);
+}
+
+int fkmc_max_bin(int nmeasures) { return nmeasures < 1 ? FKGPU_EINVAL : fkmc::binning::max_bin((size_t)nmeasures); }
+
+int fkmc_jackknife_cv(int n, const double* e, const double* d2e, double beta, int volume, int depth, double* table) {
+    if (n < 2 || !e || !d2e || volume < 1 || depth < 0 || depth > 30 || !table) {
+        g_err = "invalid jackknife arguments";
+        return FKGPU_EINVAL;
+    }
+    namespace b = fkmc::binning;
+    std::vector<double> E(e, e + n), E2(n), D(d2e, d2e + n);
+    for (int i = 0; i < n; ++i) E2[i] = e[i] * e[i];
+    const auto F = [beta, volume](const std::vector<double>& m) {
+        return beta * beta * (m[1] - m[2] - m[0] * m[0]) / volume;
+    };
+    const auto st = b::accumulate_jackknife(F, {E, E2, D}, depth);
+    for (size_t i = 0; i < st.size(); ++i) {
+        double* r = table + 4 * i;
+        r[0] = st[i].size, r[1] = st[i].mean, r[2] = st[i].disp, r[3] = st[i].sqerror;
+    }
+    return b::estimate_bin(st);
 }
 
 }  // extern "C"

@@ -46,6 +46,7 @@
 #include <string>
 #include <vector>
 
+#include "binning.hpp"
 #include "fkgpu.h"
 
 namespace {
@@ -404,14 +405,27 @@ int run(params& p) {
     const double wd = p.d("dos_width");
     std::vector<double> grid(npts), dos(npts, 0.0);
     for (int i = 0; i < npts; ++i) grid[i] = -wd + 2.0 * wd * i / (1.0 * npts);
+    // per-record DOS (and IPR(w) with measure_ipr) on the grid and at w = 0 (last column), for
+    // the error bars of save_glocal and save_ipr (prog/data_save.hxx:283-328,486-532)
+    const int nrec = history ? (int)std::min<int64_t>(count, (int64_t)S) * R : 0;
+    std::vector<double> grid0(grid);
+    grid0.push_back(0.0);
+    std::vector<double> drec, irec;
     {
-        const std::vector<double>& src = history && S > 0 ? sh : spec;
+        const std::vector<double>& src = nrec > 0 ? sh : spec;
         const int ns = (int)(src.size() / N);
-        std::vector<double> d((size_t)ns * npts);
-        if (ns > 0 && npts > 0) {
-            FK_CHECK(fkgpu_dos(ctx, ns, src.data(), npts, grid.data(), p.d("dos_offset"), d.data()), ctx);
+        std::vector<double> d((size_t)ns * (npts + 1));
+        if (ns > 0) {
+            FK_CHECK(fkgpu_dos(ctx, ns, src.data(), npts + 1, grid0.data(), p.d("dos_offset"), d.data()), ctx);
             for (int s = 0; s < ns; ++s)
-                for (int i = 0; i < npts; ++i) dos[i] += d[(size_t)s * npts + i] / ns;
+                for (int i = 0; i < npts; ++i) dos[i] += d[(size_t)s * (npts + 1) + i] / ns;
+        }
+        if (nrec > 0) drec.swap(d);
+        if (nrec > 0 && ipr) {
+            irec.resize((size_t)nrec * (npts + 1));
+            FK_CHECK(fkgpu_ipr_w(ctx, nrec, sh.data(), ih.data(), npts + 1, grid0.data(), p.d("dos_offset"),
+                                 irec.data()),
+                     ctx);
         }
     }
     // data_saver::save_gwr (prog/data_save.hxx:72-79,535-708) on the recorded eigenvectors of
@@ -494,6 +508,58 @@ int run(params& p) {
     line("nf/N", nfm);
     line("acceptance", accr);
     if (stiff) line("stiffness", stfm);
+    // the reference's error analysis (prog/data_save.hxx:158-345,486-532) on the series of all
+    // chains in chain order (its merge of the ranks' histories), binned from the last
+    // measurement backwards: energies and the jackknifed specific heat, dos0 / dos_err and,
+    // with measure_ipr, ipr0 / ipr_err
+    if (S > 0 && count > 0) {
+        namespace bn = fkmc::binning;
+        mkdirs(out + "/binning");
+        mkdirs(out + "/stats");
+        const size_t nm = (size_t)R * std::min<int64_t>(count, (int64_t)S);
+        const int maxb = bn::max_bin(nm);
+        auto series = [&](const double* x, size_t stride, size_t n) {  // reversed
+            std::vector<double> v(n);
+            for (size_t q = 0; q < n; ++q) v[q] = x[(n - 1 - q) * stride];
+            return v;
+        };
+        auto save = [&](const std::string& name, const std::vector<bn::stats>& sv) {
+            const std::vector<double> cl = bn::cor_length(sv);
+            std::vector<double> t;
+            for (size_t i = 0; i < sv.size(); ++i) t.insert(t.end(), {sv[i].size, sv[i].mean, sv[i].disp, sv[i].sqerror, cl[i]});
+            write_npy(out + "/binning/" + name + ".npy", t.data(), {sv.size(), 5});
+            const int b = bn::estimate_bin(sv);
+            const double row[4] = {sv[b].size, sv[b].mean, sv[b].disp, sv[b].sqerror};
+            write_npy(out + "/stats/" + name + ".npy", row, {4});
+            st << name << " = " << sv[b].mean << " +- " << sv[b].sqerror << " (" << sv[b].size << " samples, bin "
+               << b << ")\n";
+            return b;
+        };
+        const std::vector<double> E = series(e.data(), 1, nm), D2 = series(d2.data(), 1, nm),
+                                  CE = series(ce.data(), 1, nm);
+        save("energy", bn::accumulate(E, maxb));
+        save("d2energy", bn::accumulate(D2, maxb));
+        save("c_energy", bn::accumulate(CE, maxb));
+        std::vector<double> E2(nm);
+        for (size_t q = 0; q < nm; ++q) E2[q] = E[q] * E[q];
+        const int vol = N;
+        save("cv", bn::accumulate_jackknife(
+                       [beta, vol](const std::vector<double>& m) { return beta * beta * (m[1] - m[2] - m[0] * m[0]) / vol; },
+                       {E, E2, D2}, maxb));
+        auto err_table = [&](const std::string& name, const std::vector<double>& rec) {
+            const size_t W = npts + 1;
+            const int b = save(name + "0", bn::accumulate(series(rec.data() + npts, W, nm), maxb));
+            std::vector<double> t;
+            for (int i = 0; i < npts; ++i) {
+                const bn::stats x = bn::bin(series(rec.data() + i, W, nm), b);
+                t.insert(t.end(), {grid[i], x.mean, x.sqerror});
+            }
+            write_npy(out + "/stats/" + name + "_err.npy", t.data(), {(size_t)npts, 3});
+            if (p.b("plaintext")) write_txt(out + "/stats/" + name + "_err.dat", t.data(), npts, 3);
+        };
+        if (!drec.empty() && nm * (npts + 1) == drec.size()) err_table("dos", drec);
+        if (!irec.empty()) err_table("ipr", irec);
+    }
     st << "seconds = " << secs << "\nmoves_per_second = " << (double)R * (ntherm + nsweeps) * sweep_len / secs
        << "\n";
     if (show) std::printf("  %.3f s, %.4g moves/s, output in %s/\n", secs,

@@ -43,6 +43,9 @@ def lib():
         L.fkmc_run_mc.argtypes = [P, C.c_uint32, _i32, C.c_int, _i32, _d, C.c_int, C.c_int, C.c_int, _d, _d, _d,
                                   _d, _d, _d]
         L.fkmc_measure_config.argtypes = [P, _i32, _d, _d, _d, _d, C.c_int, _d, C.c_double, _d, _d, C.c_double, _d]
+        L.fkmc_binning.argtypes = [C.c_int, _d, C.c_int, _d]
+        L.fkmc_max_bin.argtypes = [C.c_int]
+        L.fkmc_jackknife_cv.argtypes = [C.c_int, _d, _d, C.c_double, C.c_int, C.c_int, _d]
         _LIB = L
     return _LIB
 
@@ -113,3 +116,26 @@ def measure_config(m, f, wgrid=(0.0,), offset=0.05, eta=0.05, evecs=True, stiffn
                                      _p(out["ipr"], _d), _p(out["evecs"], _d), len(wg), _p(wg, _d), offset,
                                      _p(out["stiffness"], _d), _p(out["cond"], _d), eta, _p(out["dos"], _d)))
     return out
+
+
+def binning(x, depth):
+    """Binning table [depth + 1, 5] (size, mean, variance, error, correlation length) of the
+    series x in the given order, and the level estimate_bin picks."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    t = np.zeros((depth + 1, 5))
+    b = _check(lib().fkmc_binning(len(x), _p(x, _d), int(depth), _p(t, _d)))
+    return t, b
+
+
+def max_bin(nmeasures):
+    return _check(lib().fkmc_max_bin(int(nmeasures)))
+
+
+def jackknife_cv(e, d2e, beta, volume, depth):
+    """Jackknife table [depth + 1, 4] (size, value, variance, error) of the specific heat and
+    the level estimate_bin picks."""
+    e = np.ascontiguousarray(e, dtype=np.float64).ravel()
+    d = np.ascontiguousarray(d2e, dtype=np.float64).ravel()
+    t = np.zeros((depth + 1, 4))
+    b = _check(lib().fkmc_jackknife_cv(len(e), _p(e, _d), _p(d, _d), float(beta), int(volume), int(depth), _p(t, _d)))
+    return t, b

@@ -120,6 +120,11 @@ int fkgpu_measure_stiffness(fkgpu_ctx* ctx, int batch, const int32_t* f, const d
  * (1/N) sum_k (eta/pi) / ((wgrid[w] - spectra[s*N + k])^2 + eta^2); N = fkgpu_msize. */
 int fkgpu_dos(fkgpu_ctx* ctx, int nspectra, const double* spectra, int nw, const double* wgrid, double eta,
               double* dos);
+/* ipr_f of data_saver::save_ipr (prog/data_save.hxx:490-501) per spectrum and frequency:
+ * out[s*nw + w] = sum_k L_k ipr[s*N + k]^4 / sum_k L_k, L_k = eta / ((wgrid[w] - e_k)^2 + eta^2),
+ * with ipr as the IPR measure records it (the 4th root of sum_i v_i^4). */
+int fkgpu_ipr_w(fkgpu_ctx* ctx, int nspectra, const double* spectra, const double* ipr, int nw, const double* wgrid,
+                double eta, double* out);
 /* Adjoint of Omega = -logZ/beta (no reference counterpart; the closest reference data are the
  * eigenvector densities sum_k f_k V_ik^2 from ed_cache::cached_evecs / cached_fermi,
  * include/fk_mc/configuration.hpp:28-31).  dOmega/dH = rho = V f(Lambda) V^T.

@@ -64,6 +64,19 @@ int fkmc_measure_config(const fkmc_model* m, const int32_t* f, double* energy, d
                         double* evecs, int nw, const double* wgrid, double offset, double* stiffness, double* cond,
                         double eta, double* dos);
 
+/* Binning analysis of a series, in the order given (the reference bins its histories from
+ * the last measurement backwards): table[(depth + 1) x 5] = {size, mean, variance, error,
+ * correlation length} per level (include/fk_mc/binning.hpp:21-171).  Returns the level
+ * estimate_bin picks (prog/data_save.hpp:107-122). */
+int fkmc_binning(int n, const double* x, int depth, double* table);
+/* max_bin_ of data_saver for nmeasures measurements (prog/data_save.hpp:90). */
+int fkmc_max_bin(int nmeasures);
+/* Jackknife of the specific heat beta^2 (<E^2> - <d2E> - <E>^2) / volume
+ * (prog/data_save.hxx:183-198, include/fk_mc/jackknife.hpp:40-81) on binning levels
+ * 0..depth: table[(depth + 1) x 4] = {size, value, variance, error}.  Returns the level
+ * estimate_bin picks. */
+int fkmc_jackknife_cv(int n, const double* e, const double* d2e, double beta, int volume, int depth, double* table);
+
 #ifdef __cplusplus
 }
 #endif

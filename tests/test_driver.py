@@ -213,3 +213,48 @@ def test_driver_eigenfunctions_and_gwr(tmp_path, orc):
     np.testing.assert_allclose(tv[:, 4], lor.sum(axis=1) / (R * ns * N), rtol=1e-10)
     assert (tv[:, 2] <= tv[:, 4] * (1 + 1e-12)).all()  # geometric mean <= arithmetic mean
     np.testing.assert_allclose(tv[:, 6], tv[:, 2] / tv[:, 4], rtol=1e-14)
+
+
+@pytest.mark.gpu
+def test_driver_error_analysis(tmp_path):
+    """binning/ and stats/ as data_saver writes them (prog/data_save.hxx:158-345,486-532):
+    the series of all chains in chain order, binned from the last measurement backwards,
+    each equal to the host library's binning / jackknife of the recorded series; dos_err and
+    ipr_err on the DOS grid at the bin that dos0 / ipr0 pick."""
+    from fk_mc_amd import host_api as host
+    L, R, ns, beta, U, off = 4, 3, 40, 4.0, 2.0, 0.05
+    N = L * L
+    out = _run(tmp_path, "--lattice", "cubic2d", "--L", L, "--beta", beta, "--U", U, "--mu_c", 1, "--mu_f", 1,
+               "--Nf_start", 8, "--nsweeps", ns, "--sweep_len", 2, "--ntherm_sweeps", 1, "--chains", R,
+               "--seed", 9, "--measure_ipr", "true", "--dos_npts", 16, "--dos_width", 4, "--dos_offset", off)
+    md, sd, bd = out / "mc_data", out / "stats", out / "binning"
+    n = R * ns
+    depth = host.max_bin(n)
+    rev = lambda a: np.ascontiguousarray(a.reshape(-1)[::-1])  # noqa: E731
+    for name in ("energies", "d2energies", "c_energies"):
+        t, b = host.binning(rev(np.load(md / f"{name}.npy")), depth)
+        key = name[:-3] + "y" if name != "energies" else "energy"
+        np.testing.assert_array_equal(np.load(bd / f"{key}.npy"), t)
+        np.testing.assert_array_equal(np.load(sd / f"{key}.npy"), t[b, :4])
+    e, d2 = rev(np.load(md / "energies.npy")), rev(np.load(md / "d2energies.npy"))
+    t, b = host.jackknife_cv(e, d2, beta, N, depth)
+    np.testing.assert_allclose(np.load(sd / "cv.npy"), t[b], rtol=1e-12)
+    sh = np.load(md / "spectrum_history.npy").reshape(n, N)[::-1]
+    ih = np.load(md / "ipr_history.npy").reshape(n, N)[::-1]
+    grid = -4.0 + 8.0 * np.arange(16) / 16
+    lor = off / ((grid[:, None, None] - sh[None]) ** 2 + off * off)  # [w][record][k]
+    dos_rec = lor.sum(axis=2) / (np.pi * N)
+    ipr_rec = (lor * ih[None] ** 4).sum(axis=2) / lor.sum(axis=2)
+    for name, rec in (("dos", dos_rec), ("ipr", ipr_rec)):
+        s0 = np.load(sd / f"{name}0.npy")
+        tab = np.load(sd / f"{name}_err.npy")
+        assert tab.shape == (16, 3)
+        np.testing.assert_allclose(tab[:, 0], grid, rtol=1e-15)
+        bt = np.load(bd / f"{name}0.npy")
+        b = int(np.flatnonzero((bt[:, :4] == s0).all(axis=1))[0])  # the level estimate_bin chose
+        used = int(bt[b, 0]) << b  # the records that level covers
+        np.testing.assert_allclose(tab[:, 1], rec[:, :used].mean(axis=1), rtol=1e-10)
+        for q in (0, 7, 15):
+            tq, _ = host.binning(rec[q], b)
+            np.testing.assert_allclose(tab[q, 2], tq[b, 3], rtol=1e-9)
+    assert "cv = " in open(out / "stats.txt").read()

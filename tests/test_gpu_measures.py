@@ -105,6 +105,40 @@ def test_dos(fk):
     ctx.close()
 
 
+def test_ipr_w(fk):
+    """ipr_f of save_ipr (prog/data_save.hxx:490-501): a Lorentzian-weighted mean of ipr^4,
+    so constant for constant IPR, linear in ipr^4, tending to ipr_k^4 at an isolated
+    eigenvalue as eta -> 0, and its weights are those of the DOS."""
+    print("case ipr_w", flush=True)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, 4, 1.0, 0.0)
+    rng = np.random.default_rng(12)
+    S, N = 70000, 16  # more spectra than one grid extent of 65535
+    spec = np.sort(rng.uniform(-4, 4, (S, N)), axis=1)
+    w = np.array([-3.0, -0.5, 0.0, 1.7])
+    out = ctx.ipr_w(spec, np.full((S, N), 0.5), w, 0.05)
+    np.testing.assert_allclose(out, 0.0625, rtol=1e-13)
+    a4 = rng.uniform(0.05, 1.0, (S, N))
+    b4 = rng.uniform(0.05, 1.0, (S, N))
+    oa = ctx.ipr_w(spec[:50], a4[:50] ** 0.25, w, 0.1)
+    ob = ctx.ipr_w(spec[:50], b4[:50] ** 0.25, w, 0.1)
+    oab = ctx.ipr_w(spec[:50], (2 * a4[:50] + 3 * b4[:50]) ** 0.25, w, 0.1)
+    np.testing.assert_allclose(oab, 2 * oa + 3 * ob, rtol=1e-12)
+    # at an eigenvalue with a tiny width the weight concentrates on it
+    s0 = np.linspace(-3, 3, N)
+    r0 = rng.uniform(0.2, 0.9, N)
+    o = ctx.ipr_w(s0, r0, s0, 1e-7)[0]
+    np.testing.assert_allclose(o, r0 ** 4, rtol=1e-6)
+    # the denominator is pi N DOS(w): with ipr^4 = 1 at one level only, out = L_k / (pi N dos)
+    r1 = np.zeros(N)
+    r1[5] = 1.0
+    o = ctx.ipr_w(s0, r1, w, 0.2)[0]
+    dos = ctx.dos(s0, w, 0.2)[0]
+    np.testing.assert_allclose(o, 0.2 / ((w - s0[5]) ** 2 + 0.04) / (np.pi * N * dos), rtol=1e-12)
+    with pytest.raises(fk.FkgpuError):
+        ctx.ipr_w(s0, r0, w, 0.0)
+    ctx.close()
+
+
 @pytest.mark.parametrize("kind,D,L", [(0, 2, 8), (1, 2, 6), (0, 3, 4)])
 def test_adjoint_vs_finite_differences(fk, orc, kind, D, L):
     print(f"case adjoint kind={kind} L={L}", flush=True)

@@ -29,7 +29,9 @@
 // at w = 0 + i dos_offset) -- plus dos.npy (the DOS of the recorded spectra on the
 // reference's real grid, prog/data_save.hxx:67-69,264-345), parameters.txt and
 // stats.txt (means with errors from the spread over chains, and the specific heat
-// beta^2 (<E^2> - <d2E> - <E>^2) / N of prog/data_save.hxx:185-190).
+// beta^2 (<E^2> - <d2E> - <E>^2) / N of prog/data_save.hxx:185-190), and the reference's
+// binning / jackknife analysis under binning/ and stats/ (energies, cv, dos0 / dos_err and,
+// with measure_ipr, ipr0 / ipr_err).
 #include <sys/stat.h>
 
 #include <algorithm>

@@ -119,3 +119,25 @@ def test_green_errors(fk):
     assert np.isfinite(re).all() and np.isfinite(im).all()
     ch.close()
     ctx.close()
+
+
+def test_green_headline_size(fk):
+    """At the headline lattice (cubic2d L=16, N=256: 4 x 4 output tiles, records split over the
+    grid) through size-independent identities: G is symmetric, -Im G_ii / pi is the local DOS,
+    and the site mean of the local DOS is the DOS of the recorded spectra."""
+    print("case green headline size L=16", flush=True)
+    ctx, ch, U = _run(fk, 2, 16, 32, 2, 21)
+    N = 256
+    h = ch.histories()
+    w = np.array([-2.0, 0.0, 0.35, 3.1])
+    ld = ch.local_dos(w, 0.05)
+    dos = ctx.dos(h["spectrum"].reshape(-1, N), w, 0.05).mean(axis=0)
+    np.testing.assert_allclose(ld.mean(axis=1), dos, rtol=1e-10, atol=1e-13)
+    assert (ld > 0).all()
+    re, im = ch.green(0.35, 0.05)
+    scale = np.abs(re).max() + np.abs(im).max()
+    assert np.abs(re - re.T).max() <= 1e-12 * scale
+    assert np.abs(im - im.T).max() <= 1e-12 * scale
+    np.testing.assert_allclose(-np.diag(im) / np.pi, ld[2], rtol=1e-10, atol=1e-13)
+    ch.close()
+    ctx.close()

@@ -263,8 +263,9 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
  *   data_saver::save_gwr (G(w,r), typical DOS)  prog/data_save.hxx:535-708
  * The spectrum is the chain's own (the rank-one, mixed and FP64 modes keep it); chain
  * sets on the stochastic trace diagonalise their configurations at each measurement,
- * as the reference's calc_ed(false) does, and the IPR takes the eigenvectors of the
- * rank-one mode or of an FP64 decomposition (N <= 1024).  A measurement changes no
+ * as the reference's calc_ed(false) does, and the IPR, stiffness and eigenfunction
+ * records take the eigenvectors of the rank-one mode or of an FP64 decomposition
+ * (N <= 1024).  A measurement changes no
  * chain state: the chains draw and decide exactly as without it.
  * The records of the first `capacity` measurements are kept (series and histories,
  * chain-major); later ones enter the means and the count only. */

@@ -258,3 +258,36 @@ def test_driver_error_analysis(tmp_path):
             tq, _ = host.binning(rec[q], b)
             np.testing.assert_allclose(tab[q, 2], tq[b, 3], rtol=1e-9)
     assert "cv = " in open(out / "stats.txt").read()
+
+
+@pytest.mark.gpu
+def test_driver_gwr_cubic3d(tmp_path, orc):
+    """save_gwr on a 3-D lattice: G(r_j - r_i) is averaged over the first two lattice axes
+    (the site index runs fastest along the last axis), so gr_w holds sums of G over site
+    pairs with the same (dx, dy) offset; its 2-D transform is numpy's fft2 of it."""
+    L, U, R, ns, off = 4, 3.0, 2, 2, 0.1
+    N = L ** 3
+    out = _run(tmp_path, "--lattice", "cubic3d", "--L", L, "--beta", 2, "--U", U, "--mu_c", U / 2, "--mu_f",
+               U / 2, "--Nf_start", 20, "--nsweeps", ns, "--sweep_len", 2, "--ntherm_sweeps", 1, "--chains", R,
+               "--seed", 4, "--measure_eigenfunctions", "true", "--dos_offset", off, "--dos_npts", 8)
+    md, sd = out / "mc_data", out / "stats"
+    fh = np.load(md / "focc_history.npy")
+    hop = orc.lattice_hop(orc.KIND_HYPERCUBIC, 3, L, 1.0, 0.0)[0]
+    G = np.zeros((N, N), complex)
+    for r in range(R):
+        for m in range(ns):
+            G += np.linalg.inv(1j * off * np.eye(N) - (hop + np.diag(-U / 2 + U * fh[r, m])))
+    G /= R * ns
+    ws = "0._0.1"
+    g = np.load(sd / f"gr_full_w{ws}_re.npy") + 1j * np.load(sd / f"gr_full_w{ws}_im.npy")
+    np.testing.assert_allclose(g, G, atol=1e-10 * np.abs(G).max())
+    r2 = np.load(sd / f"gr_w{ws}_re.npy") + 1j * np.load(sd / f"gr_w{ws}_im.npy")
+    assert r2.shape == (L, L)
+    idx = np.arange(N)
+    x, y = idx // (L * L), (idx // L) % L
+    want = np.zeros((L, L), complex)
+    np.add.at(want, ((x[None, :] - x[:, None]) % L, (y[None, :] - y[:, None]) % L), G / N)
+    np.testing.assert_allclose(r2, want, atol=1e-12 * np.abs(want).max())
+    gk = np.load(sd / f"gk_w{ws}_re.npy") + 1j * np.load(sd / f"gk_w{ws}_im.npy")
+    np.testing.assert_allclose(gk, np.fft.fft2(r2), atol=1e-11 * np.abs(gk).max())
+    assert not (md / "eig_history.npy").exists()  # only with save_eigenfunctions

@@ -1282,6 +1282,12 @@ int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, c
         sum += weights[m];
     }
     if (!(sum > 0.0)) return fail(ctx, FKGPU_EINVAL, "move weights sum to zero");
+    // The exchange draws a neighbour slot uniformly from the site's z neighbours, so
+    // every site needs one (a CSR lattice may hold isolated sites).
+    for (int m = 0; m < nmoves; ++m)
+        if (kinds[m] == FKGPU_MOVE_HOP && weights[m] > 0.0)
+            for (int i = 0; i < ctx->N; ++i)
+                if (ctx->z[i] == 0) return fail(ctx, FKGPU_EINVAL, "nearest-neighbour exchange on a lattice with an isolated site");
     if (ch->fu)
         for (int m = 0; m < nmoves; ++m)
             if (kinds[m] == FKGPU_MOVE_RANDOMIZE)

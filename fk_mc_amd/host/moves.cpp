@@ -2,6 +2,8 @@
 // src/moves.cpp so that a chain consumes the generator exactly as the reference.
 #include "fkmc/moves.hpp"
 
+#include <stdexcept>
+
 namespace fk {
 
 move_flip::mc_weight_type move_flip::attempt() {
@@ -57,6 +59,7 @@ move_flip::mc_weight_type move_hop::attempt() {
     std::uniform_int_distribution<> distr(0, n - 1);
     const int i = distr(RND);
     const std::vector<size_t> nb = config.lattice_.hop_neighbors(i);
+    if (nb.empty()) throw std::logic_error("nearest-neighbour exchange from an isolated site");
     std::uniform_int_distribution<> dn(0, (int)nb.size() - 1);
     const int j = (int)nb[dn(RND)];
     last_a = i;

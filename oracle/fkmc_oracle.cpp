@@ -344,6 +344,15 @@ struct config {
 // ---------------------------------------------------------------------------
 enum { MOVE_ADDREMOVE = 0, MOVE_FLIP = 1, MOVE_HOP = 2, MOVE_RANDOMIZE = 3 };
 
+// Test hooks.  g_mutation plants a deliberate mistake in the nearest-neighbour
+// exchange so that the tests can show their checks catch it (0: none; 1: the sign
+// of dlogZ in its weight flipped; 2: the sign of the f-f term flipped; 3: a
+// neighbour slot whose occupation equals site i's is replaced by the next slot
+// with a different occupation, an f-dependent proposal with no Hastings factor).
+// g_chain_W is the f-f interaction W given to the chains orc_run_chain builds.
+static int g_mutation = 0;
+static std::vector<double> g_chain_W;
+
 struct attempt_rec {
     int site_a = -1, site_b = -1;
     int solved = 0;
@@ -419,7 +428,11 @@ struct chain {
         int i = distr(rng);
         const std::vector<int>& nb = cur.lat->nbr[i];
         std::uniform_int_distribution<> dn(0, (int)nb.size() - 1);
-        int j = nb[dn(rng)];
+        int slot = dn(rng);
+        int j = nb[slot];
+        if (g_mutation == 3) {
+            for (int k = 1; k < (int)nb.size() && cur.f[j] == cur.f[i]; ++k) j = nb[(slot + k) % nb.size()];
+        }
         r.site_a = i;
         r.site_b = j;
         prop.f = cur.f;
@@ -431,8 +444,11 @@ struct chain {
         r.solved = 1;
         r.logz_new = prop.logZ;
         double ff_diff = prop.ff_energy() - cur.ff_energy();
-        r.logw = prop.logZ - cur.logZ - cur.beta * ff_diff;
-        return std::exp(prop.logZ - cur.logZ) * std::exp(-cur.beta * ff_diff);
+        double dlz = prop.logZ - cur.logZ;
+        if (g_mutation == 1) dlz = -dlz;
+        if (g_mutation == 2) ff_diff = -ff_diff;
+        r.logw = dlz - cur.beta * ff_diff;
+        return std::exp(dlz) * std::exp(-cur.beta * ff_diff);
     }
     // src/moves.cpp:35-49 (including its clamp quirk)
     double attempt_randomize(attempt_rec& r) {
@@ -584,6 +600,7 @@ int orc_run_chain(int kind, int D, int L, double t, double tp,
         chain ch(&l, beta, U, mu_c, mu_f, seed);
         ch.cur.use_stable_logz = ch.prop.use_stable_logz = stable_logz;
         ch.cur.f.assign(f_init, f_init + l.n);
+        ch.cur.W = ch.prop.W = g_chain_W;
         for (int m = 0; m < nmoves_kinds; ++m) ch.add_move(move_kinds[m], move_probs[m]);
         ch.cur.calc_hamiltonian();
         ch.cur.calc_ed(false);
@@ -607,6 +624,9 @@ int orc_run_chain(int kind, int D, int L, double t, double tp,
         return (int)ch.naccept;
     } catch (...) { return -1; }
 }
+
+void orc_set_mutation(int m) { g_mutation = m; }
+void orc_set_chain_ff(int nW, const double* W) { g_chain_W.assign(W, W + nW); }
 
 // Raw libstdc++ draws, for pinning the device port of the generator and
 // distributions: mode 0 = mt19937 words, 1 = uniform_int(0, hi), 2 =

@@ -52,6 +52,10 @@ def lib():
                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint32,
                                     _i, C.c_int, _i, _d, C.c_int, _i, C.c_int,
                                     _i, _i, _i, _d, _d, _d, _i, _i, _d, _i, _d]
+        L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_set_mutation.restype = None
+        L.orc_set_chain_ff.argtypes = [C.c_int, _d]
+        L.orc_set_chain_ff.restype = None
         L.orc_rng_draws.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, _d, C.c_int, _d]
         L.orc_measure_energy.argtypes = [C.c_int, _d, _d, C.c_double, C.c_int, C.c_double, _d, _d, _d]
         L.orc_measure_ipr.argtypes = [C.c_int, _d, _d]
@@ -177,6 +181,19 @@ def run_chain(kind, D, L, beta, U, mu_c, mu_f, seed, f_init, moves, nmoves, t=1.
     tr["f_final"] = ffin
     tr["spectrum_final"] = spec
     return tr
+
+
+def set_mutation(m):
+    """Plant (m > 0) or remove (m = 0) a deliberate mistake in the nearest-neighbour
+    exchange, for tests that show their checks catch it (see g_mutation in
+    fkmc_oracle.cpp)."""
+    lib().orc_set_mutation(int(m))
+
+
+def set_chain_ff(W):
+    """f-f interaction W (1-D lattices) of the chains run_chain builds from now on."""
+    W = np.ascontiguousarray(W if W is not None else [], dtype=np.float64)
+    lib().orc_set_chain_ff(len(W), _pd(W) if len(W) else None)
 
 
 def rng_draws(seed, mode, count, hi=0, probs=None):

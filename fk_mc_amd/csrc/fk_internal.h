@@ -125,6 +125,12 @@ int fk_solver_needs_ws(int kind);
 int fk_solver_nmax(int kind);
 size_t fk_solver_ws_doubles(int kind, int N);
 hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st);
+
+// Static device scratch of fkgpu_eig_logz_device_noalloc (fk_noalloc.hip): 256 MiB per
+// device.  fk_noalloc_scratch returns this device's address of it (resolved once per
+// device with hipGetSymbolAddress) and its size in doubles.
+#define FK_NOALLOC_SCRATCH_BYTES (256u << 20)
+hipError_t fk_noalloc_scratch(int device, double** ptr, size_t* doubles);
 hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t st);
 // randomize_f of every chain from its own generator (nf_start: B counts, 0 = drawn)
 hipError_t fk_launch_randomize_f(int B, int N, uint32_t* mt, uint8_t* f, int32_t* nf, const int32_t* nf_start,

@@ -32,6 +32,10 @@ struct fkgpu_ctx {
     int32_t* d_nbr = nullptr;
     int32_t* d_z = nullptr;
     int zmax = 0;
+    // this device's static scratch (fkgpu_eig_logz_device_noalloc), resolved at creation
+    double* noalloc_scratch = nullptr;
+    size_t noalloc_doubles = 0;
+    std::string noalloc_err;
     std::vector<double> hop;
     std::vector<int32_t> nbr, z;
     std::vector<double> hopv;
@@ -297,6 +301,9 @@ int fkgpu_create(int device, fkgpu_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
     if (ctx->num_cu <= 0) ctx->num_cu = 256;
+    // the allocation-free entry point's scratch: its device address, once per device
+    const hipError_t se = fk_noalloc_scratch(device, &ctx->noalloc_scratch, &ctx->noalloc_doubles);
+    if (se != hipSuccess) ctx->noalloc_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
     *out = ctx;
     return FKGPU_OK;
 }
@@ -716,6 +723,45 @@ int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const d
     }
     hipSetDevice(ctx->device);
     FK_HIP(ctx, fk_launch_eig(kind, a, grid, ctx->stream));
+    return FKGPU_OK;
+}
+
+int fkgpu_eig_logz_device_noalloc(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta,
+                                  const double* d_U, const double* d_mu_c, double* d_evals, double* d_logZ) {
+    if (!ctx) return FKGPU_EINVAL;
+    if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
+    if (batch <= 0 || !d_f || !d_beta || !d_U || !d_mu_c || !d_evals || !d_logZ)
+        return fail(ctx, FKGPU_EINVAL, "null argument or empty batch");
+    if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "dense eigensolver implemented for N <= 2048");
+    const int N = ctx->N;
+    const int kind = pick_kind(ctx, batch);
+    eig_args a{};
+    a.B = batch;
+    a.N = N;
+    a.hop = ctx->d_hop;
+    a.f = d_f;
+    a.beta = d_beta;
+    a.U = d_U;
+    a.mu_c = d_mu_c;
+    a.evals = d_evals;
+    a.logz = d_logZ;
+    a.ws_stride = fk_solver_ws_doubles(kind, N);
+    if (fk_solver_needs_ws(kind)) {
+        // one workgroup per configuration, each with its own slice of the static scratch
+        if (!ctx->noalloc_scratch) return fail(ctx, FKGPU_EHIP, "static scratch unavailable: " + ctx->noalloc_err);
+        const size_t need = (size_t)batch * a.ws_stride;
+        if (need > ctx->noalloc_doubles) {
+            const size_t most = ctx->noalloc_doubles / a.ws_stride;
+            return fail(ctx, FKGPU_EUNSUPPORTED,
+                        "batch of " + std::to_string(batch) + " configurations at N=" + std::to_string(N) + " needs " +
+                            std::to_string(need * sizeof(double) >> 20) + " MiB of scratch, more than the " +
+                            std::to_string(FK_NOALLOC_SCRATCH_BYTES >> 20) + " MiB static scratch (at most " +
+                            std::to_string(most) + " configurations per call at this N)");
+        }
+        a.ws = ctx->noalloc_scratch;
+    }
+    hipSetDevice(ctx->device);
+    FK_HIP(ctx, fk_launch_eig(kind, a, batch, ctx->stream));
     return FKGPU_OK;
 }
 

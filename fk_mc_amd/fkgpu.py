@@ -77,6 +77,7 @@ def lib():
         L.fkgpu_msize.argtypes = [P]
         L.fkgpu_eig_logz.argtypes = [P, C.c_int, _i32, _d, _d, _d, C.c_int, _d, _d, _d, _d, _d]
         L.fkgpu_eig_logz_device.argtypes = [P, C.c_int, P, P, P, P, P, P]
+        L.fkgpu_eig_logz_device_noalloc.argtypes = [P, C.c_int, P, P, P, P, P, P]
         L.fkgpu_chains_create.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _u32, C.POINTER(P)]
         L.fkgpu_chains_destroy.argtypes = [P]
         L.fkgpu_chains_destroy.restype = None

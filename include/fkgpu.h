@@ -102,6 +102,18 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
 int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta, const double* d_U,
                           const double* d_mu_c, double* d_evals, double* d_logZ);
 
+/* Same as fkgpu_eig_logz_device, allocating nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The HBM-workspace solvers (N > 256)
+ * work in a static device array of 256 MiB per GPU (held from library load to process
+ * end) instead of a hipMalloc'd buffer; a batch whose workspace exceeds it, N x N
+ * doubles per configuration, is refused with FKGPU_EUNSUPPORTED and a message naming
+ * the largest batch at that N.  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU (other streams, other contexts, other threads) race; calls
+ * ordered on one stream do not. */
+int fkgpu_eig_logz_device_noalloc(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const double* d_beta,
+                                  const double* d_U, const double* d_mu_c, double* d_evals, double* d_logZ);
+
 /* ---- Eigenvector measures and the adjoint (eigenvectors stay on the device) ----
  * Each call runs calc_hamiltonian + calc_ed(true) for `batch` configurations and
  * then the measure, without copying the N x N eigenvector matrices to the host. */

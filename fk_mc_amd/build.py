@@ -21,7 +21,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 GPU_SOURCES = ["csrc/fk_kernels.hip", "csrc/fk_vectors.hip", "csrc/fk_measures.hip", "csrc/fk_cheb.hip",
-               "csrc/fk_mixed.hip", "csrc/fk_update.hip", "csrc/fk_green.hip", "csrc/fk_noalloc.hip",
+               "csrc/fk_mixed.hip", "csrc/fk_update.hip", "csrc/fk_green.hip", "csrc/fk_noalloc.hip", "csrc/fk_kpm_local.hip",
                "csrc/fkgpu.cpp"]
 HOST_SOURCES = ["host/lattice.cpp", "host/configuration.cpp", "host/moves.cpp",
                 "host/mc_metropolis.cpp", "host/measures.cpp", "host/capi.cpp"]

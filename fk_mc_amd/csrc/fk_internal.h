@@ -177,6 +177,16 @@ hipError_t fk_cheb_exact_moments(int N, const int32_t* nbr, const int32_t* z, in
                                  double* parts, double* scal, double* traces, hipStream_t st);
 // hopq (host pointer, zmax values or nullptr): the hopping value of neighbour slot q when it
 // is the same at every site (every site then has zmax neighbours)
+// dlogZ of one- or two-site proposals of the stochastic-trace chains from local
+// Chebyshev moments (M of them, FP64) and n0 Matsubara frequencies (fk_kpm_local.hip);
+// lz_out = logz + dlogZ.  s2c, s4c: sums of (2n+1)^-2 and (2n+1)^-4 over n >= n0.
+#define FK_KPM_LOCAL_NMAX 4096
+#define FK_KPM_LOCAL_MMAX 2048
+hipError_t fk_launch_kpm_local(int B, int N, int M, int n0, const uint8_t* f, const int32_t* fprop,
+                               const move_prop* props, const int32_t* skip, const double* U, const double* mu_c,
+                               const double* beta, const double* ab, const int32_t* nbr, const int32_t* z, int zmax,
+                               const double* hopv, const double* hopq, double s2c, double s4c, double* mom,
+                               const double* logz, double* lz_out, hipStream_t st);
 hipError_t fk_launch_kpm(int B, int N, int R, int M, const int32_t* f, const double* U, const double* mu_c,
                          const double* ab, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
                          const double* hopq, uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,

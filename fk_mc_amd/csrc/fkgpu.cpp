@@ -113,6 +113,12 @@ struct fkgpu_chains {
     move_prop* k_props = nullptr;
     int32_t *k_fprop = nullptr, *k_skip = nullptr;
     double *k_ab = nullptr, *k_coef = nullptr, *k_mom = nullptr, *k_lz = nullptr;
+    // how a proposal's dlogZ is estimated: local Chebyshev moments + Matsubara sum
+    // (kl_M moments, kl_n0 frequencies; kl_s2, kl_s4 the tail sums of (2n+1)^-2, ^-4)
+    // or the difference of the two configurations' stochastic traces
+    int kdelta = FKGPU_KPM_DELTA_LOCAL, kl_M = 0, kl_n0 = 0;
+    double kl_s2 = 0.0, kl_s4 = 0.0;
+    double* k_lmom = nullptr;
     bool dense_ok = true;
     // mixed-precision mode (FP32 reduction + FP64 refinement)
     int mixed = 0;
@@ -1176,7 +1182,7 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     void* ptrs[] = {ch->f, ch->nf, ch->logz, ch->spectrum, ch->mt, ch->beta, ch->U, ch->mu_c, ch->mu_f,
                     ch->naccept, ch->nsolve, ch->ws, ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw,
                     ch->t_logz, ch->t_acc, ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef,
-                    ch->k_mom, ch->k_lz, ch->q_ctr, ch->q_ring, ch->q_err};
+                    ch->k_mom, ch->k_lz, ch->k_lmom, ch->q_ctr, ch->q_ring, ch->q_err};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ch->ev0) hipEventDestroy(ch->ev0);
@@ -1423,6 +1429,13 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
     if (!ch->kpm) return fk_launch_sweep(ch->kind, a, ch->grid, st);
     hipError_t e = fk_launch_kpm_propose(a, ch->k_props, ch->k_fprop, ch->k_skip, st);
     if (e != hipSuccess) return e;
+    if (ch->kdelta == FKGPU_KPM_DELTA_LOCAL) {
+        e = fk_launch_kpm_local(ch->B, ch->N, ch->kl_M, ch->kl_n0, ch->f, ch->k_fprop, ch->k_props, ch->k_skip, ch->U,
+                                ch->mu_c, ch->beta, ch->k_ab, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv,
+                                hopq_ptr(ctx), ch->kl_s2, ch->kl_s4, ch->k_lmom, ch->logz, ch->k_lz, st);
+        if (e != hipSuccess) return e;
+        return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->k_lz, st);
+    }
     e = fk_launch_kpm(ch->B, ch->N, ch->kR, ch->kM, ch->k_fprop, ch->U, ch->mu_c, ch->k_ab, ctx->d_nbr, ctx->d_z,
                       ctx->zmax, ctx->d_hopv, hopq_ptr(ctx), ch->kseed, 1, ch->k_skip, ch->k_coef, ch->kjack, ch->k_mom, nullptr,
                       ch->k_lz, st);
@@ -1724,6 +1737,77 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
     ch->kseed = seed;
     ch->kjack = jackson;
     drop_graphs(ch);
+    // proposals' dlogZ from local moments by default
+    return fkgpu_chains_set_kpm_delta(ch, FKGPU_KPM_DELTA_LOCAL, 0, 0);
+}
+
+// Sums of (2n + 1)^-k over n >= n0 (k = 2, 4): direct sum of 2^20 terms in long double,
+// then the Euler-Maclaurin remainder (integral + half the first omitted term).
+static double odd_power_tail(int n0, int k) {
+    const long nt = 1L << 20;
+    long double sum = 0.0L;
+    for (long n = n0 + nt - 1; n >= n0; --n) sum += 1.0L / powl(2.0L * n + 1.0L, k);
+    const long double x = 2.0L * (n0 + nt) + 1.0L;
+    sum += 1.0L / ((k - 1) * 2.0L * powl(x, k - 1)) + 0.5L / powl(x, k);
+    return (double)sum;
+}
+
+int fkgpu_chains_set_kpm_delta(fkgpu_chains* ch, int mode, int moments, int frequencies) {
+    if (!ch) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = ch->ctx;
+    if (!ch->kpm) return fail(ctx, FKGPU_EINVAL, "the chain set is not on the stochastic-trace path (fkgpu_chains_set_kpm)");
+    if (mode == FKGPU_KPM_DELTA_STOCHASTIC) {
+        ch->kdelta = mode;
+        drop_graphs(ch);
+        return FKGPU_OK;
+    }
+    if (mode != FKGPU_KPM_DELTA_LOCAL) return fail(ctx, FKGPU_EINVAL, "unknown dlogZ estimator");
+    if (ch->N > FK_KPM_LOCAL_NMAX) return fail(ctx, FKGPU_EUNSUPPORTED, "local dlogZ estimator needs N <= 4096");
+    if (moments < 0 || moments == 1 || moments > FK_KPM_LOCAL_MMAX || frequencies < 0 || frequencies == 1)
+        return fail(ctx, FKGPU_EINVAL, "moments must be 2..2048 and frequencies >= 2 (0: automatic)");
+    const int B = ch->B;
+    hipSetDevice(ctx->device);
+    std::vector<double> beta(B), ab(2 * (size_t)B);
+    FK_HIP(ctx, hipMemcpy(beta.data(), ch->beta, sizeof(double) * B, hipMemcpyDeviceToHost));
+    FK_HIP(ctx, hipMemcpy(ab.data(), ch->k_ab, sizeof(double) * 2 * B, hipMemcpyDeviceToHost));
+    if (moments == 0) {
+        // the series of 1/(z' - x) at the lowest frequency decays like |r|^m with
+        // |r| <= 1 - pi / (beta a) (z' on the imaginary axis, the slowest case): enough
+        // terms for |r|^M < 1e-12, rounded up to a multiple of 32
+        double worst = 0.0;
+        for (int c = 0; c < B; ++c) worst = std::max(worst, beta[c] / ab[2 * c]);
+        const double need = std::log(1e12) * worst / M_PI;
+        moments = std::max(64, std::min(FK_KPM_LOCAL_MMAX, 32 * (int)std::ceil(need / 32.0)));
+    }
+    if (frequencies == 0) frequencies = 512;
+    if (ch->k_lmom) hipFree(ch->k_lmom);
+    ch->k_lmom = nullptr;
+    if (dalloc(&ch->k_lmom, (size_t)B * 3 * moments) != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "local moment buffers");
+    ch->kl_M = moments;
+    ch->kl_n0 = frequencies;
+    ch->kl_s2 = odd_power_tail(frequencies, 2);
+    ch->kl_s4 = odd_power_tail(frequencies, 4);
+    ch->kdelta = FKGPU_KPM_DELTA_LOCAL;
+    drop_graphs(ch);
+    return FKGPU_OK;
+}
+
+int fkgpu_chains_kpm_delta(const fkgpu_chains* ch, int* mode, int* moments, int* frequencies) {
+    if (!ch) return FKGPU_EINVAL;
+    if (mode) *mode = ch->kpm ? ch->kdelta : -1;
+    if (moments) *moments = ch->kl_M;
+    if (frequencies) *frequencies = ch->kl_n0;
+    return FKGPU_OK;
+}
+
+// The local dlogZ estimator covers moves that change one or two sites.
+static int kpm_moves_ok(fkgpu_chains* ch) {
+    if (!ch->kpm || ch->kdelta != FKGPU_KPM_DELTA_LOCAL) return FKGPU_OK;
+    for (int m = 0; m < ch->args.ncp; ++m)
+        if (ch->args.kinds[m] == FKGPU_MOVE_RANDOMIZE)
+            return fail(ch->ctx, FKGPU_EUNSUPPORTED,
+                        "the local dlogZ estimator covers moves of one or two sites; randomize needs "
+                        "FKGPU_KPM_DELTA_STOCHASTIC");
     return FKGPU_OK;
 }
 
@@ -1731,6 +1815,7 @@ int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
     if (nsteps < 0) return fail(ctx, FKGPU_EINVAL, "negative step count");
+    if (int r = kpm_moves_ok(ch)) return r;
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     sweep_args a = ch->args;
@@ -1794,6 +1879,7 @@ int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
     if (nsteps <= 0) return fail(ctx, FKGPU_EINVAL, "step count must be positive");
+    if (int r = kpm_moves_ok(ch)) return r;
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     if (!ch->gexec || ch->graph_steps != nsteps) {
@@ -2212,7 +2298,9 @@ int fkgpu_sweep_measure(fkgpu_chains* ch, int nsweeps, int sweep_len) {
     if (nsweeps < 0 || sweep_len <= 0) return fail(ctx, FKGPU_EINVAL, "invalid sweep count or length");
     if (!ch->kpm && !ch->dense_ok)
         return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
-    int r = meas_prepare(ch);
+    int r = kpm_moves_ok(ch);
+    if (r) return r;
+    r = meas_prepare(ch);
     if (r) return r;
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;

@@ -19,6 +19,7 @@ MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
 SOLVER_AUTO, SOLVER_VARIANTS = -1, 7
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 UPDATE_DENSE, UPDATE_RANK1 = 0, 1
+KPM_DELTA_LOCAL, KPM_DELTA_STOCHASTIC = 0, 1
 MEASURE_SPECTRUM_HISTORY, MEASURE_FOCC_HISTORY, MEASURE_IPR, MEASURE_STIFFNESS = 1, 2, 4, 8
 MEASURE_EIGENFUNCTIONS = 16
 
@@ -85,6 +86,8 @@ def lib():
         L.fkgpu_chains_set_ff.argtypes = [P, C.c_int, _d]
         L.fkgpu_sweep.argtypes = [P, C.c_int, C.POINTER(_Trace)]
         L.fkgpu_chains_set_kpm.argtypes = [P, C.c_int, C.c_int, C.c_uint64, C.c_int]
+        L.fkgpu_chains_set_kpm_delta.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        L.fkgpu_chains_kpm_delta.argtypes = [P, _i32, _i32, _i32]
         L.fkgpu_sweep_graph.argtypes = [P, C.c_int]
         L.fkgpu_chains_last_ms.argtypes = [P, _d]
         L.fkgpu_chains_state.argtypes = [P, _i32, _i32, _d, _d, _i64, _i64]
@@ -387,6 +390,14 @@ class Chains:
 
     def set_kpm(self, R=64, M=256, seed=1, jackson=True):
         self.ctx._check(lib().fkgpu_chains_set_kpm(self._h, R, M, seed, int(jackson)))
+
+    def set_kpm_delta(self, mode, moments=0, frequencies=0):
+        self.ctx._check(lib().fkgpu_chains_set_kpm_delta(self._h, mode, moments, frequencies))
+
+    def kpm_delta(self):
+        v = np.zeros(3, np.int32)
+        self.ctx._check(lib().fkgpu_chains_kpm_delta(self._h, _p(v[0:], _i32), _p(v[1:], _i32), _p(v[2:], _i32)))
+        return dict(mode=int(v[0]), moments=int(v[1]), frequencies=int(v[2]))
 
     def set_ff(self, W):
         W = np.ascontiguousarray(W, dtype=np.float64)

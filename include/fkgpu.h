@@ -230,14 +230,38 @@ int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision);
 int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps);
 /* Configurations the mixed-precision refinement handed to the FP64 solver so far. */
 int fkgpu_chains_fallbacks(const fkgpu_chains* ch, int64_t* n);
-/* Switch the chain set to the stochastic-trace logZ (the Chebyshev moves of
- * src/moves_chebyshev.cpp with the kernel-polynomial estimate instead of the exact
- * trace): R probe vectors per chain (fixed for the whole run, so old and new
- * configurations see the same vectors), M moments, Jackson damping if jackson != 0,
- * spectral bounds valid for every configuration of the chain.  The chains' logZ is
- * re-estimated; each move step then takes three launches (propose, trace, accept).
- * Required for lattices above the dense solver's size (N > 2048). */
+/* Switch the chain set to the Chebyshev path (the moves of src/moves_chebyshev.cpp)
+ * with the kernel-polynomial estimate of logZ instead of the exact trace: R probe
+ * vectors per chain (fixed for the whole run), M moments, Jackson damping if
+ * jackson != 0, spectral bounds valid for every configuration of the chain.  The chains'
+ * logZ is estimated from the probes; a proposal's dlogZ then comes from the local
+ * estimator below (fkgpu_chains_set_kpm_delta), each move step in four launches
+ * (propose, moments, frequencies, accept).  Required for lattices above the dense
+ * solver's size (N > 2048). */
 int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jackson);
+/* How the chains on the Chebyshev path weigh a proposal (the dlogZ of
+ * src/moves_chebyshev.cpp:6-72; the reference takes it as the difference of two exact
+ * traces, src/configuration.cpp:94-206):
+ *   FKGPU_KPM_DELTA_LOCAL       (default after fkgpu_chains_set_kpm) the exact dlogZ of a
+ *                               move of one or two sites, to truncation error: FP64
+ *                               Chebyshev vectors started at the changed sites give
+ *                               G_pq(z) = <e_p|(z - H)^-1|e_q> at the Matsubara
+ *                               frequencies z = i(2n+1)pi/beta, and dlogZ =
+ *                               -beta Tr(Delta)/2 + sum_n log|det(1 - G_P Delta_P)|^2
+ *                               with a fitted 1/w^2 + 1/w^4 tail after `frequencies`
+ *                               terms.  `moments` Chebyshev moments (0: enough for a
+ *                               truncation below 1e-12 at the chains' beta and spectral
+ *                               scale; at most 2048), `frequencies` (0: 512).  N <= 4096;
+ *                               the randomize move is refused.
+ *   FKGPU_KPM_DELTA_STOCHASTIC  the difference of the two configurations' stochastic
+ *                               traces with the chain's fixed probe vectors (biased by the
+ *                               probe set; every move kind). */
+#define FKGPU_KPM_DELTA_LOCAL 0
+#define FKGPU_KPM_DELTA_STOCHASTIC 1
+int fkgpu_chains_set_kpm_delta(fkgpu_chains* ch, int mode, int moments, int frequencies);
+/* The estimator in use (-1 when the chain set is not on the Chebyshev path) and the local
+ * estimator's moment and frequency counts (nullable outputs). */
+int fkgpu_chains_kpm_delta(const fkgpu_chains* ch, int* mode, int* moments, int* frequencies);
 /* Move steps of every chain fused into one launch in the FP64 dense mode (no reference
  * counterpart; default 16, 1 = one launch per step).  One workgroup per (step, chain) item, chains taken from a queue of
  * ready chains; a chain's step starts after its previous step, so every chain draws and

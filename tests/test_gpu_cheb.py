@@ -117,9 +117,10 @@ def test_kpm_moments_lattices(fk, kind):
 
 
 def test_kpm_chains(fk, orc):
-    """Chebyshev-move chains (src/moves_chebyshev.cpp) with the stochastic trace:
-    draws follow the reference order, dlogZ of each solved proposal agrees with
-    dense ED within the estimator's noise, graph replay equals eager launches."""
+    """Chebyshev-move chains (src/moves_chebyshev.cpp) weighed by the stochastic trace
+    (FKGPU_KPM_DELTA_STOCHASTIC): draws follow the reference order, dlogZ of each solved
+    proposal agrees with dense ED within the estimator's noise, graph replay equals eager
+    launches."""
     print("case kpm chains", flush=True)
     L, U, beta = 32, 2.0, 10.0
     n = L * L
@@ -131,6 +132,7 @@ def test_kpm_chains(fk, orc):
     ch = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
     ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
     ch.set_kpm(R=64, M=256, seed=3, jackson=True)
+    ch.set_kpm_delta(fk.KPM_DELTA_STOCHASTIC)
     lz0 = ch.state()["logZ"].copy()
     tr = ch.sweep(12, trace=True)
     # first move of every chain: same site and uniform as the reference chain
@@ -157,6 +159,7 @@ def test_kpm_chains(fk, orc):
     for x in (a, b):
         x.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
         x.set_kpm(R=16, M=64, seed=9, jackson=True)
+        x.set_kpm_delta(fk.KPM_DELTA_STOCHASTIC)
     for rep in range(2):
         a.sweep(8)
         b.sweep_graph(8)
@@ -181,11 +184,11 @@ def _exact_logz_2d(L, f, beta, U, mu):
 
 
 def test_kpm_chain_large_lattice(fk):
-    """cubic2d L=64 (N=4096), beta=10, U=2 runs on the stochastic trace only (above the
-    dense range).  dOmega of the first proposals against an exact FP64 dense eigensolve:
-    the chain's estimate within the estimator's spread (sd of dlogZ about 1.3 at R=64,
-    measured over independent probe sets), and the mean over 16 independent probe sets
-    within 4 standard errors (the estimator is unbiased up to the Jackson smoothing)."""
+    """cubic2d L=64 (N=4096), beta=10, U=2 with proposals weighed by the stochastic trace
+    (above the dense range).  dOmega of the first proposals against an exact FP64 dense
+    eigensolve: the chain's estimate within the estimator's spread (sd of dlogZ about 1.3
+    at R=64, measured over independent probe sets), and the mean over 16 independent probe
+    sets within 4 standard errors (the estimator is unbiased up to the Jackson smoothing)."""
     print("case kpm L=64", flush=True)
     L = 64
     n = L * L
@@ -198,6 +201,7 @@ def test_kpm_chain_large_lattice(fk):
         ch.sweep(1)  # no dense solver at this size
     ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
     ch.set_kpm(R=64, M=256, seed=5, jackson=True)
+    ch.set_kpm_delta(fk.KPM_DELTA_STOCHASTIC)
     lz0 = ch.state()["logZ"].copy()
     tr = ch.sweep(4, trace=True)
     st = ch.state()
@@ -220,4 +224,164 @@ def test_kpm_chain_large_lattice(fk):
             print("16 probe sets: mean dlogZ", ds.mean(), "sd", ds.std(ddof=1), "exact", d_ex, flush=True)
             assert 0.3 < ds.std(ddof=1) < 4.0
             assert abs(ds.mean() - d_ex) < 4 * se + 0.05
+    ctx.close()
+
+
+# ---- local estimator of a proposal's dlogZ (FKGPU_KPM_DELTA_LOCAL) -------------------
+
+NEAR_TIE = 1e-9
+
+
+def _compare_local_chain(orc, kind, D, L, beta, U, mu, seed, f0, moves, tr, tp, W=None):
+    """Device chain with locally estimated dlogZ against the oracle's dense-ED chain:
+    draws bit for bit, log-weights within 1e-9, decisions equal except at near-ties
+    (where the oracle follows the device).  Returns (#near-ties, max |dlogw|)."""
+    nsteps = len(tr["u"])
+    forced = -np.ones(nsteps, np.int32)
+    orc.set_chain_ff(W)
+    try:
+        for _attempt in range(8):
+            ref = orc.run_chain(kind, D, L, beta, U, mu, mu, seed, f0, moves, nsteps, tp=tp, forced=forced)
+            assert np.array_equal(tr["kind"], ref["kind"]), "move choice differs"
+            assert np.array_equal(tr["a"], ref["a"]), "site draw differs"
+            assert np.array_equal(tr["b"], ref["b"]), "second site differs"
+            assert np.array_equal(tr["u"], ref["u"]), "uniform differs"
+            fin = np.isfinite(ref["logw"])
+            assert np.array_equal(fin, np.isfinite(tr["logw"]))
+            dw = np.abs(tr["logw"][fin] - ref["logw"][fin]).max(initial=0.0)
+            assert dw < 1e-9, f"log-weight differs by {dw}"
+            mism = np.nonzero(tr["acc"] != ref["acc"])[0]
+            if len(mism) == 0:
+                return int(np.sum(forced >= 0)), dw
+            s = mism[0]
+            gap = abs(np.log(ref["u"][s]) - ref["logw"][s])
+            assert gap < NEAR_TIE, f"accept differs at move {s}: |log u - log w| = {gap}"
+            forced[s] = tr["acc"][s]
+    finally:
+        orc.set_chain_ff(None)
+    raise AssertionError("too many near-ties")
+
+
+LOCAL_CASES = [
+    # name, kind, D, L, beta, U, moves, W
+    ("square8", 0, 2, 8, 10.0, 2.0, "ah", None),
+    ("square12_flip", 0, 2, 12, 5.0, 4.0, "afh", None),
+    ("triangular6", 1, 2, 6, 8.0, 3.0, "ah", None),
+    ("cubic3d4", 0, 3, 4, 2.0, 5.0, "ah", None),
+    ("chain24_ff", 0, 1, 24, 3.0, 2.0, "ah", [0.0, 0.5, 0.2]),
+]
+
+
+@pytest.mark.parametrize("name,kind,D,L,beta,U,mv,W", LOCAL_CASES)
+def test_kpm_local_chains_match_oracle(fk, orc, name, kind, D, L, beta, U, mv, W):
+    """Chains on the Chebyshev path with the local dlogZ estimator reproduce the
+    reference's ED chains (src/moves.cpp, src/mc_metropolis.cpp:34-52) of the same seeds."""
+    print(f"case kpm local {name}", flush=True)
+    n = L ** D
+    tp = 1.0 if kind == 1 else 0.0
+    table = {"a": (fk.MOVE_ADDREMOVE, orc.MOVE_ADDREMOVE), "f": (fk.MOVE_FLIP, orc.MOVE_FLIP),
+             "h": (fk.MOVE_HOP, orc.MOVE_HOP)}
+    w = 1.0 / len(mv)
+    moves_fk = [(table[c][0], w) for c in mv]
+    moves_orc = [(table[c][1], w) for c in mv]
+    rng = np.random.default_rng(n + 3)
+    B = 3
+    f0 = rng.integers(0, 2, (B, n)).astype(np.int32)
+    seeds = np.arange(B, dtype=np.uint32) + 900
+    ctx = fk.Context(0).set_lattice(kind, D, L, 1.0, tp)
+    ch = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, seeds)
+    ch.set_moves(moves_fk)
+    if W is not None:
+        ch.set_ff(W)
+    ch.set_kpm(R=8, M=64, seed=1, jackson=True)
+    info = ch.kpm_delta()
+    assert info["mode"] == fk.KPM_DELTA_LOCAL and info["moments"] >= 64 and info["frequencies"] == 512
+    tr = ch.sweep(150, trace=True)
+    worst = 0.0
+    for c in range(B):
+        trc = {k: v[:, c] for k, v in tr.items()}
+        _, dw = _compare_local_chain(orc, kind, D, L, beta, U, U / 2, int(seeds[c]), f0[c], moves_orc, trc, tp, W)
+        worst = max(worst, dw)
+    print(name, "moments", info["moments"], "max |dlogw| vs oracle", worst, flush=True)
+    ch.close()
+    ctx.close()
+
+
+def test_kpm_local_graph_replay_and_refusals(fk):
+    """Graph replay equals eager launches; randomize is refused with the local estimator
+    and runs with the stochastic one."""
+    print("case kpm local graph", flush=True)
+    L, n = 16, 256
+    rng = np.random.default_rng(4)
+    f0 = rng.integers(0, 2, (4, n)).astype(np.int32)
+    seeds = np.arange(4, dtype=np.uint32) + 70
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    a = fk.Chains(ctx, f0, 6.0, 2.0, 1.0, 1.0, seeds)
+    b = fk.Chains(ctx, f0, 6.0, 2.0, 1.0, 1.0, seeds)
+    for x in (a, b):
+        x.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+        x.set_kpm(R=8, M=64, seed=2, jackson=True)
+    for rep in range(3):
+        a.sweep(8)
+        b.sweep_graph(8)
+        sa, sb = a.state(), b.state()
+        assert np.array_equal(sa["f"], sb["f"]) and np.array_equal(sa["logZ"], sb["logZ"]), rep
+    a.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_RANDOMIZE, 0.5)])
+    with pytest.raises(fk.FkgpuError, match="randomize"):
+        a.sweep(2)
+    a.set_kpm_delta(fk.KPM_DELTA_STOCHASTIC)
+    a.sweep(2)
+    with pytest.raises(fk.FkgpuError):
+        a.set_kpm_delta(fk.KPM_DELTA_LOCAL, moments=1)
+    a.close()
+    b.close()
+    ctx.close()
+
+
+def test_kpm_local_large_lattice(fk):
+    """cubic2d L=64 (N=4096), beta=10, U=2, addremove / exchange: dOmega of the traced
+    proposals against exact FP64 dense eigensolves of both configurations (LAPACK via
+    numpy) within 1e-9; the reference's own Chebyshev weights are held to 6e-2
+    (benchmark/fast_update.cpp:67-72)."""
+    print("case kpm local L=64", flush=True)
+    L = 64
+    n = L * L
+    beta, U = 10.0, 2.0
+    rng = np.random.default_rng(2)
+    B = 2
+    f0 = rng.integers(0, 2, (B, n)).astype(np.int32)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, f0, beta, U, U / 2, U / 2, [1, 2])
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+    ch.set_kpm(R=64, M=256, seed=5, jackson=True)
+    lz0 = ch.state()["logZ"].copy()
+    tr = ch.sweep(3, trace=True)
+    checked = 0
+    for c in range(B):
+        f = f0[c].copy()
+        lz = lz0[c]
+        ex_cur = _exact_logz_2d(L, f, beta, U, U / 2)
+        for s in range(3):
+            a, b = tr["a"][s, c], tr["b"][s, c]
+            g = f.copy()
+            if tr["kind"][s, c] == fk.MOVE_HOP:
+                g[a], g[b] = f[b], f[a]
+            else:
+                g[a] ^= 1
+            solved = tr["logz"][s, c] != 0.0
+            ex_new = None
+            if solved:
+                ex_new = _exact_logz_2d(L, g, beta, U, U / 2)
+                d_loc = -(tr["logz"][s, c] - lz) / beta
+                d_ex = -(ex_new - ex_cur) / beta
+                print("chain", c, "step", s, "dOmega local", d_loc, "exact", d_ex, flush=True)
+                assert abs(d_loc - d_ex) < 1e-9
+                checked += 1
+            if tr["acc"][s, c]:
+                f = g
+                if solved:
+                    lz = tr["logz"][s, c]
+                    ex_cur = ex_new
+    assert checked >= 2
+    ch.close()
     ctx.close()

@@ -56,6 +56,8 @@ FP32_PEAK_TFLOPS = 157.3
 # WRITE_SIZE x 1.00 (scales from a 256 MiB copy, bench.py --copy-calibration); keyed by
 # solver variant
 TRAFFIC_BYTES = {5: 124.5e6}
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the 16-step launch, scaled by a plain copy "
+                  "(profiles/k_sweep_q_cubic2d_L16_B1024_hbm_traffic.txt)")
 KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",
                 2: "fkk::k_sweep_q<reg_solver<32,8>>", 3: "fkk::k_sweep_q<glob_solver>",
                 4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver>",
@@ -87,8 +89,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the cubic3d L=8 mixed-vs-FP64 comparison")
     ap.add_argument("--no-c4", action="store_true", help="skip the triangular adjoint workload")
     ap.add_argument("--no-c1", action="store_true", help="skip the single-chain cubic2d L=8 timing")
-    ap.add_argument("--no-green", action="store_true",
-                    help="skip the G(w,r) / local-DOS timing over recorded eigenvectors")
+    ap.add_argument("--green", action="store_true",
+                    help="also time G(w,r) / local DOS over recorded eigenvectors (post-processing, off by default)")
     ap.add_argument("--rank1-steps", type=int, default=64, help="rank-one update chain timing steps (0: skip)")
     ap.add_argument("--measure-sweeps", type=int, default=4,
                     help="measured sweeps of 16 moves with on-device measures (0: skip)")
@@ -121,15 +123,23 @@ def main():
     ch = fk.Chains(ctx, f, beta, U, U / 2, U / 2, seeds)
     ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
 
-    # warmup (untimed), then exactly `steps` move steps between synchronisations
+    # warmup (untimed), then exactly `steps` move steps between synchronisations.  The
+    # timed steps are issued as calls of steps_per_launch steps (one fused launch each,
+    # plus the remainder), so the HIP-event time of every launch is known; each call is
+    # synchronous, which adds a host round trip per launch (tens of us against ~80 ms).
     if args.warmup:
         ch.sweep(args.warmup)
+    spl = ch.steps_per_launch
+    chunks = [spl] * (args.steps // spl) + ([args.steps % spl] if args.steps % spl else [])
     s0 = ch.state()
+    launch_ms = []
     t0 = time.perf_counter()
-    ch.sweep(args.steps)  # synchronous: returns after the last launch completed
+    for k in chunks:
+        ch.sweep(k)  # synchronous: returns after the launch completed
+        launch_ms.append(ch.last_ms())
     t1 = time.perf_counter()
-    kern_ms = ch.last_ms()
     s1 = ch.state()
+    kern_ms = float(np.sum(launch_ms))
     wall_ms = (t1 - t0) * 1e3
     moves = B * args.steps
     value = moves / (wall_ms / 1e3)
@@ -137,17 +147,25 @@ def main():
     acc = int((s1["naccept"] - s0["naccept"]).sum())
     flop = solves * (4.0 / 3.0) * N ** 3
     achieved = flop / (kern_ms / 1e3) / 1e12
-    # one launch runs steps_per_launch move steps of every chain (one workgroup per
-    # (step, chain) item); the HIP-event time also holds one tiny queue-reset launch each
-    spl = ch.steps_per_launch
-    launches = -(-args.steps // spl)
+    full = [ms for k, ms in zip(chunks, launch_ms) if k == spl]
+    # HBM bytes of one launch of steps_per_launch steps at the default workload, from the
+    # rocprofv3 counters (profiles/, FETCH_SIZE and WRITE_SIZE scaled by a copy kernel);
+    # the traffic is about proportional to the (step, chain) items, so the mean over the
+    # timed launches scales with the steps each launch ran
+    per_launch_bytes = TRAFFIC_BYTES.get(ctx.get_solver()) if (L, B, spl) == (16, 1024, 16) else None
+    traffic = per_launch_bytes * (args.steps / spl) / len(chunks) if per_launch_bytes else None
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                "traffic": TRAFFIC_BYTES.get(ctx.get_solver())
-                if (L, B, spl) == (16, 1024, 16) and args.steps % spl == 0 else None,
+                "traffic": traffic,
+                "traffic_source": TRAFFIC_SOURCE if traffic else None,
+                "traffic_per_full_launch": per_launch_bytes,
                 "kernel": KERNEL_NAMES.get(ctx.get_solver(), "k_sweep_q"), "steps_per_launch": spl,
-                "kernel_ms_per_launch": round(kern_ms / launches, 4),
-                "flop_per_launch": flop / launches, "eigensolves_per_launch": solves / launches}
+                "launches": len(chunks), "steps_per_timed_launch": chunks,
+                "kernel_ms_per_launch": round(float(np.mean(full)) if full else kern_ms / len(chunks), 4),
+                "kernel_ms_per_launch_all": round(kern_ms / len(chunks), 4),
+                "flop_per_launch": flop / len(chunks), "eigensolves_per_launch": solves / len(chunks),
+                "work": "(4/3) N^3 FP64 flop per eigensolve (Householder tridiagonalisation), eigensolves "
+                        "counted on the device, divided by the HIP-event time of the launches"}
 
     # plain single-chain drop-in: one calc_hamiltonian+calc_ed per move, host buffers
     plain = None
@@ -176,7 +194,7 @@ def main():
     if args.measure_sweeps > 0:
         measured = measured_block(fk, ctx, f, beta, U, seeds, args.measure_sweeps)
     green = None
-    if not args.no_green:
+    if args.green:
         green = green_block(fk, ctx, f, beta, U, seeds)
     mixed = None
     if args.mixed_steps > 0:
@@ -215,6 +233,8 @@ def main():
                                "moves addremove/nn-exchange 0.5/0.5, one move per chain per step",
                    "lattice": "cubic2d", "L": L, "N": N, "chains": B, "global_batch": B},
         "accept_rate": round(acc / moves, 4),
+        "eigensolves_per_s": round(solves / (wall_ms / 1e3), 1),
+        "null_moves": round(1.0 - solves / moves, 4),
         "roofline": roofline,
         "plain": plain,
         "single_chain_cubic2d_L8": c1,
@@ -232,23 +252,38 @@ def main():
     ctx.close()
 
 
+EPS = np.finfo(np.float64).eps
+DOMEGA_ERR_DEF = "|dOmega - dOmega_FP64| / max(|dOmega_FP64|, eps |Omega_FP64|), eps = 2^-52"
+
+
 def compare_traces(t64, tmx, beta):
-    """Per-move |dOmega_mixed - dOmega_FP64| while the two chains agree, and the accept
-    mismatches (a chain is compared up to its first differing decision)."""
+    """Per-move dOmega differences while the two chains agree, and the accept mismatches
+    (a chain is compared up to its first differing decision).  The relative error is
+    taken against max(|dOmega|, eps |Omega|): dOmega is a difference of two O(N) sums,
+    so near-zero differences are measured against the rounding of Omega itself."""
     steps, B = t64["acc"].shape
-    diffs, mism, ties = [], 0, 0
+    diffs, rels, mism, ties = [], [], 0, 0
     for c in range(B):
+        lz = t64["logz0"][c]
         for s in range(steps):
             if t64["acc"][s, c] != tmx["acc"][s, c]:
                 mism += 1
                 if abs(np.log(t64["u"][s, c]) - t64["logw"][s, c]) < 1e-9:
                     ties += 1
                 break
-            if np.isfinite(t64["logw"][s, c]) and t64["logz"][s, c] != 0.0:
-                diffs.append(abs(tmx["logw"][s, c] - t64["logw"][s, c]) / beta[c])
+            lzn = t64["logz"][s, c]
+            if np.isfinite(t64["logw"][s, c]) and lzn != 0.0:
+                d = abs(tmx["logw"][s, c] - t64["logw"][s, c]) / beta[c]
+                d_om = abs(lzn - lz) / beta[c]
+                diffs.append(d)
+                rels.append(d / max(d_om, EPS * abs(lzn) / beta[c]))
+            if t64["acc"][s, c] and lzn != 0.0:
+                lz = lzn
     d = np.array(diffs) if diffs else np.zeros(1)
+    r = np.array(rels) if rels else np.zeros(1)
     return {"moves_compared": len(diffs), "dOmega_max_abs_diff_vs_fp64": float(d.max()),
-            "dOmega_mean_abs_diff_vs_fp64": float(d.mean()), "accept_mismatches": mism,
+            "dOmega_mean_abs_diff_vs_fp64": float(d.mean()), "dOmega_max_rel_err_vs_fp64": float(r.max()),
+            "dOmega_err_definition": DOMEGA_ERR_DEF, "accept_mismatches": mism,
             "accept_mismatches_at_near_ties": ties}
 
 
@@ -257,7 +292,9 @@ def run_chains(fk, ctx, f, beta, U, seeds, precision, steps, trace=False, warmup
     ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
     ch.set_precision(precision)
     if trace:
+        lz0 = ch.state()["logZ"].copy()
         tr = ch.sweep(steps, trace=True)
+        tr["logz0"] = lz0
         ch.close()
         return tr
     if warmup:
@@ -606,18 +643,55 @@ def c5_block(fk, steps, check_steps, every=100):
 
 
 def cpu_baseline(L, seconds, ch, ctx, f, beta, U, seeds):
-    """Oracle (C++ restatement of the reference CPU path) on the host cores, and
-    the dOmega error of a short traced device sweep against it."""
+    """Oracle (the C++ restatement of the reference CPU path: Householder + implicit QR per
+    proposal, FP64, libstdc++ draws) on the host cores, on the batched FP64 workload: one
+    chain per thread, the chains spread evenly over the batch's beta / U mix, addremove /
+    nearest-neighbour exchange 0.5 / 0.5; a bounded sample of about `seconds`.  Beside it
+    the single-chain cubic2d L=8 case and a cubic3d L=8 sample, and the dOmega of a short
+    traced device sweep against the oracle chain of the same seeds."""
     from oracle import oracle as orc
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    # calibrate one move, then size the sample to about `seconds` of CPU work per thread
+    nproc = os.cpu_count() or 1
+    # one GPU's share of the box's CPUs (16 threads per GPU) bounds the worker pool
+    threads = max(1, min(nproc, 16))
+    B = len(beta)
+    idx = np.round(np.linspace(0, B - 1, threads)).astype(int)
+    moves = [(orc.MOVE_ADDREMOVE, 0.5), (orc.MOVE_HOP, 0.5)]
+
+    def run(sel, nmoves):
+        t = time.perf_counter()
+        tot, ns = orc.bench_chain_set(orc.KIND_HYPERCUBIC, 2, L, beta[sel], U[sel], U[sel] / 2, f[sel], seeds[sel],
+                                      moves, nmoves, t=1.0, tp=0.0)
+        return tot, ns, time.perf_counter() - t
+
+    # single core first (also calibrates the sample size)
+    tot1, ns1, dt1 = run(idx[:1], 24)
+    tot1, ns1, dt1 = run(idx[:1], max(24, int(min(3.0, seconds / 4) / (dt1 / 24))))
+    per_move = dt1 / tot1
+    nmoves = max(8, int(seconds / per_move))
+    tot, ns, dt = run(idx, nmoves)
+    out = {"value": round(tot / dt, 3), "unit": "moves/s", "cores": threads, "host_nproc": nproc, "kind": "port",
+           "sample": f"{threads} chains (beta, U of chains {idx[0]}..{idx[-1]} of the batch) x {nmoves} moves, "
+                     f"cubic2d L={L}, addremove/nn-exchange 0.5/0.5, FP64 Householder+implicit QR per proposal "
+                     f"({dt:.1f} s)",
+           "eigensolves_per_s": round(ns / dt, 3),
+           "single_core_moves_per_s": round(tot1 / dt1, 3), "single_core_eigensolves_per_s": round(ns1 / dt1, 3)}
+    # the single-chain workload: cubic2d L=8, U=1, beta=1, mu=0.5, addremove, one core
+    rng = np.random.default_rng(8)
+    f8 = (rng.random((1, 64)) < 0.5).astype(np.int32)
     t = time.perf_counter()
-    orc.bench_chains(orc.KIND_HYPERCUBIC, 2, L, 2.0, 2.0, 1.0, 99, 1, 3, t=1.0, tp=0.0)
-    per_move = (time.perf_counter() - t) / 3
-    nmoves = max(5, int(seconds / per_move))
+    tc1, _ = orc.bench_chain_set(orc.KIND_HYPERCUBIC, 2, 8, [1.0], [1.0], [0.5], f8, [7],
+                                 [(orc.MOVE_ADDREMOVE, 1.0)], 2000, t=1.0, tp=0.0)
+    out["single_chain_cubic2d_L8_moves_per_s"] = round(tc1 / (time.perf_counter() - t), 1)
+    # cubic3d L=8 (N=512): U in {2,5} x beta in {2,10}, filling 0.5, one chain per thread
+    b3 = np.array([2.0, 10.0] * (threads // 2 + 1))[:threads]
+    u3 = np.array([2.0, 2.0, 5.0, 5.0] * (threads // 4 + 1))[:threads]
+    f3 = (rng.random((threads, 512)) < 0.5).astype(np.int32)
     t = time.perf_counter()
-    done = orc.bench_chains(orc.KIND_HYPERCUBIC, 2, L, 2.0, 2.0, 1.0, 99, threads, nmoves, t=1.0, tp=0.0)
-    dt = time.perf_counter() - t
+    t3, n3 = orc.bench_chain_set(orc.KIND_HYPERCUBIC, 3, 8, b3, u3, u3 / 2, f3, np.arange(threads) + 11, moves, 6,
+                                 t=1.0, tp=0.0)
+    dt3 = time.perf_counter() - t
+    out["cubic3d_L8"] = {"moves_per_s": round(t3 / dt3, 2), "eigensolves_per_s": round(n3 / dt3, 2),
+                         "cores": threads, "sample": f"{threads} chains x 6 moves ({dt3:.1f} s)"}
     # dOmega = -(logZ_new - logZ_old)/beta of proposals: device trace vs oracle chain
     import fk_mc_amd as fk
     nchk, msteps = 4, 40
@@ -631,20 +705,19 @@ def cpu_baseline(L, seconds, ch, ctx, f, beta, U, seeds):
                             [(orc.MOVE_ADDREMOVE, 0.5), (orc.MOVE_HOP, 0.5)], msteps, t=1.0, tp=0.0,
                             forced=tr["acc"][:, c].astype(np.int32))
         solved = ref["logz"] != 0.0
-        d_gpu = -(tr["logz"][solved, c] - np.concatenate([[0], ref["logz_cur"][:-1]])[solved]) / beta[c]
-        d_ref = -(ref["logz"][solved] - np.concatenate([[0], ref["logz_cur"][:-1]])[solved]) / beta[c]
+        lz_old = np.concatenate([[0], ref["logz_cur"][:-1]])
+        d_gpu = -(tr["logz"][solved, c] - lz_old[solved]) / beta[c]
+        d_ref = -(ref["logz"][solved] - lz_old[solved]) / beta[c]
+        om = -ref["logz"][solved] / beta[c]
         if solved[0]:
-            d_gpu, d_ref = d_gpu[1:], d_ref[1:]
-        # relative to max(|dOmega|, 1e-6) so that near-zero differences do not blow up
-        scale = np.maximum(np.abs(d_ref), 1e-6)
+            d_gpu, d_ref, om = d_gpu[1:], d_ref[1:], om[1:]
+        scale = np.maximum(np.abs(d_ref), EPS * np.abs(om))
         errs.append(np.max(np.abs(d_gpu - d_ref) / scale) if len(d_ref) else 0.0)
         abs_errs.append(np.max(np.abs(d_gpu - d_ref)) if len(d_ref) else 0.0)
-    return {"value": round(done / dt, 3), "unit": "moves/s", "cores": threads, "kind": "port",
-            "sample": f"{threads} chains x {nmoves} addremove moves, cubic2d L={L}, beta=2, U=2, "
-                      f"FP64 Householder+implicit QR per move ({dt:.1f} s)",
-            "single_core_moves_per_s": round(1.0 / per_move, 3),
-            "dOmega_max_rel_err_vs_fp64_oracle": float(max(errs)),
-            "dOmega_max_abs_err_vs_fp64_oracle": float(max(abs_errs))}
+    out.update({"dOmega_max_rel_err_vs_fp64_oracle": float(max(errs)),
+                "dOmega_max_abs_err_vs_fp64_oracle": float(max(abs_errs)),
+                "dOmega_err_definition": DOMEGA_ERR_DEF})
+    return out
 
 
 if __name__ == "__main__":

@@ -22,12 +22,15 @@ namespace fks {
 constexpr int WAVE = 64;
 
 // Cross-lane moves on the VALU (no LDS traffic): DPP within 16-lane rows,
-// v_permlane16/32_swap across rows.  A double moves as two dwords.
+// v_permlane16/32_swap across rows.  A double moves as two dwords.  bound_ctrl
+// writes 0 into a lane whose source is invalid, so no "old" value has to be
+// materialised first (two v_mov per move fewer than an update with old = 0, same
+// result: every control used here reads a valid lane of the same row anyway).
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
     int2 w = *reinterpret_cast<int2*>(&v);
-    w.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, 0xF, 0xF, false);
-    w.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, 0xF, 0xF, false);
+    w.x = __builtin_amdgcn_mov_dpp(w.x, CTRL, 0xF, 0xF, true);
+    w.y = __builtin_amdgcn_mov_dpp(w.y, CTRL, 0xF, 0xF, true);
     return *reinterpret_cast<double*>(&w);
 }
 constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]

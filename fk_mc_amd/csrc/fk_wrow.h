@@ -165,13 +165,15 @@ struct wrow_solver {
                         sm.rs[w + 16 * a] = rsum;
                     }
                 }
-                // column sums of the blocks strictly below the diagonal blocks
+                // column sums of the blocks strictly below the diagonal blocks (rows
+                // i <= k, where v_i = 0, skipped per wave)
 #pragma unroll
                 for (int b = 0; b < NBC; ++b) {
                     double s = 0.0;
                     if (b >= bmin) {
 #pragma unroll
                         for (int a = 4 * (b + 1); a < NA; ++a) {
+                            if (w + 16 * a <= k) continue;  // wave-uniform
                             const double vr = readlane_d(vc[a >> 2], (w + 16 * a) & 63);
                             s = fma(A[sl(a, b)], vr, s);
                         }
@@ -208,7 +210,8 @@ struct wrow_solver {
                     if (g < bmin) continue;
 #pragma unroll
                     for (int a = 4 * g; a < 4 * g + 4; ++a) {
-                        // row values by broadcast LDS reads, w_i formed as the lanes form w_j
+                        // row values by broadcast LDS reads, w_i formed as the lanes form w_j;
+                        // a finished row (i <= k: v_i = w_i = 0) is left as it is (per wave)
                         const int i = w + 16 * a;
                         const double vra = sm.v[i];
                         const double wra = fma(-coef, vra, sm.p[i]);

@@ -879,4 +879,35 @@ long orc_bench_chains(int kind, int D, int L, double t, double tp, double beta, 
     return tot;
 }
 
+// Chains with their own (beta, U, mu_c = mu_f = mu, starting f, seed) and move mix, one
+// per thread, nmoves moves each: the CPU figure of a batched workload.  Returns the total
+// number of moves; *nsolves_out (nullable) the number of eigensolves they needed.
+long orc_bench_chain_set(int kind, int D, int L, double t, double tp, int nchains, const double* beta,
+                         const double* U, const double* mu, const int* f0, const uint32_t* seeds, int nkinds,
+                         const int* kinds, const double* probs, int nmoves, long* nsolves_out) {
+    lattice l;
+    l.kind = kind; l.D = D; l.L = L; l.t = t; l.tp = tp;
+    try { build_lattice(l); } catch (...) { return -1; }
+    std::vector<std::thread> th;
+    std::vector<long> done(nchains, 0), solves(nchains, 0);
+    for (int k = 0; k < nchains; ++k)
+        th.emplace_back([&, k]() {
+            chain ch(&l, beta[k], U[k], mu[k], mu[k], seeds[k]);
+            ch.cur.f.assign(f0 + (size_t)k * l.n, f0 + (size_t)(k + 1) * l.n);
+            for (int m = 0; m < nkinds; ++m) ch.add_move(kinds[m], probs[m]);
+            ch.cur.calc_ed(false);
+            for (int s = 0; s < nmoves; ++s) {
+                attempt_rec r; int kd; double u;
+                ch.step(-1, &kd, r, &u);
+                solves[k] += r.solved;
+            }
+            done[k] = nmoves;
+        });
+    for (auto& x : th) x.join();
+    long tot = 0, ns = 0;
+    for (int k = 0; k < nchains; ++k) { tot += done[k]; ns += solves[k]; }
+    if (nsolves_out) *nsolves_out = ns;
+    return tot;
+}
+
 }  // extern "C"

@@ -69,6 +69,10 @@ def lib():
         L.orc_bench_chains.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_uint32, C.c_int, C.c_int]
         L.orc_bench_chains.restype = C.c_long
+        L.orc_bench_chain_set.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _d, _d, _d,
+                                          _i, C.POINTER(C.c_uint32), C.c_int, _i, _d, C.c_int,
+                                          C.POINTER(C.c_long)]
+        L.orc_bench_chain_set.restype = C.c_long
         _LIB = L
     return _LIB
 
@@ -254,3 +258,23 @@ def calc_chebyshev(kind, D, L, f, beta, U, mu_c, max_moment, grid, t=1.0, tp=1.0
 
 def bench_chains(kind, D, L, beta, U, mu, seed, nthreads, nmoves, t=1.0, tp=1.0):
     return lib().orc_bench_chains(kind, D, L, t, tp, beta, U, mu, seed, nthreads, nmoves)
+
+
+def bench_chain_set(kind, D, L, beta, U, mu, f0, seeds, moves, nmoves, t=1.0, tp=1.0):
+    """One chain per thread (chain k: beta[k], U[k], mu[k], f0[k], seeds[k]), nmoves moves
+    each; returns (total moves, eigensolves)."""
+    n = len(beta)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    u = np.ascontiguousarray(U, dtype=np.float64)
+    m = np.ascontiguousarray(mu, dtype=np.float64)
+    f = np.ascontiguousarray(f0, dtype=np.int32)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+    kinds = np.array([x[0] for x in moves], dtype=np.int32)
+    probs = np.array([x[1] for x in moves], dtype=np.float64)
+    ns = C.c_long(0)
+    tot = lib().orc_bench_chain_set(kind, D, L, t, tp, n, _pd(b), _pd(u), _pd(m), _pi(f),
+                                    sd.ctypes.data_as(C.POINTER(C.c_uint32)), len(kinds), _pi(kinds), _pd(probs),
+                                    nmoves, C.byref(ns))
+    if tot < 0:
+        raise RuntimeError("oracle bench failed")
+    return int(tot), int(ns.value)

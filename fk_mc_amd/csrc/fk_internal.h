@@ -52,6 +52,8 @@ struct sweep_args {
     int kinds[4];          // move kind of each registered move
     int nW;                // f-f interaction range (1-D lattices only)
     double W[8];
+    int flip_ff;           // the global exchange's weight includes -beta dE_ff (the Chebyshev
+                           // move_flip, src/moves_chebyshev.cpp:22; the ED one has none)
     int trace;             // record the per-move trace
     int step;              // trace row
     int32_t* tr_kind;

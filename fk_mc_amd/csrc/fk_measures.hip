@@ -299,13 +299,16 @@ __global__ void __launch_bounds__(CM_T) k_chain_measure(chain_meas_args a) {
     const size_t row = (size_t)b * N;
     const size_t hrow = ((size_t)b * a.cap + (size_t)(rec ? m : 0)) * N;
     double ec = 0.0, d2 = 0.0;
+    const bool spec = a.spectrum != nullptr;  // uniform: null when only occupations are measured
     for (int k = tid; k < N; k += CM_T) {
-        const double e = a.spectrum[row + k];
-        const double x = exp(beta * e);
-        ec += e / (1.0 + x);
-        d2 += e * e / (1.0 + 0.5 * (x + 1.0 / x));
-        a.spec_mean[row + k] = running_mean(a.spec_mean[row + k], z, e);
-        if (rec && a.spec_hist) a.spec_hist[hrow + k] = e;
+        if (spec) {
+            const double e = a.spectrum[row + k];
+            const double x = exp(beta * e);
+            ec += e / (1.0 + x);
+            d2 += e * e / (1.0 + 0.5 * (x + 1.0 / x));
+            a.spec_mean[row + k] = running_mean(a.spec_mean[row + k], z, e);
+            if (rec && a.spec_hist) a.spec_hist[hrow + k] = e;
+        }
         if (a.ipr && rec && a.ipr_hist) a.ipr_hist[hrow + k] = a.ipr[row + k];
         const uint8_t fi = a.f[row + k];
         a.focc_sum[row + k] += fi;
@@ -338,7 +341,12 @@ __global__ void __launch_bounds__(CM_T) k_chain_measure(chain_meas_args a) {
         red[1][w] = d2;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0 && !spec && rec) {  // occupations only: the energy records are zero
+        a.e[(size_t)b * a.cap + m] = 0.0;
+        a.d2e[(size_t)b * a.cap + m] = 0.0;
+        a.ce[(size_t)b * a.cap + m] = 0.0;
+    }
+    if (tid == 0 && spec) {
         ec = 0.0;
         d2 = 0.0;
         for (int i = 0; i < CM_T / WAVE; ++i) {

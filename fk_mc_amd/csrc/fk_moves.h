@@ -83,7 +83,9 @@ __device__ __noinline__ move_prop propose_move(uint32_t* mt, uint8_t* f, int N, 
     }
     if (a.nW > 0 && p.solve) ff_new = ff_energy_dev(f, N, a.nW, a.W);
     p.u = fkrng::canonical(mt);
-    p.ffdiff = ff_new - ff_old;
+    // the ED flip weighs no f-f term (src/moves.cpp:19-20), the Chebyshev one does
+    // (src/moves_chebyshev.cpp:22)
+    p.ffdiff = (p.kind == FKGPU_MOVE_FLIP && !a.flip_ff) ? 0.0 : ff_new - ff_old;
     return p;
 }
 
@@ -104,9 +106,9 @@ __device__ __noinline__ int decide_move(const move_prop& p, int f_new_sa, double
         w = (f_new_sa ? ratio * ebmf : ratio / ebmf) * exp(-beta * p.ffdiff);
         logw = (lz_new - lz_old) + (f_new_sa ? 1.0 : -1.0) * beta * mu_f - beta * p.ffdiff;
     } else if (p.kind == FKGPU_MOVE_FLIP) {
-        // src/moves.cpp:19-20
-        w = exp(lz_new - lz_old);
-        logw = lz_new - lz_old;
+        // src/moves.cpp:19-20 (ffdiff 0), src/moves_chebyshev.cpp:22
+        w = exp(lz_new - lz_old - beta * p.ffdiff);
+        logw = lz_new - lz_old - beta * p.ffdiff;
     } else if (p.kind == FKGPU_MOVE_HOP) {
         w = exp(lz_new - lz_old) * exp(-beta * p.ffdiff);
         logw = lz_new - lz_old - beta * p.ffdiff;

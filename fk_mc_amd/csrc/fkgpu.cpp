@@ -1427,7 +1427,10 @@ static hipError_t launch_step(fkgpu_chains* ch, const sweep_args& a, hipStream_t
         return fk_launch_kpm_accept(a, ch->k_props, ch->k_fprop, ch->mx->lz, st, ch->mx->ev);
     }
     if (!ch->kpm) return fk_launch_sweep(ch->kind, a, ch->grid, st);
-    hipError_t e = fk_launch_kpm_propose(a, ch->k_props, ch->k_fprop, ch->k_skip, st);
+    // the moves of src/moves_chebyshev.cpp: their global exchange includes -beta dE_ff
+    sweep_args ak = a;
+    ak.flip_ff = 1;
+    hipError_t e = fk_launch_kpm_propose(ak, ch->k_props, ch->k_fprop, ch->k_skip, st);
     if (e != hipSuccess) return e;
     if (ch->kdelta == FKGPU_KPM_DELTA_LOCAL) {
         e = fk_launch_kpm_local(ch->B, ch->N, ch->kl_M, ch->kl_n0, ch->f, ch->k_fprop, ch->k_props, ch->k_skip, ch->U,
@@ -1946,6 +1949,7 @@ static int meas_prepare(fkgpu_chains* ch) {
     fkgpu_ctx* ctx = ch->ctx;
     chain_meas_bufs* m = ch->ms;
     if (!m) return fail(ctx, FKGPU_EINVAL, "measurements are not enabled on this chain set");
+    if (m->flags & FKGPU_MEASURE_NO_SPECTRUM) return FKGPU_OK;  // occupations only: no eigensolve
     if (!ch->kpm && !ch->dense_ok)
         return fail(ctx, FKGPU_EUNSUPPORTED, "no dense solver for this size: enable the stochastic trace first");
     const bool ipr = m->flags & (FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS | FKGPU_MEASURE_EIGENFUNCTIONS);
@@ -1980,8 +1984,9 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
     const double* spec = ch->spectrum;
     const double* V = (ipr && ch->fu) ? ch->fu_V : nullptr;
     const double* specV = ch->spectrum;  // the eigenvalues that belong to V
+    const bool nospec = m->flags & FKGPU_MEASURE_NO_SPECTRUM;
     hipError_t e;
-    if (ch->kpm || (ipr && !ch->fu)) {
+    if (!nospec && (ch->kpm || (ipr && !ch->fu))) {
         if ((e = fk_launch_f8to32(B * N, ch->f, m->f32, st)) != hipSuccess) return e;
         eig_args a{};
         a.B = B;
@@ -2028,7 +2033,7 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
     for (int i = 0; i < 8; ++i) a.W[i] = ch->args.W[i];
     a.f = ch->f;
     a.nf = ch->nf;
-    a.spectrum = spec;
+    a.spectrum = nospec ? nullptr : spec;
     a.beta = ch->beta;
     a.mu_f = ch->mu_f;
     a.ipr = (V && (m->flags & FKGPU_MEASURE_IPR)) ? m->ipr : nullptr;
@@ -2069,8 +2074,10 @@ int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
     const int vecs = FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS | FKGPU_MEASURE_EIGENFUNCTIONS;
-    const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | vecs;
+    const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | vecs | FKGPU_MEASURE_NO_SPECTRUM;
     if (capacity < 0 || (flags & ~all)) return fail(ctx, FKGPU_EINVAL, "invalid capacity or measurement flags");
+    if ((flags & FKGPU_MEASURE_NO_SPECTRUM) && (flags & (vecs | FKGPU_MEASURE_SPECTRUM_HISTORY)))
+        return fail(ctx, FKGPU_EINVAL, "FKGPU_MEASURE_NO_SPECTRUM records occupations only");
     if ((flags & vecs) && (!ch->dense_ok || ch->N > 1024))
         return fail(ctx, FKGPU_EUNSUPPORTED, "IPR, stiffness and eigenfunctions need eigenvectors, implemented for N <= 1024");
     if ((flags & FKGPU_MEASURE_STIFFNESS) && (ctx->lat_kind != FKGPU_LATTICE_HYPERCUBIC || ctx->lat_ndim < 2))

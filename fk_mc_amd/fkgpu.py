@@ -22,6 +22,7 @@ UPDATE_DENSE, UPDATE_RANK1 = 0, 1
 KPM_DELTA_LOCAL, KPM_DELTA_STOCHASTIC = 0, 1
 MEASURE_SPECTRUM_HISTORY, MEASURE_FOCC_HISTORY, MEASURE_IPR, MEASURE_STIFFNESS = 1, 2, 4, 8
 MEASURE_EIGENFUNCTIONS = 16
+MEASURE_NO_SPECTRUM = 32
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)

@@ -103,7 +103,7 @@ params define_parameters() {
     p.define("mc_flip", "0", "Make flip moves");
     p.define("mc_add_remove", "1", "Make add/remove moves");
     p.define("mc_reshuffle", "0", "Make reshuffle moves");
-    p.define("cheb_moves", "false", "Allow moves using Chebyshev sampling (not in the batched program)");
+    p.define("cheb_moves", "false", "Allow moves using Chebyshev sampling");
     p.define("cheb_prefactor", "2.2", "Prefactor for number of Chebyshev polynomials = #ln(Volume)");
     p.define("measure_history", "true", "Measure the history");
     p.define("Nf_start", "5", "Starting number of f-electrons");
@@ -306,8 +306,15 @@ int run(params& p) {
     else if (lat == "cubic3d") D = 3;
     else if (lat == "triangular") kind = FKGPU_LATTICE_TRIANGULAR;
     else throw std::runtime_error("unknown lattice " + lat);
-    if (p.b("cheb_moves"))
-        throw std::runtime_error("cheb_moves is not part of the batched program (use the host layer)");
+    // include/fk_mc/fk_mc.hxx:59-78: the Chebyshev moves replace the ED ones; their weights
+    // here are the exact dlogZ of the local estimator (fkgpu_chains_set_kpm_delta), which
+    // covers moves of one or two sites
+    const bool cheb = p.b("cheb_moves");
+    if (cheb && p.d("mc_reshuffle") > 2.220446049250313e-16)
+        throw std::runtime_error("cheb_moves: the batched program's Chebyshev weights cover moves of one or two "
+                                 "sites (flip, add_remove, mc_hop), not mc_reshuffle");
+    if (cheb && (p.str("precision") == "mixed" || p.str("update") == "rank1"))
+        throw std::runtime_error("cheb_moves replaces the dense eigensolve: no --precision mixed / --update rank1");
     const int L = (int)p.i("L"), R = (int)p.i("chains");
     const long nsweeps = p.i("nsweeps"), sweep_len = p.i("sweep_len"), ntherm = p.i("ntherm_sweeps");
     if (R < 1 || nsweeps < 0 || sweep_len < 1 || ntherm < 0) throw std::runtime_error("invalid chains / sweeps");
@@ -317,6 +324,9 @@ int run(params& p) {
     const bool ipr = p.b("measure_ipr"), eig = p.b("measure_eigenfunctions");
     // prog/fk_mc_exec.cpp:62: the IPR and the eigenfunctions switch the histories on
     const bool history = p.b("measure_history") || ipr || eig;
+    // include/fk_mc/fk_mc.hxx:115-124: Chebyshev runs measure the energy and the spectrum only
+    // with measure_ipr (calc_spectrum); without the spectrum measures only occupations are kept
+    const bool spectra = !cheb || ipr || eig || p.b("measure_stiffness");
 
     fkgpu_ctx* ctx = nullptr;
     if (fkgpu_create((int)p.i("device"), &ctx) != FKGPU_OK) throw std::runtime_error("no HIP device");
@@ -350,6 +360,14 @@ int run(params& p) {
         FK_CHECK(fkgpu_chains_set_ff(ch, (int)W.size(), W.data()), ctx);
     }
     if (p.str("precision") == "mixed") FK_CHECK(fkgpu_chains_set_precision(ch, FKGPU_PRECISION_MIXED), ctx);
+    if (cheb) {
+        // chebyshev_eval(cheb_size, grid) of fk_mc.hxx:62-65 sets the moment count of the chains'
+        // stochastic logZ estimate (8 probes; no kernel damping, as calc_chebyshev); the
+        // proposals' weights come from the local estimator, which needs neither
+        int cheb_size = (int)(std::log((double)N) * p.d("cheb_prefactor"));
+        cheb_size += cheb_size % 2;
+        FK_CHECK(fkgpu_chains_set_kpm(ch, 8, std::max(2, std::min(512, cheb_size)), (uint64_t)seed, 0), ctx);
+    }
     if (p.str("update") == "rank1") FK_CHECK(fkgpu_chains_set_update(ch, FKGPU_UPDATE_RANK1, (int)p.i("refresh")), ctx);
     // conductivity grid of prog/fk_mc_exec.cpp:127-139: 2 cond_npoints + 1 frequencies, log-spaced
     // from wmax e^-15 to wmax = max(8, 2U) on either side of 0
@@ -367,6 +385,7 @@ int run(params& p) {
     }
     int flags = (history ? FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY : 0) | (ipr ? FKGPU_MEASURE_IPR : 0) |
                 (stiff ? FKGPU_MEASURE_STIFFNESS : 0) | (eig ? FKGPU_MEASURE_EIGENFUNCTIONS : 0);
+    if (!spectra) flags = (history ? FKGPU_MEASURE_FOCC_HISTORY : 0) | FKGPU_MEASURE_NO_SPECTRUM;
     FK_CHECK(fkgpu_chains_measure_enable(ch, (int)nsweeps, flags), ctx);
 
     const bool show = p.b("show_output");
@@ -391,10 +410,11 @@ int run(params& p) {
     std::vector<double> e(R * S), d2(R * S), ce(R * S), em(R), d2m(R), spec(RN), focc(RN);
     FK_CHECK(fkgpu_chains_energies(ch, &count, e.data(), d2.data(), ce.data()), ctx);
     FK_CHECK(fkgpu_chains_means(ch, &count, em.data(), d2m.data(), spec.data(), focc.data()), ctx);
-    std::vector<double> sh(history ? R * S * N : 0), ih(ipr ? R * S * N : 0);
+    const bool shist = history && spectra;
+    std::vector<double> sh(shist ? R * S * N : 0), ih(ipr ? R * S * N : 0);
     std::vector<int32_t> fh(history ? R * S * N : 0);
     if (history || ipr)
-        FK_CHECK(fkgpu_chains_histories(ch, &count, history ? sh.data() : nullptr, history ? fh.data() : nullptr,
+        FK_CHECK(fkgpu_chains_histories(ch, &count, shist ? sh.data() : nullptr, history ? fh.data() : nullptr,
                                         ipr ? ih.data() : nullptr),
                  ctx);
     const size_t NW = wgrid.size();
@@ -409,11 +429,11 @@ int run(params& p) {
     for (int i = 0; i < npts; ++i) grid[i] = -wd + 2.0 * wd * i / (1.0 * npts);
     // per-record DOS (and IPR(w) with measure_ipr) on the grid and at w = 0 (last column), for
     // the error bars of save_glocal and save_ipr (prog/data_save.hxx:283-328,486-532)
-    const int nrec = history ? (int)std::min<int64_t>(count, (int64_t)S) * R : 0;
+    const int nrec = shist ? (int)std::min<int64_t>(count, (int64_t)S) * R : 0;
     std::vector<double> grid0(grid);
     grid0.push_back(0.0);
     std::vector<double> drec, irec;
-    {
+    if (spectra) {
         const std::vector<double>& src = nrec > 0 ? sh : spec;
         const int ns = (int)(src.size() / N);
         std::vector<double> d((size_t)ns * (npts + 1));
@@ -449,12 +469,14 @@ int run(params& p) {
     if (out.size() > 3 && out.substr(out.size() - 3) == ".h5") out = out.substr(0, out.size() - 3);
     mkdirs(out + "/mc_data");
     const std::string md = out + "/mc_data/";
-    write_npy(md + "energies.npy", e.data(), {(size_t)R, S});
-    write_npy(md + "d2energies.npy", d2.data(), {(size_t)R, S});
-    write_npy(md + "c_energies.npy", ce.data(), {(size_t)R, S});
-    write_npy(md + "spectrum.npy", spec.data(), {(size_t)R, (size_t)N});
+    if (spectra) {
+        write_npy(md + "energies.npy", e.data(), {(size_t)R, S});
+        write_npy(md + "d2energies.npy", d2.data(), {(size_t)R, S});
+        write_npy(md + "c_energies.npy", ce.data(), {(size_t)R, S});
+        write_npy(md + "spectrum.npy", spec.data(), {(size_t)R, (size_t)N});
+    }
     if (history) {
-        write_npy(md + "spectrum_history.npy", sh.data(), {(size_t)R, S, (size_t)N});
+        if (shist) write_npy(md + "spectrum_history.npy", sh.data(), {(size_t)R, S, (size_t)N});
         std::vector<double> fd(fh.begin(), fh.end());
         write_npy(md + "focc_history.npy", fd.data(), {(size_t)R, S, (size_t)N});
     }
@@ -468,8 +490,8 @@ int run(params& p) {
     }
     std::vector<double> dg(2 * npts);
     for (int i = 0; i < npts; ++i) dg[2 * i] = grid[i], dg[2 * i + 1] = dos[i];
-    write_npy(out + "/dos.npy", dg.data(), {(size_t)npts, 2});
-    if (p.b("plaintext")) {
+    if (spectra) write_npy(out + "/dos.npy", dg.data(), {(size_t)npts, 2});
+    if (p.b("plaintext") && spectra) {
         write_txt(out + "/energies.dat", e.data(), R, S);
         write_txt(out + "/dos.dat", dg.data(), npts, 2);
     }
@@ -505,8 +527,10 @@ int run(params& p) {
         if (show) std::printf("  %-12s %.10g +- %.3g\n", name, me.mean, me.err);
     };
     st << "# means over " << R << " chains x " << count << " measurements; errors from the spread over chains\n";
-    line("energy/N", eN);
-    line("cv", cv);
+    if (spectra) {
+        line("energy/N", eN);
+        line("cv", cv);
+    }
     line("nf/N", nfm);
     line("acceptance", accr);
     if (stiff) line("stiffness", stfm);
@@ -514,7 +538,7 @@ int run(params& p) {
     // chains in chain order (its merge of the ranks' histories), binned from the last
     // measurement backwards: energies and the jackknifed specific heat, dos0 / dos_err and,
     // with measure_ipr, ipr0 / ipr_err
-    if (S > 0 && count > 0) {
+    if (S > 0 && count > 0 && spectra) {
         namespace bn = fkmc::binning;
         mkdirs(out + "/binning");
         mkdirs(out + "/stats");

@@ -310,6 +310,11 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
 #define FKGPU_MEASURE_IPR 4
 #define FKGPU_MEASURE_STIFFNESS 8 /* hypercubic lattices, D >= 2, N <= 1024 */
 #define FKGPU_MEASURE_EIGENFUNCTIONS 16 /* N <= 1024; capacity x N^2 doubles per chain */
+/* Occupations only (mean and, with FKGPU_MEASURE_FOCC_HISTORY, history): no eigensolve at a
+ * measurement, any N; the energy series are zero, the spectrum mean and history stay empty.  The
+ * reference's Chebyshev runs measure so without measure_ipr (include/fk_mc/fk_mc.hxx:115-124).
+ * Not with the spectrum history or the eigenvector measures. */
+#define FKGPU_MEASURE_NO_SPECTRUM 32
 /* Allocate the records and clear them (measurement count 0).  Replaces the
  * observables the reference's measures are registered on (include/fk_mc/fk_mc.hxx:97-123). */
 int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags);

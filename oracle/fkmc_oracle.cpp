@@ -352,6 +352,9 @@ enum { MOVE_ADDREMOVE = 0, MOVE_FLIP = 1, MOVE_HOP = 2, MOVE_RANDOMIZE = 3 };
 // g_chain_W is the f-f interaction W given to the chains orc_run_chain builds.
 static int g_mutation = 0;
 static std::vector<double> g_chain_W;
+// g_flip_ff: the global exchange weighs -beta dE_ff as the Chebyshev move_flip does
+// (src/moves_chebyshev.cpp:22); 0 (default) is the ED move_flip (src/moves.cpp:19-20).
+static int g_flip_ff = 0;
 
 struct attempt_rec {
     int site_a = -1, site_b = -1;
@@ -416,8 +419,9 @@ struct chain {
         prop.calc_ed(false);
         r.solved = 1;
         r.logz_new = prop.logZ;
-        r.logw = prop.logZ - cur.logZ;
-        return std::exp(prop.logZ - cur.logZ);
+        const double ff_diff = g_flip_ff ? prop.ff_energy() - cur.ff_energy() : 0.0;
+        r.logw = prop.logZ - cur.logZ - cur.beta * ff_diff;
+        return std::exp(prop.logZ - cur.logZ - cur.beta * ff_diff);
     }
     // Nearest-neighbour exchange: site i ~ U{0..V-1}, neighbour slot ~ U{0..z_i-1};
     // the pair is swapped when the occupations differ, otherwise it is a null move
@@ -626,6 +630,7 @@ int orc_run_chain(int kind, int D, int L, double t, double tp,
 }
 
 void orc_set_mutation(int m) { g_mutation = m; }
+void orc_set_flip_ff(int on) { g_flip_ff = on; }
 void orc_set_chain_ff(int nW, const double* W) { g_chain_W.assign(W, W + nW); }
 
 // Raw libstdc++ draws, for pinning the device port of the generator and

@@ -54,6 +54,8 @@ def lib():
                                     _i, _i, _i, _d, _d, _d, _i, _i, _d, _i, _d]
         L.orc_set_mutation.argtypes = [C.c_int]
         L.orc_set_mutation.restype = None
+        L.orc_set_flip_ff.argtypes = [C.c_int]
+        L.orc_set_flip_ff.restype = None
         L.orc_set_chain_ff.argtypes = [C.c_int, _d]
         L.orc_set_chain_ff.restype = None
         L.orc_rng_draws.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, _d, C.c_int, _d]
@@ -192,6 +194,12 @@ def set_mutation(m):
     exchange, for tests that show their checks catch it (see g_mutation in
     fkmc_oracle.cpp)."""
     lib().orc_set_mutation(int(m))
+
+
+def set_flip_ff(on):
+    """Global exchange with (1) or without (0, default) the f-f term in its weight: the
+    Chebyshev move_flip (src/moves_chebyshev.cpp:22) or the ED one (src/moves.cpp:19-20)."""
+    lib().orc_set_flip_ff(int(on))
 
 
 def set_chain_ff(W):

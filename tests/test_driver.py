@@ -291,3 +291,41 @@ def test_driver_gwr_cubic3d(tmp_path, orc):
     gk = np.load(sd / f"gk_w{ws}_re.npy") + 1j * np.load(sd / f"gk_w{ws}_im.npy")
     np.testing.assert_allclose(gk, np.fft.fft2(r2), atol=1e-11 * np.abs(gk).max())
     assert not (md / "eig_history.npy").exists()  # only with save_eigenfunctions
+
+
+@pytest.mark.gpu
+def test_driver_cheb_moves(tmp_path, fk):
+    """cheb_moves (include/fk_mc/fk_mc.hxx:59-78): the moves run on the Chebyshev path, the
+    global exchange with its f-f term (src/moves_chebyshev.cpp:22), and without measure_ipr
+    only occupations are recorded (fk_mc.hxx:115-124).  Each chain equals a library chain set
+    built the same way: randomize_f(Nf_start) from the chain's generator, the moves in the
+    reference's order, Chebyshev weights, occupation-only measurements."""
+    L, R, seed, beta, U, mu = 24, 3, 11, 3.0, 2.0, 1.0
+    ns, sl, nt = 10, 4, 2
+    W = [0.0, 0.5, 0.2]
+    out = _run(tmp_path, "--lattice", "cubic1d", "--L", L, "--beta", beta, "--U", U, "--mu_c", mu, "--mu_f", mu,
+               "--Nf_start", L // 2, "--nsweeps", ns, "--sweep_len", sl, "--ntherm_sweeps", nt, "--chains", R,
+               "--seed", seed, "--cheb_moves", "true", "--mc_flip", 0.5, "--mc_add_remove", 0.5,
+               "--measure_history", "true", "--W", ",".join(str(x) for x in W))
+    md = out / "mc_data"
+    fh = np.load(md / "focc_history.npy")
+    assert fh.shape == (R, ns, L)
+    assert not (md / "energies.npy").exists() and not (md / "spectrum_history.npy").exists()
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 1, L, 1.0, 0.0)
+    ch = fk.Chains(ctx, np.zeros((R, L), np.int32), beta, U, mu, mu, np.arange(R, dtype=np.uint32) + seed)
+    ch.randomize_f(np.full(R, L // 2, np.int32))
+    ch.set_moves([(fk.MOVE_FLIP, 0.5), (fk.MOVE_ADDREMOVE, 0.5)])
+    ch.set_ff(W)
+    ch.set_kpm(R=8, M=8, seed=1, jackson=False)
+    ch.measure_enable(ns, fk.MEASURE_FOCC_HISTORY | fk.MEASURE_NO_SPECTRUM)
+    ch.sweep_graph(nt * sl)
+    ch.sweep_measure(ns, sl)
+    np.testing.assert_array_equal(ch.histories()["focc"], fh.astype(np.int32))
+    ch.close()
+    ctx.close()
+    # the mixed / rank-one modes and the reshuffle move are refused with Chebyshev moves
+    for extra in (["--mc_reshuffle", 0.1], ["--precision", "mixed"]):
+        cmd = [EXE, "--lattice", "cubic1d", "--L", L, "--cheb_moves", "true", "--nsweeps", 2, "--chains", 1,
+               "--output", str(tmp_path / "bad.h5"), "--show_output", "false"] + extra
+        r = subprocess.run([str(c) for c in cmd], capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and "cheb_moves" in r.stderr

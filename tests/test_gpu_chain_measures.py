@@ -227,7 +227,7 @@ def test_measure_errors(fk):
     with pytest.raises(fk.FkgpuError):
         ch.measure_enable(-1, 0)
     with pytest.raises(fk.FkgpuError):
-        ch.measure_enable(4, 32)
+        ch.measure_enable(4, 64)  # no such flag
     with pytest.raises(fk.FkgpuError):
         ch.measure_conductivity([0.0], -1.0)
     ch.measure_enable(4, 1)

@@ -239,6 +239,7 @@ def _compare_local_chain(orc, kind, D, L, beta, U, mu, seed, f0, moves, tr, tp, 
     nsteps = len(tr["u"])
     forced = -np.ones(nsteps, np.int32)
     orc.set_chain_ff(W)
+    orc.set_flip_ff(1)  # the Chebyshev move_flip weighs dE_ff (src/moves_chebyshev.cpp:22)
     try:
         for _attempt in range(8):
             ref = orc.run_chain(kind, D, L, beta, U, mu, mu, seed, f0, moves, nsteps, tp=tp, forced=forced)
@@ -259,6 +260,7 @@ def _compare_local_chain(orc, kind, D, L, beta, U, mu, seed, f0, moves, tr, tp, 
             forced[s] = tr["acc"][s]
     finally:
         orc.set_chain_ff(None)
+        orc.set_flip_ff(0)
     raise AssertionError("too many near-ties")
 
 
@@ -269,6 +271,7 @@ LOCAL_CASES = [
     ("triangular6", 1, 2, 6, 8.0, 3.0, "ah", None),
     ("cubic3d4", 0, 3, 4, 2.0, 5.0, "ah", None),
     ("chain24_ff", 0, 1, 24, 3.0, 2.0, "ah", [0.0, 0.5, 0.2]),
+    ("chain24_flip_ff", 0, 1, 24, 3.0, 2.0, "af", [0.0, 0.5, 0.2]),
 ]
 
 
@@ -385,3 +388,45 @@ def test_kpm_local_large_lattice(fk):
     assert checked >= 2
     ch.close()
     ctx.close()
+
+
+def test_occupation_only_measurements(fk):
+    """FKGPU_MEASURE_NO_SPECTRUM: occupations recorded without an eigensolve, also above
+    the dense range (cubic2d L=64 on the Chebyshev path); at a size with a dense solver
+    the occupation records equal those of a fully measured twin chain set."""
+    print("case occupation-only measures", flush=True)
+    for L, kpm in ((64, True), (12, False)):
+        n = L * L
+        rng = np.random.default_rng(L)
+        f0 = rng.integers(0, 2, (3, n)).astype(np.int32)
+        ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+        sets = []
+        for flags in (fk.MEASURE_FOCC_HISTORY | fk.MEASURE_NO_SPECTRUM,
+                      fk.MEASURE_FOCC_HISTORY | fk.MEASURE_SPECTRUM_HISTORY):
+            if kpm and not (flags & fk.MEASURE_NO_SPECTRUM):
+                continue
+            ch = fk.Chains(ctx, f0, 4.0, 2.0, 1.0, 1.0, [3, 4, 5])
+            ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
+            if kpm:
+                ch.set_kpm(R=8, M=32, seed=1, jackson=False)
+            ch.measure_enable(6, flags)
+            ch.sweep_measure(6, 2)
+            c, e, _, _ = ch.energies()
+            h = ch.histories()
+            st = ch.state()
+            assert c == 6
+            np.testing.assert_array_equal(h["focc"][:, -1], st["f"])
+            if flags & fk.MEASURE_NO_SPECTRUM:
+                assert np.all(e == 0.0)
+            sets.append((h["focc"], ch.means()["focc"]))
+            ch.close()
+        if len(sets) == 2:
+            np.testing.assert_array_equal(sets[0][0], sets[1][0])
+            np.testing.assert_array_equal(sets[0][1], sets[1][1])
+        with pytest.raises(fk.FkgpuError):
+            bad = fk.Chains(ctx, f0, 4.0, 2.0, 1.0, 1.0, [3, 4, 5])
+            try:
+                bad.measure_enable(4, fk.MEASURE_NO_SPECTRUM | fk.MEASURE_SPECTRUM_HISTORY)
+            finally:
+                bad.close()
+        ctx.close()

@@ -211,3 +211,19 @@ def test_addremove_hop_detailed_balance(orc, name):
     assert _tv(orc, name, MIXED, 400000, 23) < 0.05
     mutation = 2 if LATTICES[name][3] is not None else 1
     assert _tv(orc, name, MIXED, 400000, 23, mutation=mutation) > 0.08
+
+
+def test_chebyshev_flip_weighs_ff_energy(orc):
+    """The global exchange of src/moves_chebyshev.cpp:22 weighs -beta dE_ff, so on a 1-D
+    chain with f-f interaction flip + addremove samples exp(logZ + beta mu_f N_f - beta E_ff);
+    the ED move_flip (src/moves.cpp:19-20) omits the term and samples something else."""
+    name = "chain9_ff"
+    moves = [(orc.MOVE_ADDREMOVE, 0.3), (orc.MOVE_FLIP, 0.7)]
+    orc.set_flip_ff(1)
+    try:
+        tv_cheb = _tv(orc, name, moves, 300000, 31)
+    finally:
+        orc.set_flip_ff(0)
+    tv_ed = _tv(orc, name, moves, 300000, 31)
+    assert tv_cheb < 0.05
+    assert tv_ed > 0.08

@@ -322,9 +322,113 @@ __global__ void __launch_bounds__(KF_NT) k_kpm_finish(int B, int N, int R, int M
     if (tid == 0) logz[b] = part[0] * (double)N;
 }
 
+// ---- Lanczos for the extreme eigenvalues (the ARPACK calls of src/configuration.cpp:99-100) ----
+// One workgroup runs `iters` Lanczos steps on H = hop + diag(d) from its saved state
+// (vectors in global memory, L2-resident; a workgroup's barrier orders them):
+//   w = H v_j - beta_{j-1} v_{j-1};  alpha_j = <w, v_j>;  w -= alpha_j v_j;
+//   beta_j = |w|;  v_{j+1} = w / beta_j
+// state: v_prev [N], v_cur [N], w [N], and st[0] = beta_{j-1}, st[1] = j (as double).
+// alpha[j], beta[j] are appended for j = st[1] ... st[1] + iters - 1.  Without
+// reorthogonalisation the extreme Ritz values still converge to the extreme eigenvalues
+// (lost orthogonality only repeats converged values).  Fixed-order reductions.
+constexpr int LZ_NT = 1024;
+__device__ __forceinline__ double block_sum_lz(double v, double* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / WAVE;
+    __syncthreads();
+    if ((threadIdx.x & (WAVE - 1)) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < LZ_NT / WAVE; ++i) s += red[i];
+    return s;
+}
+__global__ void __launch_bounds__(LZ_NT)
+    k_lanczos(int N, const int32_t* __restrict__ nbr, const int32_t* __restrict__ z, int zmax,
+              const double* __restrict__ hopv, const double* __restrict__ diag, int iters, double* vp, double* vc,
+              double* w, double* st, double* __restrict__ alpha, double* __restrict__ beta) {
+    __shared__ double red[LZ_NT / WAVE];
+    const int tid = threadIdx.x;
+    double bprev = st[0];
+    int j = (int)st[1];
+    if (j == 0) {  // normalise the start vector
+        double pn = 0.0;
+        for (int i = tid; i < N; i += LZ_NT) pn = fma(vc[i], vc[i], pn);
+        const double inv = 1.0 / sqrt(block_sum_lz(pn, red));
+        for (int i = tid; i < N; i += LZ_NT) vc[i] *= inv;
+        __syncthreads();
+    }
+    for (int it = 0; it < iters; ++it, ++j) {
+        double pa = 0.0;
+        for (int i = tid; i < N; i += LZ_NT) {
+            double s = diag[i] * vc[i];
+            for (int q = 0; q < z[i]; ++q) s = fma(hopv[(size_t)i * zmax + q], vc[nbr[(size_t)i * zmax + q]], s);
+            s = fma(-bprev, vp[i], s);
+            w[i] = s;
+            pa = fma(s, vc[i], pa);
+        }
+        const double a = block_sum_lz(pa, red);
+        double pb = 0.0;
+        for (int i = tid; i < N; i += LZ_NT) {
+            const double s = fma(-a, vc[i], w[i]);
+            w[i] = s;
+            pb = fma(s, s, pb);
+        }
+        const double bn = sqrt(block_sum_lz(pb, red));
+        const double inv = bn > 0.0 ? 1.0 / bn : 0.0;
+        for (int i = tid; i < N; i += LZ_NT) {
+            vp[i] = vc[i];
+            vc[i] = w[i] * inv;
+        }
+        if (tid == 0) {
+            alpha[j] = a;
+            beta[j] = bn;
+        }
+        bprev = bn;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        st[0] = bprev;
+        st[1] = (double)j;
+    }
+}
+
+__global__ void k_lanczos_start(int N, double* vp, double* vc, double* st) {
+    // a fixed pseudo-random start vector (normalised on the host side of the first step)
+    const double s = rsqrt((double)N);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t)i * 2654435761u;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        vc[i] = s * (0.5 + (double)(h & 0xFFFF) / 65536.0);
+        vp[i] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[0] = 0.0;
+        st[1] = 0.0;
+    }
+}
+
 }  // namespace fkc
 
 using namespace fkc;
+
+hipError_t fk_launch_lanczos(int N, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
+                             const double* diag, int first, int iters, double* work, double* alpha, double* beta,
+                             hipStream_t st) {
+    double* vp = work;
+    double* vc = work + N;
+    double* w = work + 2 * (size_t)N;
+    double* s = work + 3 * (size_t)N;
+    if (first) {
+        hipLaunchKernelGGL(k_lanczos_start, dim3(64), dim3(256), 0, st, N, vp, vc, s);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_lanczos, dim3(1), dim3(LZ_NT), 0, st, N, nbr, z, zmax, hopv, diag, iters, vp, vc, w, s, alpha,
+                       beta);
+    return hipGetLastError();
+}
 
 hipError_t fk_cheb_exact_moments(int N, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
                                  const double* diag, double a, double b, int M, double* T0, double* T1,

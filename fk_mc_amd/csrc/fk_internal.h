@@ -189,6 +189,11 @@ hipError_t fk_launch_kpm_local(int B, int N, int M, int n0, const uint8_t* f, co
                                const double* beta, const double* ab, const int32_t* nbr, const int32_t* z, int zmax,
                                const double* hopv, const double* hopq, double s2c, double s4c, double* mom,
                                const double* logz, double* lz_out, hipStream_t st);
+// Lanczos steps on H = hop + diag(d) (fk_cheb.hip): work = 3N + 2 doubles of state; first != 0
+// starts from a fixed vector; alpha, beta receive the tridiagonal entries of the steps run.
+hipError_t fk_launch_lanczos(int N, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
+                             const double* diag, int first, int iters, double* work, double* alpha, double* beta,
+                             hipStream_t st);
 hipError_t fk_launch_kpm(int B, int N, int R, int M, const int32_t* f, const double* U, const double* mu_c,
                          const double* ab, const int32_t* nbr, const int32_t* z, int zmax, const double* hopv,
                          const double* hopq, uint64_t seed, int per_item_seed, const int32_t* skip, const double* coef, int jackson,

@@ -214,13 +214,19 @@ int grid_for(fkgpu_ctx* ctx, int kind, int B) {
 }
 
 // Site index order of hypercubic_lattice::index_to_pos: last coordinate fastest.
+// The dense N x N template (used by the dense eigensolvers) is kept up to N = 4096 only;
+// the neighbour table and its per-slot values (hopv) describe the lattice at any size.
+constexpr int kDenseTemplateMax = 4096;
+
 void build_hypercubic(fkgpu_ctx* ctx, int kind, int D, int L, double t, double tp) {
     int n = 1;
     for (int i = 0; i < D; ++i) n *= L;
     ctx->N = n;
-    ctx->hop.assign((size_t)n * n, 0.0);
+    const bool dense = n <= kDenseTemplateMax;
+    ctx->hop.assign(dense ? (size_t)n * n : 0, 0.0);
     ctx->zmax = 2 * D + (kind == FKGPU_LATTICE_TRIANGULAR ? 2 : 0);
     ctx->nbr.assign((size_t)n * ctx->zmax, 0);
+    ctx->hopv.assign((size_t)n * ctx->zmax, 0.0);
     ctx->z.assign(n, 0);
     int stride[3] = {1, 1, 1};
     for (int d = D - 2; d >= 0; --d) stride[d] = stride[d + 1] * L;
@@ -228,7 +234,8 @@ void build_hypercubic(fkgpu_ctx* ctx, int kind, int D, int L, double t, double t
         int p[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) p[d] = (i / stride[d]) % L;
         auto add = [&](int j, double v) {
-            ctx->hop[(size_t)i * n + j] += v;
+            if (dense) ctx->hop[(size_t)i * n + j] += v;
+            ctx->hopv[(size_t)i * ctx->zmax + ctx->z[i]] = v;  // L >= 3: no two slots share a site
             ctx->nbr[(size_t)i * ctx->zmax + ctx->z[i]++] = j;
         };
         for (int d = 0; d < D; ++d) {
@@ -256,8 +263,11 @@ int upload_lattice(fkgpu_ctx* ctx) {
     ctx->d_hop = ctx->d_hopv = nullptr;
     ctx->d_nbr = ctx->d_z = nullptr;
     size_t n = ctx->N;
-    ctx->hopv.assign(ctx->nbr.size(), 0.0);
-    for (size_t i = 0; i < n; ++i)
+    const bool dense = !ctx->hop.empty();
+    // per-slot values from the dense template (a CSR lattice may list a site in several slots);
+    // stencil lattices above the dense range filled hopv themselves
+    if (dense) ctx->hopv.assign(ctx->nbr.size(), 0.0);
+    for (size_t i = 0; i < n && dense; ++i)
         for (int q = 0; q < ctx->z[i]; ++q) {
             const int j = ctx->nbr[i * ctx->zmax + q];
             // the dense template sums all slots pointing at j; split the value evenly so that
@@ -274,10 +284,12 @@ int upload_lattice(fkgpu_ctx* ctx) {
     }
     FK_HIP(ctx, hipMalloc(&ctx->d_hopv, sizeof(double) * ctx->hopv.size()));
     FK_HIP(ctx, hipMemcpy(ctx->d_hopv, ctx->hopv.data(), sizeof(double) * ctx->hopv.size(), hipMemcpyHostToDevice));
-    FK_HIP(ctx, hipMalloc(&ctx->d_hop, sizeof(double) * n * n));
+    if (dense) {
+        FK_HIP(ctx, hipMalloc(&ctx->d_hop, sizeof(double) * n * n));
+        FK_HIP(ctx, hipMemcpy(ctx->d_hop, ctx->hop.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+    }
     FK_HIP(ctx, hipMalloc(&ctx->d_nbr, sizeof(int32_t) * ctx->nbr.size()));
     FK_HIP(ctx, hipMalloc(&ctx->d_z, sizeof(int32_t) * n));
-    FK_HIP(ctx, hipMemcpy(ctx->d_hop, ctx->hop.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
     FK_HIP(ctx, hipMemcpy(ctx->d_nbr, ctx->nbr.data(), sizeof(int32_t) * ctx->nbr.size(), hipMemcpyHostToDevice));
     FK_HIP(ctx, hipMemcpy(ctx->d_z, ctx->z.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     return FKGPU_OK;
@@ -352,7 +364,7 @@ int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, doubl
     if (!std::isfinite(t) || !std::isfinite(tp)) return fail(ctx, FKGPU_EINVAL, "non-finite hopping");
     long n = 1;
     for (int i = 0; i < ndim; ++i) n *= L;
-    if (n > 4096) return fail(ctx, FKGPU_EUNSUPPORTED, "lattice volume above 4096 sites");
+    if (n > 16384) return fail(ctx, FKGPU_EUNSUPPORTED, "lattice volume above 16384 sites");
     hipSetDevice(ctx->device);
     build_hypercubic(ctx, kind, ndim, L, t, tp);
     ctx->lat_kind = kind;
@@ -1042,22 +1054,93 @@ struct cheb_quad {
 };
 }  // namespace
 
+// Extreme eigenvalues of a symmetric tridiagonal (alpha, beta: n and n-1 entries) by
+// bisection on Sturm counts.
+static double tri_sturm_count(const std::vector<double>& al, const std::vector<double>& be, int n, double x) {
+    int c = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {
+        const double b2 = i > 0 ? be[i - 1] * be[i - 1] : 0.0;
+        q = (al[i] - x) - (i > 0 ? b2 / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0.0) ++c;
+    }
+    return c;
+}
+static void tri_extremes(const std::vector<double>& al, const std::vector<double>& be, int n, double* lo_out,
+                         double* hi_out) {
+    double glo = 1e300, ghi = -1e300;
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? std::fabs(be[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(be[i]) : 0.0);
+        glo = std::min(glo, al[i] - r);
+        ghi = std::max(ghi, al[i] + r);
+    }
+    for (int which = 0; which < 2; ++which) {
+        double lo = glo, hi = ghi;  // count(lo) <= k < count(hi), k = 0 or n - 1
+        const int k = which == 0 ? 0 : n - 1;
+        for (int it = 0; it < 200 && hi - lo > 0.0; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (mid <= lo || mid >= hi) break;
+            if (tri_sturm_count(al, be, n, mid) > k) hi = mid;
+            else lo = mid;
+        }
+        (which == 0 ? *lo_out : *hi_out) = 0.5 * (lo + hi);
+    }
+}
+
+// lambda_min, lambda_max of H = hop + diag(d) by Lanczos on the device (the ARPACK calls of
+// src/configuration.cpp:99-100, which ask for full accuracy): steps in chunks of 64 until both
+// extreme Ritz values change by at most 4 eps max(|lambda|, 1) over a chunk, the Krylov space
+// is exhausted (beta = 0) or N steps were taken.
+static int lanczos_bounds(fkgpu_ctx* ctx, const double* d_diag, double* e_min, double* e_max, int* steps) {
+    const int N = ctx->N;
+    const int cap = std::min(N, 6000), chunk = 64;
+    double *work, *al_d;
+    int r;
+    if ((r = ensure(ctx, ctx->b_m4, (size_t)3 * N + 2, &work)) || (r = ensure(ctx, ctx->b_m5, (size_t)2 * cap + 2, &al_d)))
+        return r;
+    double* be_d = al_d + cap + 1;
+    hipStream_t st = ctx->stream;
+    std::vector<double> al(cap + 1), be(cap + 1);
+    double plo = 0.0, phi = 0.0;
+    int done = 0;
+    for (;;) {
+        const int it = std::min(chunk, cap - done);
+        FK_HIP(ctx, fk_launch_lanczos(N, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, d_diag, done == 0, it, work,
+                                      al_d, be_d, st));
+        FK_HIP(ctx, hipMemcpyAsync(al.data(), al_d, sizeof(double) * (done + it), hipMemcpyDeviceToHost, st));
+        FK_HIP(ctx, hipMemcpyAsync(be.data(), be_d, sizeof(double) * (done + it), hipMemcpyDeviceToHost, st));
+        FK_HIP(ctx, hipStreamSynchronize(st));
+        int n = done + it;
+        bool exhausted = false;
+        for (int j = done; j < n; ++j)
+            if (!(be[j] > 1e-14 * (std::fabs(al[j]) + 1.0))) { n = j + 1; exhausted = true; break; }
+        double lo, hi;
+        tri_extremes(al, be, n, &lo, &hi);
+        const double tol = 4.0 * 2.220446049250313e-16;
+        const bool conv = done > 0 && std::fabs(lo - plo) <= tol * std::max(std::fabs(lo), 1.0) &&
+                          std::fabs(hi - phi) <= tol * std::max(std::fabs(hi), 1.0);
+        plo = lo;
+        phi = hi;
+        done = n;
+        if (conv || exhausted || done >= cap) break;
+    }
+    *e_min = plo;
+    *e_max = phi;
+    if (steps) *steps = done;
+    return FKGPU_OK;
+}
+
 int fkgpu_cheb_logz(fkgpu_ctx* ctx, const int32_t* f, double beta, double U, double mu_c, int cheb_size,
                     int grid_size, double* moments, double* logZ, double* ab) {
     int r = check_params(ctx, 1, f, &beta, &U, &mu_c);
     if (r) return r;
     if (cheb_size < 2 || grid_size < 2 || !moments || !logZ || !ab) return fail(ctx, FKGPU_EINVAL, "invalid Chebyshev arguments");
-    if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "exact-trace Chebyshev needs N <= 2048");
     const int N = ctx->N;
+    if (N > 16384) return fail(ctx, FKGPU_EUNSUPPORTED, "exact-trace Chebyshev keeps two N x N matrices: N <= 16384");
     cheb_quad cq(cheb_size, grid_size);
     const int M = cq.M;
-    // extreme eigenvalues from the dense solver
-    std::vector<double> ev(N);
-    double lz;
-    if ((r = fkgpu_eig_logz(ctx, 1, f, &beta, &U, &mu_c, 0, ev.data(), nullptr, &lz, nullptr, nullptr))) return r;
-    const double e_min = ev[0], e_max = ev[N - 1];
-    const double a = (e_max - e_min) / 2.0, b = (e_max + e_min) / 2.0;
-    if (!(a > 0.0)) return fail(ctx, FKGPU_EINVAL, "degenerate spectrum: e_max == e_min");
+    hipSetDevice(ctx->device);
     const size_t NN = (size_t)N * N;
     double *T0, *T1, *parts, *traces, *diag;
     if ((r = ensure(ctx, ctx->b_c1, NN, &T0)) || (r = ensure(ctx, ctx->b_c2, NN, &T1)) ||
@@ -1065,14 +1148,18 @@ int fkgpu_cheb_logz(fkgpu_ctx* ctx, const int32_t* f, double beta, double U, dou
         return r;
     diag = traces + 3 * M;
     std::vector<double> dg(N);
-    double trX = 0.0;
-    for (int i = 0; i < N; ++i) {
-        dg[i] = -mu_c + U * f[i];
-        trX += (dg[i] - b) / a;
-    }
+    for (int i = 0; i < N; ++i) dg[i] = -mu_c + U * f[i];
     hipStream_t st = ctx->stream;
-    FK_HIP(ctx, hipMemsetAsync(traces, 0, sizeof(double) * 3 * M, st));
     FK_HIP(ctx, hipMemcpyAsync(diag, dg.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
+    // extreme eigenvalues by Lanczos (ARPACK in the reference)
+    double e_min = 0.0, e_max = 0.0;
+    int lsteps = 0;
+    if ((r = lanczos_bounds(ctx, diag, &e_min, &e_max, &lsteps))) return r;
+    const double a = (e_max - e_min) / 2.0, b = (e_max + e_min) / 2.0;
+    if (!(a > 0.0)) return fail(ctx, FKGPU_EINVAL, "degenerate spectrum: e_max == e_min");
+    double trX = 0.0;
+    for (int i = 0; i < N; ++i) trX += (dg[i] - b) / a;
+    FK_HIP(ctx, hipMemsetAsync(traces, 0, sizeof(double) * 3 * M, st));
     FK_HIP(ctx, fk_cheb_exact_moments(N, ctx->d_nbr, ctx->d_z, ctx->zmax, ctx->d_hopv, diag, a, b, M, T0, T1, parts,
                                       nullptr, traces, st));
     std::vector<double> tr(3 * M);

@@ -53,7 +53,8 @@ void* fkgpu_stream(fkgpu_ctx* ctx); /* hipStream_t */
 /* Lattice hopping structure.  Replaces hypercubic_lattice<D>(L) +
  * fill_nearest_neighbors(l, t) (include/fk_mc/lattice/hypercubic.hpp:136,166;
  * src/lattice/hypercubic.cpp:115-131) and fill_triangular(l, t, tp) (:137-155):
- * H_ij = -t for nearest neighbours, -tp on the triangular diagonal. L >= 3. */
+ * H_ij = -t for nearest neighbours, -tp on the triangular diagonal. L >= 3, at most
+ * 16384 sites (the dense eigensolvers and the chains need N <= 2048 and N <= 4096). */
 int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, double tp);
 /* Arbitrary symmetric hopping matrix in CSR form (abstract_lattice::set_hopping_m,
  * include/fk_mc/lattice.hpp:39; graph lattices). */
@@ -158,8 +159,9 @@ int fkgpu_tune_mu(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* bet
  * configuration_t::calc_chebyshev (src/configuration.cpp:94-206) with
  * chebyshev_eval(cheb_size, grid_size) (include/fk_mc/chebyshev.hpp:15-63): exact
  * trace moments mu_m = Tr T_m(X)/N of X = (H - b)/a, a = (e_max - e_min)/2,
- * b = (e_max + e_min)/2 from the extreme eigenvalues (ARPACK in the reference, the
- * dense solver here, N <= 2048), the reference's doubling identities for the upper
+ * b = (e_max + e_min)/2 from the extreme eigenvalues (ARPACK in the reference; a device
+ * Lanczos here, run until both extreme Ritz values are stable to 4 eps), two N x N
+ * matrices in HBM (N <= 16384), the reference's doubling identities for the upper
  * half of the moments, and logZ = g_0 + 2 sum_m g_m mu_m with g_m the trapezoid
  * quadrature of N log(1 + e^{-beta(a w + b)}) on the Lobatto grid.  No kernel damping.
  * cheb_size is rounded up to even as in the reference.  Outputs: moments[cheb_size],

@@ -33,6 +33,57 @@ def test_exact_trace_vs_oracle(fk, orc, D, L, U, beta):
     ctx.close()
 
 
+def _dense_h(L, f, U, mu):
+    """cubic2d (t=1, periodic) Hamiltonian as a dense matrix (numpy)."""
+    n = L * L
+    H = np.zeros((n, n))
+    idx = np.arange(n).reshape(L, L)
+    for shift in ((1, 0), (0, 1)):
+        j = np.roll(idx, (-shift[0], -shift[1]), axis=(0, 1)).ravel()
+        H[idx.ravel(), j] -= 1.0
+        H[j, idx.ravel()] -= 1.0
+    H[np.arange(n), np.arange(n)] = -mu + U * f
+    return H
+
+
+def test_exact_trace_at_scale(fk):
+    """calc_chebyshev's algorithm above the dense solver's range: cubic2d L=64 (N=4096) with
+    the reference's moment count int(2.2 ln N) rounded to even; the spectral bounds from the
+    device Lanczos (ARPACK in the reference, src/configuration.cpp:99-100) equal LAPACK's
+    extreme eigenvalues, and logZ is within 5e-2 of ED (fast_update_test.cpp:77).  L=128
+    (N=16384) runs too (no dense solve to compare with there)."""
+    print("case cheb exact L=64", flush=True)
+    L, U, beta = 64, 2.0, 10.0
+    n = L * L
+    f = np.random.default_rng(64).integers(0, 2, n).astype(np.int32)
+    M = int(np.log(n) * 2.2)
+    M += M % 2
+    G = max(2 * M, 10)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    g = ctx.cheb_logz(f, beta, U, U / 2, M, G)
+    ev = np.linalg.eigvalsh(_dense_h(L, f, U, U / 2))
+    print("lanczos bounds", g["e_min"], g["e_max"], "lapack", ev[0], ev[-1], flush=True)
+    assert abs(g["e_min"] - ev[0]) < 1e-10 and abs(g["e_max"] - ev[-1]) < 1e-10
+    lz = float(np.logaddexp(0.0, -beta * ev).sum())
+    print("cheb logZ", g["logZ"], "ed", lz, flush=True)
+    assert abs(g["logZ"] - lz) / lz < 5e-2
+    x = (ev - g["b"]) / g["a"]
+    exact = np.array([np.mean(np.cos(m * np.arccos(np.clip(x, -1, 1)))) for m in range(M)])
+    assert np.abs(g["moments"] - exact).max() < 1e-9
+    ctx.close()
+    print("case cheb exact L=128", flush=True)
+    L = 128
+    n = L * L
+    f = np.random.default_rng(128).integers(0, 2, n).astype(np.int32)
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, L, 1.0, 0.0)
+    assert ctx.N == n
+    g = ctx.cheb_logz(f, beta, U, U / 2, 22, 44)
+    assert -5.0 - 1e-9 <= g["e_min"] < g["e_max"] <= 5.0 + 1e-9  # inside the Gershgorin discs
+    assert abs(g["moments"][0] - 1.0) < 1e-14 and np.all(np.abs(g["moments"]) <= 1.0 + 1e-12)
+    assert np.isfinite(g["logZ"]) and g["logZ"] > 0
+    ctx.close()
+
+
 def test_kpm_moments_and_logz(fk, orc):
     print("case kpm", flush=True)
     L, U, beta = 32, 2.0, 10.0

@@ -27,6 +27,9 @@ Printed: one JSON line with the contract fields plus
                 the same chains with rank-one spectral updates / mixed precision,
                 and cubic3d L=8 (N=512) mixed against FP64, each with its dOmega
                 against FP64 dense chains on the same seeds;
+  multi_lattice_chains_triangular
+                128 chains on triangular L = 12..24 in one chain set, one persistent
+                launch per step block (fkgpu_chains_create_multi);
   adjoint_triangular
                 triangular L = 12..24 adjoint gradients (fkgpu_adjoint), configs/s and
                 TFLOP/s at (10/3) N^3 flop per configuration, gradients against central
@@ -207,6 +210,10 @@ def main():
     if not args.no_c4:
         c4 = c4_block(fk)
 
+    multi = None
+    if not args.no_c4:
+        multi = multi_lattice_block(fk)
+
     c5 = None
     if args.c5_steps > 0:
         c5 = c5_block(fk, args.c5_steps, args.c5_check_steps)
@@ -244,6 +251,7 @@ def main():
         "mixed": mixed,
         "mixed_cubic3d_L8": c3,
         "adjoint_triangular": c4,
+        "multi_lattice_chains_triangular": multi,
         "kpm_cubic2d_L64": c5,
         "cpu_baseline": cpu,
     }
@@ -640,6 +648,57 @@ def c5_block(fk, steps, check_steps, every=100):
                                      "whose steps are bound by LDS reads of the neighbour values"},
                 "logZ_finite": bool(np.all(np.isfinite(st["logZ"])))})
     return out
+
+
+def multi_lattice_block(fk, steps=8):
+    """128 chains of uneven sizes in one chain set: triangular (t = tp = 1) L in {12, 16, 20, 24}
+    (N = 144, 256, 400, 576), 32 chains each, U = 3, beta = 5, mu = U/2, addremove /
+    nearest-neighbour exchange 0.5 / 0.5, every chain's move steps in one persistent launch
+    (the blocked FP64 solver serves all sizes); beside it the four sizes as four chain sets
+    run one after the other."""
+    Ls = [12, 16, 20, 24]
+    ctxs = [fk.Context(0).set_lattice(fk.LATTICE_TRIANGULAR, 2, L, 1.0, 1.0) for L in Ls]
+    Ns = [c.N for c in ctxs]
+    nmax = max(Ns)
+    per = 32
+    B = per * len(Ls)
+    lat = np.repeat(np.arange(len(Ls)), per)
+    rng = np.random.default_rng(24)
+    f = np.zeros((B, nmax), np.int32)
+    for c in range(B):
+        f[c, :Ns[lat[c]]] = (rng.random(Ns[lat[c]]) < 0.5)
+    beta, U = 5.0 * np.ones(B), 3.0 * np.ones(B)
+    seeds = (2400 + np.arange(B)).astype(np.uint32)
+    moves = [(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)]
+    ch = fk.Chains(ctxs, f, beta, U, U / 2, U / 2, seeds, lattice=lat)
+    ch.set_moves(moves)
+    ch.sweep(2)
+    s0 = ch.state()
+    ch.sweep(steps)
+    ms = ch.last_ms()
+    s1 = ch.state()
+    ch.close()
+    solves = (s1["nsolve"] - s0["nsolve"]).astype(float)
+    flop = float(np.sum(solves * (4.0 / 3.0) * np.array(Ns)[lat] ** 3))
+    sep_ms = 0.0
+    for li, L in enumerate(Ls):
+        idx = lat == li
+        one = fk.Chains(ctxs[li], f[idx, :Ns[li]], beta[idx], U[idx], U[idx] / 2, U[idx] / 2, seeds[idx])
+        one.set_moves(moves)
+        one.sweep(2)
+        one.sweep(steps)
+        sep_ms += one.last_ms()
+        one.close()
+    for c in ctxs:
+        c.close()
+    achieved = flop / (ms / 1e3) / 1e12
+    return {"lattices": "triangular L=12,16,20,24 (N=144,256,400,576), 32 chains each", "chains": B,
+            "steps": steps, "one_launch_ms_per_step": round(ms / steps, 3),
+            "moves_per_s": round(B * steps / (ms / 1e3), 1),
+            "four_chain_sets_in_turn_ms_per_step": round(sep_ms / steps, 3),
+            "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+                         "work": "(4/3) N_c^3 FP64 flop per eigensolve of chain c, device solve counts, HIP-event time"}}
 
 
 def cpu_baseline(L, seconds, ch, ctx, f, beta, U, seeds):

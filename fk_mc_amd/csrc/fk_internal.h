@@ -6,6 +6,8 @@
 
 #include "../../include/fkgpu.h"
 
+#define FK_MAX_LATTICES 8  // lattices of one chain set
+
 // Batched calc_hamiltonian + calc_ed.  All pointers are device pointers.
 struct eig_args {
     int B, N;
@@ -52,6 +54,17 @@ struct sweep_args {
     int kinds[4];          // move kind of each registered move
     int nW;                // f-f interaction range (1-D lattices only)
     double W[8];
+    // chains on several lattices (fkgpu_chains_create_multi): chain c lives on lattice
+    // lat[c] (N = lN[.], dense template lhop[.], neighbour table lnbr[.] / lz[.] / lzmax[.]);
+    // f and spectrum rows have the stride ld (the largest N).  lat == null: every chain on
+    // (N, hop, nbr, z, zmax) with stride N.
+    const int32_t* lat;
+    int ld;
+    int lN[FK_MAX_LATTICES];
+    const double* lhop[FK_MAX_LATTICES];
+    const int32_t* lnbr[FK_MAX_LATTICES];
+    const int32_t* lz[FK_MAX_LATTICES];
+    int lzmax[FK_MAX_LATTICES];
     int flip_ff;           // the global exchange's weight includes -beta dE_ff (the Chebyshev
                            // move_flip, src/moves_chebyshev.cpp:22; the ED one has none)
     int trace;             // record the per-move trace

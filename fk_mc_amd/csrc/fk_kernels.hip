@@ -210,12 +210,16 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     // compiler from materialising a private copy of it in every lane
     __shared__ sweep_args sa;
     const int tid = threadIdx.x;
-    const int N = a.N;
     const int ch = blockIdx.x;  // one workgroup per chain
-    if (tid == 0) sa = a;
+    const chain_lattice cl = chain_lattice_of(a, ch);
+    const int N = cl.N;
+    if (tid == 0) {
+        sa = a;
+        cl.apply(sa);
+    }
     uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
-    for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;
+    for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * cl.ld + i] : 0;
     __syncthreads();
     const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch], mu_f = a.mu_f[ch];
     if (tid == 0) prop = propose_move(mt, f_lds, N, a.nf[ch], sa);
@@ -223,7 +227,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     double lz_new = 0.0;
     if (prop.solve) {
         double* ws = a.ws ? a.ws + (size_t)blockIdx.x * a.ws_stride : nullptr;
-        lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
+        lz_new = S::solve(sm, cl.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
     }
     if (tid == 0) {
         double logw;
@@ -234,9 +238,9 @@ __global__ void __launch_bounds__(S::NT) k_sweep(sweep_args a) {
     }
     __syncthreads();
     if (acc_s) {
-        for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * N + i] = f_lds[i];
+        for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * cl.ld + i] = f_lds[i];
         if (prop.solve && a.spectrum)
-            for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = S::evptr(sm)[i];
+            for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * cl.ld + i] = S::evptr(sm)[i];
     }
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mtg[i] = mt[i];
 }
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     __shared__ int acc_s, chain_s, step_s;
     __shared__ sweep_args sa;
     const int tid = threadIdx.x;
-    const int N = a.N, B = a.B;
+    const int B = a.B;
     if (tid == 0) {
         sa = a;
         int c = -1, st = 0;
@@ -305,13 +309,16 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     if (ch < 0) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int s = __builtin_amdgcn_readfirstlane(step_s);
+    const chain_lattice cl = chain_lattice_of(a, ch);
+    const int N = cl.N;
     uint32_t* mtg = a.mt + (size_t)ch * fkrng::MT_WORDS;
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
-    for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * N + i] : 0;
+    for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * cl.ld + i] : 0;
     __syncthreads();
     const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch], mu_f = a.mu_f[ch];
     if (tid == 0) {
         sa.step = a.step + s;
+        cl.apply(sa);
         prop = propose_move(mt, f_lds, N, a.nf[ch], sa);
     }
     __syncthreads();
@@ -319,7 +326,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     if (prop.solve) {
         // workspace solvers: one workspace per chain (a chain's steps never overlap)
         double* ws = a.ws ? a.ws + (size_t)ch * a.ws_stride : nullptr;
-        lz_new = S::solve(sm, a.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
+        lz_new = S::solve(sm, cl.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
     }
     if (tid == 0) {
         double logw;
@@ -330,9 +337,9 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     }
     __syncthreads();
     if (acc_s) {
-        for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * N + i] = f_lds[i];
+        for (int i = tid; i < N; i += S::NT) a.f[(size_t)ch * cl.ld + i] = f_lds[i];
         if (prop.solve && a.spectrum)
-            for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * N + i] = S::evptr(sm)[i];
+            for (int i = tid; i < N; i += S::NT) a.spectrum[(size_t)ch * cl.ld + i] = S::evptr(sm)[i];
     }
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mtg[i] = mt[i];
     // every wave waits until its stores have reached the L2 (s_waitcnt vmcnt(0)) before

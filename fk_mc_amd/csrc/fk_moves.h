@@ -24,6 +24,34 @@ __device__ inline double ff_energy_dev(const uint8_t* f, int V, int nW, const do
     return e;
 }
 
+// Lattice of chain ch: its size, dense template, row stride of f / spectrum, and (apply)
+// its neighbour table patched into a workgroup's copy of the argument block.
+struct chain_lattice {
+    int N, ld, li;
+    const double* hop;
+    __device__ void apply(sweep_args& s) const {
+        if (li < 0) return;
+        s.nbr = s.lnbr[li];
+        s.z = s.lz[li];
+        s.zmax = s.lzmax[li];
+    }
+};
+__device__ inline chain_lattice chain_lattice_of(const sweep_args& a, int ch) {
+    chain_lattice c;
+    if (!a.lat) {
+        c.N = a.N;
+        c.ld = a.N;
+        c.li = -1;
+        c.hop = a.hop;
+        return c;
+    }
+    c.li = __builtin_amdgcn_readfirstlane(a.lat[ch]);
+    c.N = a.lN[c.li];
+    c.ld = a.ld;
+    c.hop = a.lhop[c.li];
+    return c;
+}
+
 // Proposal of one move (thread 0 of the chain's workgroup): the draws of
 // mc_metropolis::update (src/mc_metropolis.cpp:41) and of the move's attempt()
 // (src/moves.cpp) in the reference's order, then the Metropolis uniform

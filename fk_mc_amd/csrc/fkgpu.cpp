@@ -131,6 +131,11 @@ struct fkgpu_chains {
     int32_t *fu_upd = nullptr, *fu_f32 = nullptr;
     int32_t* fu_skip = nullptr;      // per chain: the periodic re-decomposition is not due this step
     long long* fu_count = nullptr;   // move steps since the rank-one mode was set (device counter)
+    // chains on several lattices (fkgpu_chains_create_multi): the other contexts (held) and
+    // the per-chain lattice index
+    std::vector<fkgpu_ctx*> mctx;
+    int32_t* lat = nullptr;
+    int nlat = 0;
     // several dense FP64 move steps per launch (k_sweep_q work queue)
     int spl = 16;                    // move steps per launch
     unsigned* q_ctr = nullptr;
@@ -1269,7 +1274,7 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     void* ptrs[] = {ch->f, ch->nf, ch->logz, ch->spectrum, ch->mt, ch->beta, ch->U, ch->mu_c, ch->mu_f,
                     ch->naccept, ch->nsolve, ch->ws, ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw,
                     ch->t_logz, ch->t_acc, ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef,
-                    ch->k_mom, ch->k_lz, ch->k_lmom, ch->q_ctr, ch->q_ring, ch->q_err};
+                    ch->k_mom, ch->k_lz, ch->k_lmom, ch->q_ctr, ch->q_ring, ch->q_err, ch->lat};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ch->ev0) hipEventDestroy(ch->ev0);
@@ -1280,25 +1285,30 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
     for (void* p : fptrs)
         if (p) hipFree(p);
     fkgpu_ctx* ctx = ch->ctx;
+    std::vector<fkgpu_ctx*> others = ch->mctx;
     delete ch;
     ctx_release(ctx);
+    for (fkgpu_ctx* c : others) ctx_release(c);
 }
 
-int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, const double* beta, const double* U,
-                        const double* mu_c, const double* mu_f, const uint32_t* seeds, fkgpu_chains** out) {
-    if (!out) return FKGPU_EINVAL;
-    *out = nullptr;
-    int r = check_params(ctx, nchains, f_init, beta, U, mu_c);
-    if (r) return r;
-    if (!mu_f || !seeds || !finite_all(mu_f, nchains)) return fail(ctx, FKGPU_EINVAL, "invalid mu_f or seeds");
+}  // extern "C"
+
+// A chain set of B chains whose f / spectrum rows have N entries (the lattice size, or the
+// largest one of a multi-lattice set), stepped by the dense solver `kind`.  With
+// initial_eig the starting logZ and spectra come from ctx's lattice; otherwise they are
+// left at zero for the caller.
+static int chains_create_impl(fkgpu_ctx* ctx, int N, int kind, int nchains, const int32_t* f_init,
+                              const double* beta, const double* U, const double* mu_c, const double* mu_f,
+                              const uint32_t* seeds, bool initial_eig, fkgpu_chains** out) {
+    int r = 0;
     hipSetDevice(ctx->device);
-    const int N = ctx->N, B = nchains;
+    const int B = nchains;
     fkgpu_chains* ch = new fkgpu_chains();
     ch->ctx = ctx;
     ctx->refs += 1;
     ch->B = B;
     ch->N = N;
-    ch->kind = pick_kind(ctx, B);
+    ch->kind = kind;
     ch->dense_ok = ch->kind >= 0;
     ch->grid = grid_for(ctx, ch->kind, B);
     hipError_t e = hipSuccess;
@@ -1358,7 +1368,15 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     }
     // initial logZ and spectrum of every chain
     int32_t* df;
-    if ((r = ensure(ctx, ctx->b_f, (size_t)B * N, &df))) { fkgpu_chains_destroy(ch); return r; }
+    if (!initial_eig) {
+        chk(hipMemsetAsync(ch->logz, 0, sizeof(double) * B, st));
+        chk(hipMemsetAsync(ch->spectrum, 0, sizeof(double) * B * N, st));
+    }
+    if (!initial_eig) {
+    } else if ((r = ensure(ctx, ctx->b_f, (size_t)B * N, &df))) {
+        fkgpu_chains_destroy(ch);
+        return r;
+    } else {
     chk(hipMemcpyAsync(df, f_init, sizeof(int32_t) * B * N, hipMemcpyHostToDevice, st));
     eig_args a{};
     a.B = B;
@@ -1377,6 +1395,7 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     } else {  // logZ comes from fkgpu_chains_set_kpm
         chk(hipMemsetAsync(ch->logz, 0, sizeof(double) * B, st));
         chk(hipMemsetAsync(ch->spectrum, 0, sizeof(double) * B * N, st));
+    }
     }
     chk(hipStreamSynchronize(st));
     if (e != hipSuccess) {
@@ -1410,6 +1429,144 @@ int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, cons
     return FKGPU_OK;
 }
 
+extern "C" {
+
+int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, const double* beta, const double* U,
+                        const double* mu_c, const double* mu_f, const uint32_t* seeds, fkgpu_chains** out) {
+    if (!out) return FKGPU_EINVAL;
+    *out = nullptr;
+    int r = check_params(ctx, nchains, f_init, beta, U, mu_c);
+    if (r) return r;
+    if (!mu_f || !seeds || !finite_all(mu_f, nchains)) return fail(ctx, FKGPU_EINVAL, "invalid mu_f or seeds");
+    return chains_create_impl(ctx, ctx->N, pick_kind(ctx, nchains), nchains, f_init, beta, U, mu_c, mu_f, seeds, true,
+                              out);
+}
+
+int fkgpu_chains_create_multi(int ncontexts, fkgpu_ctx* const* contexts, int nchains, const int32_t* lattice,
+                              const int32_t* f_init, const double* beta, const double* U, const double* mu_c,
+                              const double* mu_f, const uint32_t* seeds, fkgpu_chains** out) {
+    if (!out) return FKGPU_EINVAL;
+    *out = nullptr;
+    if (ncontexts < 1 || !contexts || !contexts[0]) return FKGPU_EINVAL;
+    fkgpu_ctx* ctx = contexts[0];
+    if (ncontexts > FK_MAX_LATTICES) return fail(ctx, FKGPU_EUNSUPPORTED, "at most 8 lattices in one chain set");
+    if (nchains < 1 || !lattice || !f_init || !beta || !U || !mu_c || !mu_f || !seeds)
+        return fail(ctx, FKGPU_EINVAL, "null argument or empty chain set");
+    int nmax = 0;
+    for (int l = 0; l < ncontexts; ++l) {
+        const fkgpu_ctx* c = contexts[l];
+        if (!c || c->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "a context without a lattice");
+        if (c->device != ctx->device) return fail(ctx, FKGPU_EINVAL, "contexts on different devices");
+        if (fk_solver_kind(c->N) < 0 || c->N > 1024)
+            return fail(ctx, FKGPU_EUNSUPPORTED, "multi-lattice chain sets need N <= 1024 on every lattice");
+        nmax = std::max(nmax, c->N);
+    }
+    // one solver for all chains: the register-resident wave-row solver when every N <= 256,
+    // else the blocked HBM-workspace solver (both take N at run time)
+    const int kind = nmax <= 256 ? 5 : 4;
+    const int B = nchains;
+    std::vector<int32_t> fpad((size_t)B * nmax, 0);
+    for (int c = 0; c < B; ++c) {
+        if (lattice[c] < 0 || lattice[c] >= ncontexts) return fail(ctx, FKGPU_EINVAL, "lattice index out of range");
+        if (!std::isfinite(beta[c]) || !std::isfinite(U[c]) || !std::isfinite(mu_c[c]) || !std::isfinite(mu_f[c]))
+            return fail(ctx, FKGPU_EINVAL, "non-finite beta, U, mu_c or mu_f");
+        const int n = contexts[lattice[c]]->N;
+        for (int i = 0; i < n; ++i) {
+            const int32_t v = f_init[(size_t)c * nmax + i];
+            if (v != 0 && v != 1) return fail(ctx, FKGPU_EINVAL, "occupations must be 0 or 1");
+            fpad[(size_t)c * nmax + i] = v;
+        }
+    }
+    fkgpu_chains* ch = nullptr;
+    int r = chains_create_impl(ctx, nmax, kind, B, fpad.data(), beta, U, mu_c, mu_f, seeds, false, &ch);
+    if (r) return r;
+    for (int l = 1; l < ncontexts; ++l) {
+        contexts[l]->refs += 1;
+        ch->mctx.push_back(contexts[l]);
+    }
+    // starting logZ and spectra, one batched eigensolve per lattice with the chain set's solver
+    // (so that a chain starts from the bits its own steps would produce)
+    hipStream_t st = ctx->stream;
+    std::vector<double> lz(B, 0.0), spec((size_t)B * nmax, 0.0);
+    int32_t* dfl;
+    double *dbl, *dul, *dml, *devl, *dlzl;
+    if ((r = ensure(ctx, ctx->b_f, (size_t)B * nmax, &dfl)) || (r = ensure(ctx, ctx->b_beta, B, &dbl)) ||
+        (r = ensure(ctx, ctx->b_U, B, &dul)) || (r = ensure(ctx, ctx->b_mu, B, &dml)) ||
+        (r = ensure(ctx, ctx->b_ev, (size_t)B * nmax, &devl)) || (r = ensure(ctx, ctx->b_lz, B, &dlzl))) {
+        fkgpu_chains_destroy(ch);
+        return r;
+    }
+    for (int l = 0; l < ncontexts; ++l) {
+        fkgpu_ctx* c = contexts[l];
+        std::vector<int> idx;
+        for (int q = 0; q < B; ++q)
+            if (lattice[q] == l) idx.push_back(q);
+        if (idx.empty()) continue;
+        const int n = c->N, nb = (int)idx.size();
+        std::vector<int32_t> fl((size_t)nb * n);
+        std::vector<double> bl(nb), ul(nb), ml(nb), ev((size_t)nb * n), lzl(nb);
+        for (int k = 0; k < nb; ++k) {
+            std::copy(fpad.begin() + (size_t)idx[k] * nmax, fpad.begin() + (size_t)idx[k] * nmax + n,
+                      fl.begin() + (size_t)k * n);
+            bl[k] = beta[idx[k]];
+            ul[k] = U[idx[k]];
+            ml[k] = mu_c[idx[k]];
+        }
+        eig_args a{};
+        a.B = nb;
+        a.N = n;
+        a.hop = c->d_hop;
+        a.f = dfl;
+        a.beta = dbl;
+        a.U = dul;
+        a.mu_c = dml;
+        a.evals = devl;
+        a.logz = dlzl;
+        a.ws = ch->ws;  // B workspaces of the largest N: enough for nb <= B of this one
+        a.ws_stride = ch->ws_stride;
+        hipError_t he = hipMemcpyAsync(dfl, fl.data(), sizeof(int32_t) * fl.size(), hipMemcpyHostToDevice, st);
+        if (he == hipSuccess) he = hipMemcpyAsync(dbl, bl.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st);
+        if (he == hipSuccess) he = hipMemcpyAsync(dul, ul.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st);
+        if (he == hipSuccess) he = hipMemcpyAsync(dml, ml.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st);
+        if (he == hipSuccess) he = fk_launch_eig(kind, a, nb, st);
+        if (he == hipSuccess) he = hipMemcpyAsync(ev.data(), devl, sizeof(double) * ev.size(), hipMemcpyDeviceToHost, st);
+        if (he == hipSuccess) he = hipMemcpyAsync(lzl.data(), dlzl, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+        if (he == hipSuccess) he = hipStreamSynchronize(st);
+        if (he != hipSuccess) {
+            fkgpu_chains_destroy(ch);
+            return hipfail(ctx, he, "multi-lattice initial eigensolve");
+        }
+        for (int k = 0; k < nb; ++k) {
+            lz[idx[k]] = lzl[k];
+            std::copy(ev.begin() + (size_t)k * n, ev.begin() + (size_t)(k + 1) * n, spec.begin() + (size_t)idx[k] * nmax);
+        }
+    }
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    chk(dalloc(&ch->lat, B));
+    chk(hipMemcpyAsync(ch->lat, lattice, sizeof(int32_t) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->logz, lz.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(ch->spectrum, spec.data(), sizeof(double) * spec.size(), hipMemcpyHostToDevice, st));
+    chk(hipStreamSynchronize(st));
+    if (e != hipSuccess) {
+        fkgpu_chains_destroy(ch);
+        return hipfail(ctx, e, "multi-lattice chain set upload");
+    }
+    sweep_args& s = ch->args;
+    s.lat = ch->lat;
+    s.ld = nmax;
+    for (int l = 0; l < ncontexts; ++l) {
+        s.lN[l] = contexts[l]->N;
+        s.lhop[l] = contexts[l]->d_hop;
+        s.lnbr[l] = contexts[l]->d_nbr;
+        s.lz[l] = contexts[l]->d_z;
+        s.lzmax[l] = contexts[l]->zmax;
+    }
+    ch->nlat = ncontexts;
+    *out = ch;
+    return FKGPU_OK;
+}
+
 int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, const double* weights) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
@@ -1424,9 +1581,14 @@ int fkgpu_chains_set_moves(fkgpu_chains* ch, int nmoves, const int32_t* kinds, c
     // The exchange draws a neighbour slot uniformly from the site's z neighbours, so
     // every site needs one (a CSR lattice may hold isolated sites).
     for (int m = 0; m < nmoves; ++m)
-        if (kinds[m] == FKGPU_MOVE_HOP && weights[m] > 0.0)
-            for (int i = 0; i < ctx->N; ++i)
-                if (ctx->z[i] == 0) return fail(ctx, FKGPU_EINVAL, "nearest-neighbour exchange on a lattice with an isolated site");
+        if (kinds[m] == FKGPU_MOVE_HOP && weights[m] > 0.0) {
+            std::vector<const fkgpu_ctx*> lats{ctx};
+            for (const fkgpu_ctx* c : ch->mctx) lats.push_back(c);
+            for (const fkgpu_ctx* c : lats)
+                for (int i = 0; i < c->N; ++i)
+                    if (c->z[i] == 0)
+                        return fail(ctx, FKGPU_EINVAL, "nearest-neighbour exchange on a lattice with an isolated site");
+        }
     if (ch->fu)
         for (int m = 0; m < nmoves; ++m)
             if (kinds[m] == FKGPU_MOVE_RANDOMIZE)
@@ -1571,6 +1733,7 @@ static int queue_check(fkgpu_chains* ch) {
 int fkgpu_chains_randomize_f(fkgpu_chains* ch, const int32_t* nf_start) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
+    if (ch->nlat) return fail(ctx, FKGPU_EUNSUPPORTED, "randomize_f: not for chain sets on several lattices");
     const int B = ch->B, N = ch->N;
     if (!nf_start) return fail(ctx, FKGPU_EINVAL, "null nf_start");
     for (int c = 0; c < B; ++c)
@@ -1634,6 +1797,7 @@ int fkgpu_chains_set_steps_per_launch(fkgpu_chains* ch, int steps) {
 int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
+    if (ch->nlat) return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one updates: not for chain sets on several lattices");
     const int B = ch->B, N = ch->N;
     if (mode != FKGPU_UPDATE_DENSE && mode != FKGPU_UPDATE_RANK1) return fail(ctx, FKGPU_EINVAL, "unknown update mode");
     if (refresh_steps < 0) return fail(ctx, FKGPU_EINVAL, "negative refresh interval");
@@ -1692,6 +1856,7 @@ int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
 int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
+    if (ch->nlat) return fail(ctx, FKGPU_EUNSUPPORTED, "mixed precision: not for chain sets on several lattices");
     const int B = ch->B, N = ch->N;
     if (precision != FKGPU_PRECISION_FP64 && precision != FKGPU_PRECISION_MIXED)
         return fail(ctx, FKGPU_EINVAL, "unknown precision");
@@ -1758,6 +1923,7 @@ int fkgpu_chains_fallbacks(const fkgpu_chains* ch, int64_t* n) {
 int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jackson) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
+    if (ch->nlat) return fail(ctx, FKGPU_EUNSUPPORTED, "the Chebyshev path: not for chain sets on several lattices");
     const int B = ch->B, N = ch->N;
     if (R < 1 || M < 2 || (M & 1) || M > 512) return fail(ctx, FKGPU_EINVAL, "invalid KPM arguments");
     if (N > 4096 || ctx->zmax > 8) return fail(ctx, FKGPU_EUNSUPPORTED, "KPM path needs N <= 4096 and <= 8 neighbours");
@@ -2160,6 +2326,7 @@ static int meas_count(fkgpu_chains* ch, long long* count) {
 int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
+    if (ch->nlat) return fail(ctx, FKGPU_EUNSUPPORTED, "on-device measurements: not for chain sets on several lattices");
     const int vecs = FKGPU_MEASURE_IPR | FKGPU_MEASURE_STIFFNESS | FKGPU_MEASURE_EIGENFUNCTIONS;
     const int all = FKGPU_MEASURE_SPECTRUM_HISTORY | FKGPU_MEASURE_FOCC_HISTORY | vecs | FKGPU_MEASURE_NO_SPECTRUM;
     if (capacity < 0 || (flags & ~all)) return fail(ctx, FKGPU_EINVAL, "invalid capacity or measurement flags");

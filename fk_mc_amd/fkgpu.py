@@ -81,6 +81,8 @@ def lib():
         L.fkgpu_eig_logz_device.argtypes = [P, C.c_int, P, P, P, P, P, P]
         L.fkgpu_eig_logz_device_noalloc.argtypes = [P, C.c_int, P, P, P, P, P, P]
         L.fkgpu_chains_create.argtypes = [P, C.c_int, _i32, _d, _d, _d, _d, _u32, C.POINTER(P)]
+        L.fkgpu_chains_create_multi.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.c_int, _i32, _i32, _d, _d, _d, _d,
+                                                _u32, C.POINTER(P)]
         L.fkgpu_chains_destroy.argtypes = [P]
         L.fkgpu_chains_destroy.restype = None
         L.fkgpu_chains_set_moves.argtypes = [P, C.c_int, _i32, _d]
@@ -354,7 +356,7 @@ class Chains:
 
     _kind = "chains"
 
-    def __init__(self, ctx, f_init, beta, U, mu_c, mu_f, seeds):
+    def __init__(self, ctx, f_init, beta, U, mu_c, mu_f, seeds, lattice=None):
         self.ctx = ctx
         self._h = C.c_void_p()
         _LIVE.append(weakref.ref(self))
@@ -369,9 +371,21 @@ class Chains:
 
         self._keep = [vec(beta), vec(U), vec(mu_c), vec(mu_f), vec(seeds, np.uint32)]
         self._h = C.c_void_p()
-        ctx._check(lib().fkgpu_chains_create(ctx.handle, B, _p(f, _i32), _p(self._keep[0], _d),
-                                             _p(self._keep[1], _d), _p(self._keep[2], _d), _p(self._keep[3], _d),
-                                             _p(self._keep[4], _u32), C.byref(self._h)))
+        if lattice is None:
+            ctx._check(lib().fkgpu_chains_create(ctx.handle, B, _p(f, _i32), _p(self._keep[0], _d),
+                                                 _p(self._keep[1], _d), _p(self._keep[2], _d), _p(self._keep[3], _d),
+                                                 _p(self._keep[4], _u32), C.byref(self._h)))
+        else:
+            # chains on several lattices: ctx is the list of contexts, lattice[c] indexes it
+            self.contexts = list(ctx)
+            self.ctx = self.contexts[0]
+            arr = (C.c_void_p * len(self.contexts))(*[C.c_void_p(c.handle.value) for c in self.contexts])
+            lat = np.ascontiguousarray(lattice, dtype=np.int32)
+            self._keep += [arr, lat]
+            self.ctx._check(lib().fkgpu_chains_create_multi(len(self.contexts), arr, B, _p(lat, _i32), _p(f, _i32),
+                                                            _p(self._keep[0], _d), _p(self._keep[1], _d),
+                                                            _p(self._keep[2], _d), _p(self._keep[3], _d),
+                                                            _p(self._keep[4], _u32), C.byref(self._h)))
 
     def close(self):
         if self._h:

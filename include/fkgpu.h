@@ -197,6 +197,19 @@ typedef struct {
 int fkgpu_chains_create(fkgpu_ctx* ctx, int nchains, const int32_t* f_init, const double* beta, const double* U,
                         const double* mu_c, const double* mu_f, const uint32_t* seeds, fkgpu_chains** out);
 void fkgpu_chains_destroy(fkgpu_chains* ch);
+/* Chains on several lattices in one chain set (no reference counterpart: the reference runs
+ * one lattice per program): chain c lives on the lattice of contexts[lattice[c]] (at most 8
+ * contexts, one device, every N <= 1024).  f_init is nchains x Nmax with Nmax the largest N
+ * (row c uses its first N of lattice[c] entries); fkgpu_chains_state returns f and spectrum
+ * rows in the same layout, zero past each chain's N.  The FP64 dense move steps of all chains
+ * run in one persistent launch (k_sweep_q), every chain with its own size and neighbour table;
+ * one solver serves all of them (the register-resident wave-row solver when every N <= 256,
+ * else the blocked HBM-workspace solver).  The contexts must outlive the chain set (they are
+ * held until it is destroyed).  randomize_f, the rank-one, mixed-precision and Chebyshev modes
+ * and the on-device measurements are not available on such a set. */
+int fkgpu_chains_create_multi(int ncontexts, fkgpu_ctx* const* contexts, int nchains, const int32_t* lattice,
+                              const int32_t* f_init, const double* beta, const double* U, const double* mu_c,
+                              const double* mu_f, const uint32_t* seeds, fkgpu_chains** out);
 /* configuration_t::randomize_f(rng, nf) (src/configuration.cpp:47-56) of every chain,
  * drawn from the chain's own generator, as fk_mc::initialize draws the starting
  * configuration from the Metropolis generator (include/fk_mc/fk_mc.hxx:46): nf_start[c]

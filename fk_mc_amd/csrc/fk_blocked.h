@@ -48,7 +48,8 @@ template <> struct mf<double> {
 
 __host__ __device__ constexpr int blk_ld(int N) { return (N + 15) & ~15; }
 
-template <class T, int NT_>
+// NT_: threads of the workgroup (the waves per matrix); NBCAP: largest panel width.
+template <class T, int NT_, int NBCAP = 32>
 struct blocked_solver {
     static constexpr int NT = NT_;
     static constexpr int NW = NT / WAVE;
@@ -78,7 +79,7 @@ struct blocked_solver {
     // panel width for a given N (even, so that K = 2 nb is a multiple of 4)
     __host__ __device__ static int panel_width(int N) {
         int nb = PANEL_ELEMS / blk_ld(N);
-        if (nb > 32) nb = 32;
+        if (nb > NBCAP) nb = NBCAP;
         return nb & ~1;
     }
     // workspace elements (double units) per workgroup

@@ -7,6 +7,7 @@
 #include "../../include/fkgpu.h"
 
 #define FK_MAX_LATTICES 8  // lattices of one chain set
+#define FK_SOLVER_MIXED32 20  // internal solver kind: blocked FP32 reduction of the mixed path
 
 // Batched calc_hamiltonian + calc_ed.  All pointers are device pointers.
 struct eig_args {

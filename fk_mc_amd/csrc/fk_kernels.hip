@@ -482,18 +482,22 @@ using namespace fkk;
 // Solver kinds: 0-2 register-resident one-stage, block-cyclic P x P grid (N <= 64 /
 // 128 / 256), 3 unblocked HBM workspace (N <= 2048), 4 blocked FP64 with MFMA
 // trailing updates (N <= 1024), 5 register-resident one-stage, wave-row layout
-// (N <= 256), 6 register-resident one-wave (N <= 64); 9 blocked FP32 (first stage of
-// the mixed-precision path, N <= 1024).
+// (N <= 256), 6 register-resident one-wave (N <= 64); 7, 8, 9 the blocked FP64 solver in
+// other tile shapes for the autotuner: 4 and 16 waves per matrix, and panels of at most 16
+// columns; 20 blocked FP32 (first stage of the mixed-precision path, N <= 1024).
 using fkw::wrow_solver;
 using fkq::wave_solver;
 typedef blocked_solver<double, 512> blk64;
+typedef blocked_solver<double, 256> blk64w4;
+typedef blocked_solver<double, 1024> blk64w16;
+typedef blocked_solver<double, 512, 16> blk64nb16;
 typedef blocked_solver<float, 512> blk32;
 
 int fk_solver_kind(int N) {
     if (N <= 64) return 0;
     if (N <= 128) return 1;
     if (N <= 256) return 5;
-    if (N <= blk64::NMAX) return 4;
+    if (N <= blk64w16::NMAX) return 8;
     if (N <= glob_solver::NMAX) return 3;
     return -1;
 }
@@ -506,7 +510,10 @@ int fk_solver_nmax(int kind) {
         case 4: return blk64::NMAX;
         case 5: return wrow_solver::NMAX;
         case 6: return wave_solver::NMAX;
-        case 9: return blk32::NMAX;
+        case 7: return blk64w4::NMAX;
+        case 8: return blk64w16::NMAX;
+        case 9: return blk64nb16::NMAX;
+        case FK_SOLVER_MIXED32: return blk32::NMAX;
     }
     return 0;
 }
@@ -519,16 +526,22 @@ int fk_solver_threads(int kind) {
         case 4: return blk64::NT;
         case 5: return wrow_solver::NT;
         case 6: return wave_solver::NT;
-        case 9: return blk32::NT;
+        case 7: return blk64w4::NT;
+        case 8: return blk64w16::NT;
+        case 9: return blk64nb16::NT;
+        case FK_SOLVER_MIXED32: return blk32::NT;
     }
     return 0;
 }
-int fk_solver_needs_ws(int kind) { return kind == 3 || kind == 4 || kind == 9; }
+int fk_solver_needs_ws(int kind) { return kind == 3 || kind == 4 || (kind >= 7 && kind <= 9) || kind == FK_SOLVER_MIXED32; }
 size_t fk_solver_ws_doubles(int kind, int N) {
     switch (kind) {
         case 3: return (size_t)N * N;
         case 4: return blk64::ws_doubles(N);
-        case 9: return blk32::ws_doubles(N);
+        case 7: return blk64w4::ws_doubles(N);
+        case 8: return blk64w16::ws_doubles(N);
+        case 9: return blk64nb16::ws_doubles(N);
+        case FK_SOLVER_MIXED32: return blk32::ws_doubles(N);
     }
     return 0;
 }
@@ -542,7 +555,10 @@ hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) 
         case 4: return launch_eig<blk64>(a, grid, st);
         case 5: return launch_eig<wrow_solver>(a, grid, st);
         case 6: return launch_eig<wave_solver>(a, grid, st);
-        case 9: return launch_eig<blk32>(a, grid, st);
+        case 7: return launch_eig<blk64w4>(a, grid, st);
+        case 8: return launch_eig<blk64w16>(a, grid, st);
+        case 9: return launch_eig<blk64nb16>(a, grid, st);
+        case FK_SOLVER_MIXED32: return launch_eig<blk32>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }
@@ -573,6 +589,9 @@ hipError_t fk_launch_sweep_q(int kind, const sweep_args& a, int nsub, const swee
         case 4: return launch_sweep_q<blk64>(a, nsub, q, st);
         case 5: return launch_sweep_q<wrow_solver>(a, nsub, q, st);
         case 6: return launch_sweep_q<wave_solver>(a, nsub, q, st);
+        case 7: return launch_sweep_q<blk64w4>(a, nsub, q, st);
+        case 8: return launch_sweep_q<blk64w16>(a, nsub, q, st);
+        case 9: return launch_sweep_q<blk64nb16>(a, nsub, q, st);
     }
     return hipErrorInvalidValue;
 }
@@ -586,6 +605,9 @@ hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t 
         case 4: return launch_sweep<blk64>(a, grid, st);
         case 5: return launch_sweep<wrow_solver>(a, grid, st);
         case 6: return launch_sweep<wave_solver>(a, grid, st);
+        case 7: return launch_sweep<blk64w4>(a, grid, st);
+        case 8: return launch_sweep<blk64w16>(a, grid, st);
+        case 9: return launch_sweep<blk64nb16>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }

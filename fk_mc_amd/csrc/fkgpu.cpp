@@ -516,7 +516,7 @@ static hipError_t mixed_alloc(fkgpu_ctx* ctx, int B, int N, mixed_bufs** out) {
     mixed_bufs* m = new mixed_bufs();
     m->B = B;
     m->N = N;
-    m->ws_stride = fk_solver_ws_doubles(9, N);
+    m->ws_stride = fk_solver_ws_doubles(FK_SOLVER_MIXED32, N);
     m->kind64 = pick_kind(ctx);
     m->ws64_stride = fk_solver_needs_ws(m->kind64) ? fk_solver_ws_doubles(m->kind64, N) : 0;
     const size_t BN = (size_t)B * N, BNN = BN * N;
@@ -569,7 +569,7 @@ static hipError_t mixed_eig_launch(fkgpu_ctx* ctx, mixed_bufs* m, int B, const i
     a.tauout = m->tau;
     a.d_out = m->d;
     a.e_out = m->e;
-    hipError_t e = fk_launch_eig(9, a, B, st);
+    hipError_t e = fk_launch_eig(FK_SOLVER_MIXED32, a, B, st);
     if (e != hipSuccess) return e;
     e = fk_launch_evecs(B, N, m->d, m->e, m->evh, m->vh, m->tau, m->X, m->scr, st, skip);
     if (e != hipSuccess) return e;
@@ -599,7 +599,7 @@ int fkgpu_eig_logz_mixed(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
     if (r) return r;
     if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
     const int N = ctx->N;
-    if (N > fk_solver_nmax(9)) return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
+    if (N > fk_solver_nmax(FK_SOLVER_MIXED32)) return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
     hipSetDevice(ctx->device);
     const size_t BN = (size_t)batch * N;
     int32_t* df;
@@ -1463,7 +1463,7 @@ int fkgpu_chains_create_multi(int ncontexts, fkgpu_ctx* const* contexts, int nch
     }
     // one solver for all chains: the register-resident wave-row solver when every N <= 256,
     // else the blocked HBM-workspace solver (both take N at run time)
-    const int kind = nmax <= 256 ? 5 : 4;
+    const int kind = nmax <= 256 ? 5 : 8;
     const int B = nchains;
     std::vector<int32_t> fpad((size_t)B * nmax, 0);
     for (int c = 0; c < B; ++c) {
@@ -1863,7 +1863,7 @@ int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
     if (ch->kpm) return fail(ctx, FKGPU_EUNSUPPORTED, "precision applies to the dense eigensolver, not the stochastic trace");
     if (ch->fu && precision == FKGPU_PRECISION_MIXED)
         return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one update mode is FP64; switch it off first");
-    if (precision == FKGPU_PRECISION_MIXED && N > fk_solver_nmax(9))
+    if (precision == FKGPU_PRECISION_MIXED && N > fk_solver_nmax(FK_SOLVER_MIXED32))
         return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;

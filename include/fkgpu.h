@@ -71,9 +71,12 @@ int fkgpu_msize(const fkgpu_ctx* ctx);
  *   4                       blocked (panel + MFMA rank-2nb trailing update), HBM workspace, N <= 1024
  *   5                       register-resident one-stage, 16 waves x 64 lanes (wave = row residue,
  *                           lane = column residue; row values by v_readlane), N <= 256
- *   6                       register-resident one-stage, one wave (lane = column), N <= 64 */
+ *   6                       register-resident one-stage, one wave (lane = column), N <= 64
+ *   7, 8, 9                 variant 4 in other tile shapes: 4 and 16 waves per matrix (256 and
+ *                           1024 threads), and panels of at most 16 columns, N <= 1024;
+ *                           8 is the default for 256 < N <= 1024 */
 #define FKGPU_SOLVER_AUTO (-1)
-#define FKGPU_SOLVER_VARIANTS 7
+#define FKGPU_SOLVER_VARIANTS 10
 int fkgpu_set_solver(fkgpu_ctx* ctx, int variant);
 int fkgpu_get_solver(const fkgpu_ctx* ctx); /* the variant used for the current lattice */
 /* Autotuner: times every variant that fits the lattice's N on `batch` synthetic

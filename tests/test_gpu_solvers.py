@@ -1,5 +1,6 @@
 """Dense eigensolver variants (register-resident one-stage, unblocked HBM,
-blocked with MFMA trailing updates, one-wave) against the oracle, and the autotuner."""
+blocked with MFMA trailing updates in four tile shapes, one-wave) against the oracle,
+and the autotuner."""
 import numpy as np
 import pytest
 
@@ -7,7 +8,7 @@ from test_gpu_chains import compare_chain
 
 pytestmark = pytest.mark.gpu
 
-VARIANT_NMAX = [64, 128, 256, 2048, 1024, 256, 64]
+VARIANT_NMAX = [64, 128, 256, 2048, 1024, 256, 64, 1024, 1024, 1024]
 CASES = [(0, 2, 8), (0, 2, 16), (1, 2, 18), (0, 3, 7), (0, 3, 8), (1, 2, 24), (0, 2, 32), (0, 1, 3), (0, 1, 5),
          (0, 1, 64), (1, 2, 7)]
 
@@ -55,7 +56,7 @@ def test_blocked_eigenvectors(fk, orc):
     ctx.close()
 
 
-@pytest.mark.parametrize("variant", [4, 3])
+@pytest.mark.parametrize("variant", [4, 3, 8])
 def test_chain_with_hbm_solvers(fk, orc, variant):
     """Chains at N = 343 through the HBM-workspace solvers reproduce the oracle chain."""
     D, L = 3, 7
@@ -87,5 +88,5 @@ def test_autotune(fk):
     assert ctx.get_solver() == best
     ctx.set_lattice(fk.LATTICE_HYPERCUBIC, 3, 8, 1.0, 0.0)
     best, ms = ctx.autotune(16)
-    assert list(ms[:3]) == [-1.0, -1.0, -1.0] and ms[5] == -1.0 and best in (3, 4)
+    assert list(ms[:3]) == [-1.0, -1.0, -1.0] and ms[5] == -1.0 and best in (3, 4, 7, 8, 9)
     ctx.close()

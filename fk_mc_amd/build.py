@@ -3,6 +3,8 @@
   fk_mc_amd/lib/libfkgpu.so    HIP kernels + C ABI (hipcc, gfx950)
   fk_mc_amd/lib/libfkmc_host.so  C++ host layer mirroring the reference classes
   fk_mc_amd/bin/fk_mc_gpu      the reference's Monte Carlo program on a batch of chains
+  fk_mc_amd/lib/libfkmc_h5.so  its HDF5 output (built where an HDF5 C library is found:
+                               $FK_HDF5_ROOT, /opt/conda, /usr, /usr/lib/x86_64-linux-gnu/hdf5/serial)
 
 Every call recompiles (the build check must exercise the compiler).
 """
@@ -63,7 +65,33 @@ def build_driver():
     out = os.path.join(HERE, "bin", "fk_mc_gpu")
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", tmp, src,
-          "-L", LIB, "-lfkgpu", "-Wl,-rpath,$ORIGIN/../lib"])
+          "-L", LIB, "-lfkgpu", "-ldl", "-Wl,-rpath,$ORIGIN/../lib"])
+    os.replace(tmp, out)
+    return out
+
+
+def find_hdf5():
+    roots = [os.environ.get("FK_HDF5_ROOT", ""), "/opt/conda", "/usr", "/usr/lib/x86_64-linux-gnu/hdf5/serial"]
+    for r in roots:
+        if r and os.path.exists(os.path.join(r, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(r, "lib", "libhdf5.so")):
+            return r
+    return None
+
+
+def build_h5():
+    out = os.path.join(LIB, "libfkmc_h5.so")
+    root = find_hdf5()
+    if root is None:
+        print("no HDF5 C library found: libfkmc_h5.so not built (fk_mc_gpu then writes .npy "
+              "directories only)", flush=True)
+        if os.path.exists(out):
+            os.remove(out)
+        return None
+    os.makedirs(LIB, exist_ok=True)
+    tmp = out + ".tmp"
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-fPIC", "-shared", "-I", os.path.join(root, "include"), "-o", tmp,
+          "host/h5archive.c", "-L", os.path.join(root, "lib"), "-lhdf5", "-Wl,-rpath," + os.path.join(root, "lib")])
     os.replace(tmp, out)
     return out
 
@@ -72,6 +100,7 @@ def build_all():
     build_gpu()
     build_host()
     build_driver()
+    build_h5()
 
 
 if __name__ == "__main__":

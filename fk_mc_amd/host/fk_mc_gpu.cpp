@@ -18,21 +18,24 @@
 // conductivity when measure_stiffness, eigenvectors when measure_eigenfunctions), all kept
 // on the device until the end.
 //
-// Output (`output`, a directory; a trailing .h5 is dropped): the observables of
-// observables_t (include/fk_mc/fk_mc.hpp:13-28) under mc_data/ as .npy files with the
-// chain as the leading axis -- energies, d2energies, c_energies [chains][nsweeps],
-// spectrum [chains][N] (running mean), spectrum_history, focc_history, ipr_history
+// Output (`output`: an HDF5 file when it ends in .h5, as the reference writes, else a
+// directory of .npy files with the same paths): the observables of observables_t
+// (include/fk_mc/fk_mc.hpp:13-28) under mc_data/ with the chain as the leading axis (dropped
+// in an HDF5 file of one chain, which then has the reference's shapes) -- energies,
+// d2energies, c_energies [chains][nsweeps], spectrum [chains][N] (running mean), spectrum_history, focc_history, ipr_history
 // [chains][nsweeps][N], stiffness [chains][nsweeps] and cond_history [chains][nsweeps][nw]
 // with measure_stiffness on the reference's conductivity grid, eig_history
 // [chains][nsweeps][N][N] with save_eigenfunctions -- plus, with measure_eigenfunctions, the
 // outputs of save_gwr under stats/ (tdos_gwr, tdos<w>, gr_full_w<w>_re/_im, gr_w<w>, gk_w<w>
-// at w = 0 + i dos_offset) -- plus dos.npy (the DOS of the recorded spectra on the
-// reference's real grid, prog/data_save.hxx:67-69,264-345), parameters.txt and
-// stats.txt (means with errors from the spread over chains, and the specific heat
+// at w = 0 + i dos_offset) -- plus dos (the DOS of the recorded spectra on the reference's
+// real grid, prog/data_save.hxx:67-69,264-345), the parameters (/parameters in an HDF5
+// file, parameters.txt in a directory) and stats.txt (means with errors from the spread over chains, and the specific heat
 // beta^2 (<E^2> - <d2E> - <E>^2) / N of prog/data_save.hxx:185-190), and the reference's
 // binning / jackknife analysis under binning/ and stats/ (energies, cv, dos0 / dos_err and,
 // with measure_ipr, ipr0 / ipr_err).
+#include <dlfcn.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cerrno>
@@ -122,7 +125,7 @@ params define_parameters() {
     p.define("t", "1", "Hopping");
     p.define("tp", "1", "Triangular lattice : NNN Hopping");
     p.define("W", "", "1d : f-f interaction");
-    p.define("output", "output.h5", "directory to write data to (a trailing .h5 is dropped)");
+    p.define("output", "output.h5", "HDF5 file to write data to, or a directory for .npy files (no .h5 suffix)");
     p.define("plaintext", "false", "also write the series as text (.dat)");
     p.define("dos_width", "6", "Width of DOS");
     p.define("dos_npts", "240", "Number of points for DOS sampling");
@@ -196,6 +199,126 @@ void mkdirs(const std::string& path) {
     }
 }
 
+// Where the run's datasets go (prog/data_save.hxx:32-151): with `output` ending in .h5 one
+// HDF5 file, dataset /<path> for each path (through libfkmc_h5.so, loaded here only then),
+// as the reference writes; otherwise a directory holding <path>.npy.  With one chain the HDF5
+// datasets of mc_data/ drop the chain axis, so that they have the reference's shapes.  Text
+// files (parameters.txt in directory mode, stats.txt, the plaintext .dat files) go to the
+// directory, or for an HDF5 file to the directory that holds it, as the reference's savetxt
+// writes next to its output.h5 in the working directory.
+class archive {
+public:
+    archive(const std::string& output, int chains) : chains_(chains) {
+        const bool h5 = output.size() > 3 && output.substr(output.size() - 3) == ".h5";
+        if (!h5) {
+            dir_ = output;
+            mkdirs(dir_);
+            return;
+        }
+        const size_t slash = output.find_last_of('/');
+        dir_ = slash == std::string::npos ? "." : output.substr(0, slash);
+        if (dir_.empty()) dir_ = "/";
+        mkdirs(dir_);
+        char exe[4096];
+        const ssize_t n = readlink("/proc/self/exe", exe, sizeof exe - 1);
+        std::string lib = "libfkmc_h5.so";
+        if (n > 0) {
+            exe[n] = 0;
+            std::string d(exe);
+            d = d.substr(0, d.find_last_of('/'));
+            lib = d + "/../lib/libfkmc_h5.so";
+        }
+        so_ = dlopen(lib.c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!so_)
+            throw std::runtime_error("HDF5 output (`output` ending in .h5) needs " + lib + ", built where an HDF5 C " +
+                                     "library is installed (" + dlerror() + "); give `output` a directory name for .npy output");
+        create_ = (void* (*)(const char*))dlsym(so_, "fkh5_create");
+        f64_ = (int (*)(void*, const char*, const double*, int, const uint64_t*))dlsym(so_, "fkh5_write_f64");
+        i64_ = (int (*)(void*, const char*, const int64_t*, int, const uint64_t*))dlsym(so_, "fkh5_write_i64");
+        str_ = (int (*)(void*, const char*, const char*))dlsym(so_, "fkh5_write_str");
+        close_ = (int (*)(void*))dlsym(so_, "fkh5_close");
+        err_ = (const char* (*)())dlsym(so_, "fkh5_error");
+        if (!create_ || !f64_ || !i64_ || !str_ || !close_ || !err_) throw std::runtime_error(lib + ": missing symbols");
+        h_ = create_(output.c_str());
+        if (!h_) throw std::runtime_error(err_());
+    }
+    ~archive() {
+        if (h_) close_(h_);
+    }
+    bool hdf5() const { return h_ != nullptr; }
+    // dataset `path` (groups separated by '/'); chain_axis: shape[0] is the chain
+    void put(const std::string& path, const double* data, std::vector<size_t> shape, bool chain_axis = false) {
+        if (!h_) {
+            const size_t slash = path.find_last_of('/');
+            if (slash != std::string::npos) mkdirs(dir_ + "/" + path.substr(0, slash));
+            write_npy(dir_ + "/" + path + ".npy", data, shape);
+            return;
+        }
+        if (chain_axis && chains_ == 1 && shape.size() > 1) shape.erase(shape.begin());
+        std::vector<uint64_t> d(shape.begin(), shape.end());
+        if (f64_(h_, path.c_str(), data, (int)d.size(), d.data()) != 0) throw std::runtime_error(err_());
+    }
+    // /parameters/<name> (the reference's alps::hdf5::save of its parameters): integers as
+    // int64, other numbers as float64, true / false as int64 1 / 0, anything else a string;
+    // in directory mode `name = value` lines of parameters.txt
+    void put_parameters(const std::vector<std::pair<std::string, std::string>>& kv) {
+        if (!h_) {
+            std::ofstream pf(dir_ + "/parameters.txt");
+            for (auto& e : kv) pf << e.first << " = " << e.second << "\n";
+            return;
+        }
+        for (auto& e : kv) {
+            const std::string path = "parameters/" + e.first, &v = e.second;
+            int r;
+            char* end = nullptr;
+            errno = 0;
+            const long long iv = std::strtoll(v.c_str(), &end, 10);
+            if (!v.empty() && *end == 0 && errno == 0) {
+                const int64_t x = iv;
+                r = i64_(h_, path.c_str(), &x, 0, nullptr);
+            } else if (v == "true" || v == "false") {
+                const int64_t x = v == "true";
+                r = i64_(h_, path.c_str(), &x, 0, nullptr);
+            } else {
+                const double dv = std::strtod(v.c_str(), &end);
+                if (!v.empty() && *end == 0)
+                    r = f64_(h_, path.c_str(), &dv, 0, nullptr);
+                else
+                    r = str_(h_, path.c_str(), v.c_str());
+            }
+            if (r != 0) throw std::runtime_error(err_());
+        }
+    }
+    // path of text file `name` (may name a group directory, kept in directory mode)
+    std::string text(const std::string& name) const {
+        const size_t slash = name.find_last_of('/');
+        if (slash == std::string::npos) return dir_ + "/" + name;
+        if (h_) return dir_ + "/" + name.substr(slash + 1);
+        mkdirs(dir_ + "/" + name.substr(0, slash));
+        return dir_ + "/" + name;
+    }
+    void close() {
+        if (h_ && close_(h_) != 0) {
+            h_ = nullptr;
+            throw std::runtime_error(err_());
+        }
+        h_ = nullptr;
+    }
+    const std::string& dir() const { return dir_; }
+
+private:
+    int chains_;
+    std::string dir_;
+    void* so_ = nullptr;
+    void* h_ = nullptr;
+    void* (*create_)(const char*) = nullptr;
+    int (*f64_)(void*, const char*, const double*, int, const uint64_t*) = nullptr;
+    int (*i64_)(void*, const char*, const int64_t*, int, const uint64_t*) = nullptr;
+    int (*str_)(void*, const char*, const char*) = nullptr;
+    int (*close_)(void*) = nullptr;
+    const char* (*err_)() = nullptr;
+};
+
 struct mean_err {
     double mean, err;
 };
@@ -233,20 +356,20 @@ std::vector<double> tdos_row(double w, double off, const double* ld, int N) {
     return {w, off, geom, 0.0, dos, 0.0, geom / dos, 0.0};
 }
 
-void save_gwr(const std::string& dir, int D, int L, int N, double off, const std::vector<double>& gre,
+void save_gwr(archive& ar, int D, int L, int N, double off, const std::vector<double>& gre,
               const std::vector<double>& gim, const std::vector<double>& grid, const std::vector<double>& ldos,
               bool plaintext) {
-    mkdirs(dir);
+    const std::string dir = "stats/";
     const std::string ws = wlabel(0.0) + "_" + wlabel(off);
     std::vector<double> ld0(N);
     for (int i = 0; i < N; ++i) ld0[i] = gim[(size_t)i * N + i] / -M_PI;
     const std::vector<double> t0 = tdos_row(0.0, off, ld0.data(), N);
-    write_npy(dir + "tdos" + ws + ".npy", t0.data(), {8});
-    write_npy(dir + "gr_full_w" + ws + "_re.npy", gre.data(), {(size_t)N, (size_t)N});
-    write_npy(dir + "gr_full_w" + ws + "_im.npy", gim.data(), {(size_t)N, (size_t)N});
+    ar.put(dir + "tdos" + ws, t0.data(), {8});
+    ar.put(dir + "gr_full_w" + ws + "_re", gre.data(), {(size_t)N, (size_t)N});
+    ar.put(dir + "gr_full_w" + ws + "_im", gim.data(), {(size_t)N, (size_t)N});
     if (plaintext) {
-        write_txt(dir + "gr_full_w" + ws + "_re.dat", gre.data(), N, N);
-        write_txt(dir + "gr_full_w" + ws + "_im.dat", gim.data(), N, N);
+        write_txt(ar.text(dir + "gr_full_w" + ws + "_re.dat"), gre.data(), N, N);
+        write_txt(ar.text(dir + "gr_full_w" + ws + "_im.dat"), gim.data(), N, N);
     }
     if (D >= 2) {
         // G(r_j - r_i) over the first two lattice axes (the site index runs fastest along
@@ -281,10 +404,10 @@ void save_gwr(const std::string& dir, int D, int L, int N, double off, const std
                 kre[(size_t)k0 * L + k1] = sr;
                 kim[(size_t)k0 * L + k1] = si;
             }
-        write_npy(dir + "gr_w" + ws + "_re.npy", r2re.data(), {(size_t)L, (size_t)L});
-        write_npy(dir + "gr_w" + ws + "_im.npy", r2im.data(), {(size_t)L, (size_t)L});
-        write_npy(dir + "gk_w" + ws + "_re.npy", kre.data(), {(size_t)L, (size_t)L});
-        write_npy(dir + "gk_w" + ws + "_im.npy", kim.data(), {(size_t)L, (size_t)L});
+        ar.put(dir + "gr_w" + ws + "_re", r2re.data(), {(size_t)L, (size_t)L});
+        ar.put(dir + "gr_w" + ws + "_im", r2im.data(), {(size_t)L, (size_t)L});
+        ar.put(dir + "gk_w" + ws + "_re", kre.data(), {(size_t)L, (size_t)L});
+        ar.put(dir + "gk_w" + ws + "_im", kim.data(), {(size_t)L, (size_t)L});
     }
     const int npts = (int)grid.size();
     std::vector<double> tv;
@@ -293,8 +416,8 @@ void save_gwr(const std::string& dir, int D, int L, int N, double off, const std
         tv.insert(tv.end(), r.begin(), r.end());
     }
     if (!tv.empty()) {
-        write_npy(dir + "tdos_gwr.npy", tv.data(), {(size_t)npts, 8});
-        if (plaintext) write_txt(dir + "tdos_gwr.dat", tv.data(), npts, 8);
+        ar.put(dir + "tdos_gwr", tv.data(), {(size_t)npts, 8});
+        if (plaintext) write_txt(ar.text(dir + "tdos_gwr.dat"), tv.data(), npts, 8);
     }
 }
 
@@ -328,6 +451,8 @@ int run(params& p) {
     // with measure_ipr (calc_spectrum); without the spectrum measures only occupations are kept
     const bool spectra = !cheb || ipr || eig || p.b("measure_stiffness");
 
+    // opened before the run, so that an output that cannot be written ends it at once
+    archive ar(p.str("output"), R);
     fkgpu_ctx* ctx = nullptr;
     if (fkgpu_create((int)p.i("device"), &ctx) != FKGPU_OK) throw std::runtime_error("no HIP device");
     FK_CHECK(fkgpu_set_lattice(ctx, kind, D, L, p.d("t"), kind == FKGPU_LATTICE_TRIANGULAR ? p.d("tp") : 0.0), ctx);
@@ -465,39 +590,37 @@ int run(params& p) {
     fkgpu_chains_destroy(ch);
     fkgpu_destroy(ctx);
 
-    std::string out = p.str("output");
-    if (out.size() > 3 && out.substr(out.size() - 3) == ".h5") out = out.substr(0, out.size() - 3);
-    mkdirs(out + "/mc_data");
-    const std::string md = out + "/mc_data/";
+    const std::string md = "mc_data/";
     if (spectra) {
-        write_npy(md + "energies.npy", e.data(), {(size_t)R, S});
-        write_npy(md + "d2energies.npy", d2.data(), {(size_t)R, S});
-        write_npy(md + "c_energies.npy", ce.data(), {(size_t)R, S});
-        write_npy(md + "spectrum.npy", spec.data(), {(size_t)R, (size_t)N});
+        ar.put(md + "energies", e.data(), {(size_t)R, S}, true);
+        ar.put(md + "d2energies", d2.data(), {(size_t)R, S}, true);
+        ar.put(md + "c_energies", ce.data(), {(size_t)R, S}, true);
+        ar.put(md + "spectrum", spec.data(), {(size_t)R, (size_t)N}, true);
     }
     if (history) {
-        if (shist) write_npy(md + "spectrum_history.npy", sh.data(), {(size_t)R, S, (size_t)N});
+        if (shist) ar.put(md + "spectrum_history", sh.data(), {(size_t)R, S, (size_t)N}, true);
         std::vector<double> fd(fh.begin(), fh.end());
-        write_npy(md + "focc_history.npy", fd.data(), {(size_t)R, S, (size_t)N});
+        ar.put(md + "focc_history", fd.data(), {(size_t)R, S, (size_t)N}, true);
     }
-    if (ipr) write_npy(md + "ipr_history.npy", ih.data(), {(size_t)R, S, (size_t)N});
-    if (!evh.empty()) write_npy(md + "eig_history.npy", evh.data(), {(size_t)R, S, (size_t)N, (size_t)N});
-    if (gwr) save_gwr(out + "/stats/", D, L, N, doff, g0re, g0im, grid, ldos, p.b("plaintext"));
+    if (ipr) ar.put(md + "ipr_history", ih.data(), {(size_t)R, S, (size_t)N}, true);
+    if (!evh.empty()) ar.put(md + "eig_history", evh.data(), {(size_t)R, S, (size_t)N, (size_t)N}, true);
+    if (gwr) save_gwr(ar, D, L, N, doff, g0re, g0im, grid, ldos, p.b("plaintext"));
     if (stiff) {
-        write_npy(md + "stiffness.npy", stf.data(), {(size_t)R, S});
-        write_npy(md + "cond_history.npy", cnd.data(), {(size_t)R, S, NW});
-        write_npy(out + "/wgrid_conductivity.npy", wgrid.data(), {NW});
+        ar.put(md + "stiffness", stf.data(), {(size_t)R, S}, true);
+        ar.put(md + "cond_history", cnd.data(), {(size_t)R, S, NW}, true);
+        ar.put("wgrid_conductivity", wgrid.data(), {NW});
     }
     std::vector<double> dg(2 * npts);
     for (int i = 0; i < npts; ++i) dg[2 * i] = grid[i], dg[2 * i + 1] = dos[i];
-    if (spectra) write_npy(out + "/dos.npy", dg.data(), {(size_t)npts, 2});
+    if (spectra) ar.put("dos", dg.data(), {(size_t)npts, 2});
     if (p.b("plaintext") && spectra) {
-        write_txt(out + "/energies.dat", e.data(), R, S);
-        write_txt(out + "/dos.dat", dg.data(), npts, 2);
+        write_txt(ar.text("energies.dat"), e.data(), R, S);
+        write_txt(ar.text("dos.dat"), dg.data(), npts, 2);
     }
     {
-        std::ofstream pf(out + "/parameters.txt");
-        for (auto& k : p.order) pf << k << " = " << p.str(k) << "\n";
+        std::vector<std::pair<std::string, std::string>> kv;
+        for (auto& k : p.order) kv.emplace_back(k, p.str(k));
+        ar.put_parameters(kv);
     }
     // statistics: per-chain means, errors from the spread over the chains
     std::vector<double> cv(R), eN(R), accr(R);
@@ -519,7 +642,7 @@ int run(params& p) {
     std::vector<double> nfm(R, 0.0);
     for (int r = 0; r < R; ++r)
         for (int i = 0; i < N; ++i) nfm[r] += focc[(size_t)r * N + i] / N;
-    std::ofstream st(out + "/stats.txt");
+    std::ofstream st(ar.text("stats.txt"));
     st.precision(12);
     auto line = [&](const char* name, const std::vector<double>& v) {
         const mean_err me = over_chains(v);
@@ -540,8 +663,6 @@ int run(params& p) {
     // with measure_ipr, ipr0 / ipr_err
     if (S > 0 && count > 0 && spectra) {
         namespace bn = fkmc::binning;
-        mkdirs(out + "/binning");
-        mkdirs(out + "/stats");
         const size_t nm = (size_t)R * std::min<int64_t>(count, (int64_t)S);
         const int maxb = bn::max_bin(nm);
         auto series = [&](const double* x, size_t stride, size_t n) {  // reversed
@@ -553,10 +674,10 @@ int run(params& p) {
             const std::vector<double> cl = bn::cor_length(sv);
             std::vector<double> t;
             for (size_t i = 0; i < sv.size(); ++i) t.insert(t.end(), {sv[i].size, sv[i].mean, sv[i].disp, sv[i].sqerror, cl[i]});
-            write_npy(out + "/binning/" + name + ".npy", t.data(), {sv.size(), 5});
+            ar.put("binning/" + name, t.data(), {sv.size(), 5});
             const int b = bn::estimate_bin(sv);
             const double row[4] = {sv[b].size, sv[b].mean, sv[b].disp, sv[b].sqerror};
-            write_npy(out + "/stats/" + name + ".npy", row, {4});
+            ar.put("stats/" + name, row, {4});
             st << name << " = " << sv[b].mean << " +- " << sv[b].sqerror << " (" << sv[b].size << " samples, bin "
                << b << ")\n";
             return b;
@@ -580,16 +701,18 @@ int run(params& p) {
                 const bn::stats x = bn::bin(series(rec.data() + i, W, nm), b);
                 t.insert(t.end(), {grid[i], x.mean, x.sqerror});
             }
-            write_npy(out + "/stats/" + name + "_err.npy", t.data(), {(size_t)npts, 3});
-            if (p.b("plaintext")) write_txt(out + "/stats/" + name + "_err.dat", t.data(), npts, 3);
+            ar.put("stats/" + name + "_err", t.data(), {(size_t)npts, 3});
+            if (p.b("plaintext")) write_txt(ar.text("stats/" + name + "_err.dat"), t.data(), npts, 3);
         };
         if (!drec.empty() && nm * (npts + 1) == drec.size()) err_table("dos", drec);
         if (!irec.empty()) err_table("ipr", irec);
     }
     st << "seconds = " << secs << "\nmoves_per_second = " << (double)R * (ntherm + nsweeps) * sweep_len / secs
        << "\n";
-    if (show) std::printf("  %.3f s, %.4g moves/s, output in %s/\n", secs,
-                          (double)R * (ntherm + nsweeps) * sweep_len / secs, out.c_str());
+    st.close();
+    ar.close();
+    if (show) std::printf("  %.3f s, %.4g moves/s, output in %s\n", secs,
+                          (double)R * (ntherm + nsweeps) * sweep_len / secs, p.str("output").c_str());
     return 0;
 }
 

@@ -14,9 +14,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "fk_mc_amd", "bin", "fk_mc_gpu")
 
 
-def _run(tmp_path, *args):
-    out = tmp_path / "run"
-    cmd = [EXE] + [str(a) for a in args] + ["--output", str(out) + ".h5", "--show_output", "false"]
+def _run(tmp_path, *args, output="run"):
+    out = tmp_path / output
+    cmd = [EXE] + [str(a) for a in args] + ["--output", str(out), "--show_output", "false"]
     print("case driver", " ".join(cmd[1:]), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr

@@ -88,20 +88,138 @@ struct wrow_solver {
     __device__ static const double* eptr(const smem& sm) { return sm.e; }
     __device__ static double* evptr(smem& sm) { return sm.u.ea.ev; }
 
-    __device__ __forceinline__ static void emit(const double (&A)[NS], smem& sm, double* xbuf, int kk, int N, int w,
-                                                int l) {
-        if (l != (kk & 63)) return;
-        const int bb = kk >> 6;
+    // A[i][j] (a stored slot: i / 64 >= j / 64) from the thread that holds it into *dst
+    __device__ __forceinline__ static void put_elem(const double (&A)[NS], int i, int j, int w, int l, double* dst) {
+        if (w != (i & 15) || l != (j & 63)) return;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b <= (a >> 2); ++b)
+                if (a == (i >> 4) && b == (j >> 6)) *dst = A[sl(a, b)];
+    }
+
+    // Householder step k with the BM = k / 64 finished column blocks known at compile time:
+    // slots of those blocks take no part, and every other skip is per 64-row group.
+    template <int BM>
+    __device__ __forceinline__ static void step(double (&A)[NS], smem& sm, int N, int k, int w, int l, int tid,
+                                                double* __restrict__ vout, double* __restrict__ tauout) {
+        double* xk = sm.x[k & 1];
+        // column k of A (its block is BM) from lane k % 64 of every wave, rows above k included
+        // (never read); the previous step's readers of this buffer passed two barriers since
+        if (l == (k & 63)) {
+#pragma unroll
+            for (int a = 4 * BM; a < NA; ++a) xk[w + 16 * a] = A[sl(a, BM)];
+        }
+        __syncthreads();
+        // reflector from x = A[k+1:N, k] (every wave redundantly), the lane's entries l + 64b
+        const double alpha = xk[k + 1];
+        double xb[NBC], part = 0.0;
+#pragma unroll
+        for (int b = BM; b < NBC; ++b) {
+            const int j = l + 64 * b;
+            xb[b] = xk[j];
+            part += (j >= k + 2 && j < N) ? xb[b] * xb[b] : 0.0;
+        }
+        const double xn2 = wave_sum(part);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xn2 != 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            tau = (beta - alpha) * rcp_refined(beta);
+            scal = rcp_refined(alpha - beta);
+        }
+        if (tid == 0) {
+            sm.e[k] = beta;
+            sm.d[k] = xk[k];  // A[k][k] is final after step k - 1
+        }
+        if (vout) {
+            for (int i = tid; i < N; i += NT)
+                vout[(size_t)k * N + i] = (i == k + 1) ? 1.0 : ((i > k + 1) ? xk[i] * scal : 0.0);
+            if (tid == 0) tauout[k] = tau;
+        }
+        tau = uniform(tau);
+        scal = uniform(scal);
+        // v at the lane's columns; every wave stores the same values to sm.v, from where the
+        // wave reads v at its rows (its own stores are ordered before its loads)
+        double vc[NBC];
 #pragma unroll
         for (int b = 0; b < NBC; ++b) {
-            if (b != bb) continue;
+            const int j = l + 64 * b;
+            vc[b] = b < BM ? 0.0 : ((j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? xb[b] * scal : 0.0));
+            if (b >= BM) sm.v[j] = vc[b];
+        }
+        if (tau != 0.0) {
+            // row sums over the lanes' columns (four groups of 4 rows); rows of finished
+            // groups are not needed (p is zero there)
 #pragma unroll
-            for (int a = 4 * b; a < NA; ++a) {
+            for (int h = BM; h < 4; ++h) {
+                double rp[4];
+#pragma unroll
+                for (int aa = 0; aa < 4; ++aa) {
+                    const int a = 4 * h + aa;
+                    double s = 0.0;
+#pragma unroll
+                    for (int b = BM; b <= h; ++b) s = fma(A[sl(a, b)], vc[b], s);
+                    rp[aa] = s;
+                }
+                const double rsum = reduce4(rp);
+                if ((l & 15) == 0) {
+                    const int a = 4 * h + 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
+                    sm.rs[w + 16 * a] = rsum;
+                }
+            }
+            // column sums of the blocks strictly below the diagonal blocks, row values by
+            // broadcast LDS reads (v is zero on finished rows)
+            double cs[NBC];
+#pragma unroll
+            for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
+#pragma unroll
+            for (int a = 4 * (BM + 1); a < NA; ++a) {
+                const double vr = sm.v[w + 16 * a];
+#pragma unroll
+                for (int b = BM; b < (a >> 2); ++b) cs[b] = fma(A[sl(a, b)], vr, cs[b]);
+            }
+#pragma unroll
+            for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];
+        }
+        __syncthreads();
+        if (tau != 0.0) {
+            // p_i = tau (row sum + 16 column partials): four lanes per row; entries of finished
+            // rows and columns hold older partials and are masked
+            const int i = tid >> 2, q4 = tid & 3;
+            double s = (q4 == 0) ? sm.rs[i] : 0.0;
+#pragma unroll
+            for (int ww = 0; ww < 4; ++ww) s += sm.u.cpart[4 * q4 + ww][i];
+            s = quad_sum(s);
+            const double pi = (i > k && i < N) ? tau * s : 0.0;
+            if (q4 == 0) sm.p[i] = pi;
+            const double vi = (i > k) ? sm.v[i] : 0.0;
+            const double pv = wave_sum(q4 == 0 ? pi * vi : 0.0);
+            if (l == 0) sm.red[tid / WAVE] = pv;
+        }
+        __syncthreads();
+        if (tau != 0.0) {
+            double pvt = 0.0;  // fixed order: bitwise reproducible
+#pragma unroll
+            for (int q = 0; q < NWV; ++q) pvt += sm.red[q];
+            const double coef = uniform(0.5 * tau * pvt);
+            double wc[NBC];
+#pragma unroll
+            for (int b = BM; b < NBC; ++b) wc[b] = fma(-coef, vc[b], sm.p[l + 64 * b]);
+#pragma unroll
+            for (int a = 4 * BM; a < NA; ++a) {
+                // row values by broadcast LDS reads, w_i formed as the lanes form w_j; a
+                // finished row (v_i = p_i = 0) is left as it is
                 const int i = w + 16 * a;
-                if (i > kk && i < N) xbuf[i] = A[sl(a, b)];
-                if (i == kk) sm.d[kk] = A[sl(a, b)];
+                const double vra = sm.v[i];
+                const double wra = fma(-coef, vra, sm.p[i]);
+#pragma unroll
+                for (int b = BM; b <= (a >> 2); ++b)
+                    A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
             }
         }
+        // no barrier here: the next step's first LDS writes (column k + 1 into the other x
+        // buffer) conflict with no read of this step after the barrier above, and its barrier
+        // follows them
     }
 
     __device__ __forceinline__ static void tridiagonalize(double (&A)[NS], smem& sm, int N, double* __restrict__ vout,
@@ -109,128 +227,23 @@ struct wrow_solver {
         const int tid = threadIdx.x;
         const int l = tid & (WAVE - 1);
         const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
-        for (int i = tid; i < NP; i += NT) { sm.x[0][i] = 0.0; sm.x[1][i] = 0.0; sm.d[i] = 0.0; sm.e[i] = 0.0; }
-        __syncthreads();
-        emit(A, sm, sm.x[0], 0, N, w, l);
-        __syncthreads();
-        for (int k = 0; k + 2 < N; ++k) {
-            const double* xk = sm.x[k & 1];
-            double* xn = sm.x[(k + 1) & 1];
-            // reflector from x = A[k+1:N, k] (every wave redundantly)
-            const double alpha = xk[k + 1];
-            double part = 0.0;
-            for (int i = k + 2 + l; i < N; i += WAVE) part += xk[i] * xk[i];
-            const double xn2 = wave_sum(part);
-            double tau = 0.0, beta = alpha, scal = 0.0;
-            if (xn2 != 0.0) {
-                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-                tau = (beta - alpha) * rcp_refined(beta);
-                scal = rcp_refined(alpha - beta);
-            }
-            if (tid == 0) sm.e[k] = beta;
-            if (vout) {
-                for (int i = tid; i < N; i += NT)
-                    vout[(size_t)k * N + i] = (i == k + 1) ? 1.0 : ((i > k + 1) ? xk[i] * scal : 0.0);
-                if (tid == 0) tauout[k] = tau;
-            }
-            tau = uniform(tau);
-            scal = uniform(scal);
-            const int bmin = k >> 6;  // column blocks below bmin are finished
-            double vc[NBC];
-#pragma unroll
-            for (int b = 0; b < NBC; ++b) {
-                const int j = l + 64 * b;
-                vc[b] = (j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? xk[j] * scal : 0.0);
-            }
-            // Slots of finished rows / columns take part with v = w = 0 there (p is zero
-            // on finished indices), which leaves them unchanged; work is skipped only per
-            // 64-wide block, so every wave runs the same instruction stream.
-            if (tau != 0.0) {
-                // row sums over the lanes' columns (four groups of 4 rows)
-#pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    if (64 * h + 63 <= k) continue;
-                    double rp[4];
-#pragma unroll
-                    for (int aa = 0; aa < 4; ++aa) {
-                        const int a = 4 * h + aa;
-                        double s = 0.0;
-#pragma unroll
-                        for (int b = 0; b <= (a >> 2); ++b) s = fma(A[sl(a, b)], vc[b], s);
-                        rp[aa] = s;
-                    }
-                    const double rsum = reduce4(rp);
-                    if ((l & 15) == 0) {
-                        const int a = 4 * h + 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
-                        sm.rs[w + 16 * a] = rsum;
-                    }
-                }
-                // column sums of the blocks strictly below the diagonal blocks (rows
-                // i <= k, where v_i = 0, skipped per wave)
-#pragma unroll
-                for (int b = 0; b < NBC; ++b) {
-                    double s = 0.0;
-                    if (b >= bmin) {
-#pragma unroll
-                        for (int a = 4 * (b + 1); a < NA; ++a) {
-                            if (w + 16 * a <= k) continue;  // wave-uniform
-                            const double vr = readlane_d(vc[a >> 2], (w + 16 * a) & 63);
-                            s = fma(A[sl(a, b)], vr, s);
-                        }
-                    }
-                    sm.u.cpart[w][l + 64 * b] = s;
-                }
-            }
-            __syncthreads();
-            if (tau != 0.0) {
-                // p_i = tau (row sum + 16 column partials): four lanes per row
-                const int i = tid >> 2, q4 = tid & 3;
-                double s = (q4 == 0) ? sm.rs[i] : 0.0;
-#pragma unroll
-                for (int ww = 0; ww < 4; ++ww) s += sm.u.cpart[4 * q4 + ww][i];
-                s = quad_sum(s);
-                const double pi = (i > k && i < N) ? tau * s : 0.0;
-                if (q4 == 0) sm.p[i] = pi;
-                const double vi = (i == k + 1) ? 1.0 : ((i > k + 1 && i < N) ? xk[i] * scal : 0.0);
-                if (q4 == 0) sm.v[i] = vi;
-                const double pv = wave_sum(q4 == 0 ? pi * vi : 0.0);
-                if (l == 0) sm.red[tid / WAVE] = pv;
-            }
-            __syncthreads();
-            if (tau != 0.0) {
-                double pvt = 0.0;  // fixed order: bitwise reproducible
-#pragma unroll
-                for (int q = 0; q < NWV; ++q) pvt += sm.red[q];
-                const double coef = uniform(0.5 * tau * pvt);
-                double wc[NBC];
-#pragma unroll
-                for (int b = 0; b < NBC; ++b) wc[b] = fma(-coef, vc[b], sm.p[l + 64 * b]);
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    if (g < bmin) continue;
-#pragma unroll
-                    for (int a = 4 * g; a < 4 * g + 4; ++a) {
-                        // row values by broadcast LDS reads, w_i formed as the lanes form w_j;
-                        // a finished row (i <= k: v_i = w_i = 0) is left as it is (per wave)
-                        const int i = w + 16 * a;
-                        const double vra = sm.v[i];
-                        const double wra = fma(-coef, vra, sm.p[i]);
-#pragma unroll
-                        for (int b = 0; b <= g; ++b) A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
-                    }
-                }
-            }
-            emit(A, sm, xn, k + 1, N, w, l);
-            __syncthreads();
+        for (int i = tid; i < NP; i += NT) {
+            sm.x[0][i] = 0.0; sm.x[1][i] = 0.0; sm.d[i] = 0.0; sm.e[i] = 0.0;
+            sm.v[i] = 0.0; sm.p[i] = 0.0; sm.rs[i] = 0.0;
         }
+        for (int i = tid; i < NWV * NP; i += NT) (&sm.u.cpart[0][0])[i] = 0.0;
+        __syncthreads();
+        int k = 0;
+        for (; k + 2 < N && k < 64; ++k) step<0>(A, sm, N, k, w, l, tid, vout, tauout);
+        for (; k + 2 < N && k < 128; ++k) step<1>(A, sm, N, k, w, l, tid, vout, tauout);
+        for (; k + 2 < N && k < 192; ++k) step<2>(A, sm, N, k, w, l, tid, vout, tauout);
+        for (; k + 2 < N; ++k) step<3>(A, sm, N, k, w, l, tid, vout, tauout);
+        __syncthreads();
+        // the last two diagonal entries and the last off-diagonal one, from their owners
         if (N >= 2) {
-            if (tid == 0) sm.e[N - 2] = sm.x[(N - 2) & 1][N - 1];
-            const int kk = N - 1;
-            if (w == (kk & 15) && l == (kk & 63)) {
-#pragma unroll
-                for (int a = 0; a < NA; ++a)
-                    if (a == (kk >> 4)) sm.d[kk] = A[sl(a, a >> 2)];
-            }
+            put_elem(A, N - 2, N - 2, w, l, &sm.d[N - 2]);
+            put_elem(A, N - 1, N - 2, w, l, &sm.e[N - 2]);
+            put_elem(A, N - 1, N - 1, w, l, &sm.d[N - 1]);
             if (vout) {
                 for (int i = tid; i < N; i += NT) {
                     vout[(size_t)(N - 2) * N + i] = 0.0;
@@ -238,6 +251,8 @@ struct wrow_solver {
                 }
                 if (tid == 0) { tauout[N - 2] = 0.0; tauout[N - 1] = 0.0; }
             }
+        } else if (N == 1) {
+            put_elem(A, 0, 0, w, l, &sm.d[0]);
         }
         __syncthreads();
     }

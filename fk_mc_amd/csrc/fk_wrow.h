@@ -70,10 +70,10 @@ struct wrow_solver {
         double scratch[3 * NWV + 8];
     };
     struct smem {
-        double x[2][NP];
+        double x[2][NP];       // x[(k + 1) & 1]: column k + 1 of A before step k's update
         double rs[NP];
-        double p[NP];
-        double v[NP];          // Householder vector of the current step
+        double p[2][NP];       // p and v of step k in [k & 1] (read again by step k + 1)
+        double v[2][NP];
         double d[NP], e[NP];
         double red[NWV + 8];
         union {
@@ -100,24 +100,47 @@ struct wrow_solver {
 
     // Householder step k with the BM = k / 64 finished column blocks known at compile time:
     // slots of those blocks take no part, and every other skip is per 64-row group.
-    template <int BM>
+    // Householder step k with BM = k / 64 finished column blocks known at compile time
+    // (slots of those blocks take no part) and CROSS when column k + 1 lies in block BM + 1.
+    //
+    // Look-ahead: column k of A after step k - 1 is not handed over after that step's update
+    // (which would need a barrier between the update and this step's reflector); every wave
+    // forms it itself from column k before step k - 1's update (stored during step k - 1,
+    // before its updates) and step k - 1's v, p and coef, as the update forms the registers.
+    // So a wave that has finished step k - 1's update goes on into step k while others still
+    // update, and a step has two barriers.  tprev / cprev carry step k - 1's tau and coef.
+    template <int BM, bool CROSS>
     __device__ __forceinline__ static void step(double (&A)[NS], smem& sm, int N, int k, int w, int l, int tid,
-                                                double* __restrict__ vout, double* __restrict__ tauout) {
-        double* xk = sm.x[k & 1];
-        // column k of A (its block is BM) from lane k % 64 of every wave, rows above k included
-        // (never read); the previous step's readers of this buffer passed two barriers since
-        if (l == (k & 63)) {
-#pragma unroll
-            for (int a = 4 * BM; a < NA; ++a) xk[w + 16 * a] = A[sl(a, BM)];
+                                                double* __restrict__ vout, double* __restrict__ tauout,
+                                                double& tprev, double& cprev) {
+        static_assert(!CROSS || BM < 3, "column k + 1 lies in block BM or BM + 1 <= 3");
+        constexpr int CB = CROSS ? BM + 1 : BM;
+        const int pk = k & 1, pq = pk ^ 1;
+        const double* xe = sm.x[pk];
+        const double* vq = sm.v[pq];
+        const double* prq = sm.p[pq];
+        const bool upd = tprev != 0.0;  // wave-uniform
+        const double cq = cprev;
+        double vk = 0.0, wk = 0.0;
+        if (upd) {
+            vk = vq[k];
+            wk = fma(-cq, vk, prq[k]);
         }
-        __syncthreads();
+        // A[i][k] after step k - 1
+        auto col = [&](int i) {
+            const double xi = xe[i];
+            if (!upd) return xi;
+            const double vi = vq[i];
+            const double wi = fma(-cq, vi, prq[i]);
+            return fma(-vi, wk, fma(-wi, vk, xi));
+        };
         // reflector from x = A[k+1:N, k] (every wave redundantly), the lane's entries l + 64b
-        const double alpha = xk[k + 1];
+        const double alpha = col(k + 1);
         double xb[NBC], part = 0.0;
 #pragma unroll
         for (int b = BM; b < NBC; ++b) {
             const int j = l + 64 * b;
-            xb[b] = xk[j];
+            xb[b] = col(j);
             part += (j >= k + 2 && j < N) ? xb[b] * xb[b] : 0.0;
         }
         const double xn2 = wave_sum(part);
@@ -129,23 +152,33 @@ struct wrow_solver {
         }
         if (tid == 0) {
             sm.e[k] = beta;
-            sm.d[k] = xk[k];  // A[k][k] is final after step k - 1
-        }
-        if (vout) {
-            for (int i = tid; i < N; i += NT)
-                vout[(size_t)k * N + i] = (i == k + 1) ? 1.0 : ((i > k + 1) ? xk[i] * scal : 0.0);
-            if (tid == 0) tauout[k] = tau;
+            sm.d[k] = col(k);  // A[k][k] is final after step k - 1
         }
         tau = uniform(tau);
         scal = uniform(scal);
         // v at the lane's columns; every wave stores the same values to sm.v, from where the
         // wave reads v at its rows (its own stores are ordered before its loads)
+        double* vs = sm.v[pk];
         double vc[NBC];
 #pragma unroll
         for (int b = 0; b < NBC; ++b) {
             const int j = l + 64 * b;
             vc[b] = b < BM ? 0.0 : ((j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? xb[b] * scal : 0.0));
-            if (b >= BM) sm.v[j] = vc[b];
+            if (b >= BM) vs[j] = vc[b];
+        }
+        if (vout) {
+            if (w == 0) {
+#pragma unroll
+                for (int b = 0; b < NBC; ++b)
+                    if (l + 64 * b < N) vout[(size_t)k * N + l + 64 * b] = vc[b];
+            }
+            if (tid == 0) tauout[k] = tau;
+        }
+        // column k + 1 before this step's update, for the next step
+        if (l == ((k + 1) & 63)) {
+            double* xn = sm.x[pq];
+#pragma unroll
+            for (int a = 4 * CB; a < NA; ++a) xn[w + 16 * a] = A[sl(a, CB)];
         }
         if (tau != 0.0) {
             // row sums over the lanes' columns (four groups of 4 rows); rows of finished
@@ -174,7 +207,7 @@ struct wrow_solver {
             for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
 #pragma unroll
             for (int a = 4 * (BM + 1); a < NA; ++a) {
-                const double vr = sm.v[w + 16 * a];
+                const double vr = vs[w + 16 * a];
 #pragma unroll
                 for (int b = BM; b < (a >> 2); ++b) cs[b] = fma(A[sl(a, b)], vr, cs[b]);
             }
@@ -182,6 +215,7 @@ struct wrow_solver {
             for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];
         }
         __syncthreads();
+        double* ps = sm.p[pk];
         if (tau != 0.0) {
             // p_i = tau (row sum + 16 column partials): four lanes per row; entries of finished
             // rows and columns hold older partials and are masked
@@ -191,35 +225,38 @@ struct wrow_solver {
             for (int ww = 0; ww < 4; ++ww) s += sm.u.cpart[4 * q4 + ww][i];
             s = quad_sum(s);
             const double pi = (i > k && i < N) ? tau * s : 0.0;
-            if (q4 == 0) sm.p[i] = pi;
-            const double vi = (i > k) ? sm.v[i] : 0.0;
+            if (q4 == 0) ps[i] = pi;
+            const double vi = (i > k) ? vs[i] : 0.0;
             const double pv = wave_sum(q4 == 0 ? pi * vi : 0.0);
             if (l == 0) sm.red[tid / WAVE] = pv;
         }
         __syncthreads();
+        double coef = 0.0;
         if (tau != 0.0) {
             double pvt = 0.0;  // fixed order: bitwise reproducible
 #pragma unroll
             for (int q = 0; q < NWV; ++q) pvt += sm.red[q];
-            const double coef = uniform(0.5 * tau * pvt);
+            coef = uniform(0.5 * tau * pvt);
             double wc[NBC];
 #pragma unroll
-            for (int b = BM; b < NBC; ++b) wc[b] = fma(-coef, vc[b], sm.p[l + 64 * b]);
+            for (int b = BM; b < NBC; ++b) wc[b] = fma(-coef, vc[b], ps[l + 64 * b]);
 #pragma unroll
             for (int a = 4 * BM; a < NA; ++a) {
                 // row values by broadcast LDS reads, w_i formed as the lanes form w_j; a
                 // finished row (v_i = p_i = 0) is left as it is
                 const int i = w + 16 * a;
-                const double vra = sm.v[i];
-                const double wra = fma(-coef, vra, sm.p[i]);
+                const double vra = vs[i];
+                const double wra = fma(-coef, vra, ps[i]);
 #pragma unroll
                 for (int b = BM; b <= (a >> 2); ++b)
                     A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
             }
         }
-        // no barrier here: the next step's first LDS writes (column k + 1 into the other x
-        // buffer) conflict with no read of this step after the barrier above, and its barrier
-        // follows them
+        tprev = tau;
+        cprev = coef;
+        // no barrier here: this step's buffers that the next step writes before its first
+        // barrier (x[k & 1], v[(k + 1) & 1], rs, cpart) were last read before this step's
+        // second barrier, and the next step writes p, red only after its first barrier
     }
 
     __device__ __forceinline__ static void tridiagonalize(double (&A)[NS], smem& sm, int N, double* __restrict__ vout,
@@ -228,16 +265,26 @@ struct wrow_solver {
         const int l = tid & (WAVE - 1);
         const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
         for (int i = tid; i < NP; i += NT) {
-            sm.x[0][i] = 0.0; sm.x[1][i] = 0.0; sm.d[i] = 0.0; sm.e[i] = 0.0;
-            sm.v[i] = 0.0; sm.p[i] = 0.0; sm.rs[i] = 0.0;
+            sm.x[0][i] = 0.0; sm.x[1][i] = 0.0; sm.d[i] = 0.0; sm.e[i] = 0.0; sm.rs[i] = 0.0;
+            sm.v[0][i] = 0.0; sm.v[1][i] = 0.0; sm.p[0][i] = 0.0; sm.p[1][i] = 0.0;
         }
         for (int i = tid; i < NWV * NP; i += NT) (&sm.u.cpart[0][0])[i] = 0.0;
         __syncthreads();
+        // column 0 for step 0
+        if (l == 0) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) sm.x[0][w + 16 * a] = A[sl(a, 0)];
+        }
+        __syncthreads();
+        double tprev = 0.0, cprev = 0.0;
         int k = 0;
-        for (; k + 2 < N && k < 64; ++k) step<0>(A, sm, N, k, w, l, tid, vout, tauout);
-        for (; k + 2 < N && k < 128; ++k) step<1>(A, sm, N, k, w, l, tid, vout, tauout);
-        for (; k + 2 < N && k < 192; ++k) step<2>(A, sm, N, k, w, l, tid, vout, tauout);
-        for (; k + 2 < N; ++k) step<3>(A, sm, N, k, w, l, tid, vout, tauout);
+        for (; k + 2 < N && k < 63; ++k) step<0, false>(A, sm, N, k, w, l, tid, vout, tauout, tprev, cprev);
+        if (k + 2 < N && k == 63) step<0, true>(A, sm, N, k++, w, l, tid, vout, tauout, tprev, cprev);
+        for (; k + 2 < N && k < 127; ++k) step<1, false>(A, sm, N, k, w, l, tid, vout, tauout, tprev, cprev);
+        if (k + 2 < N && k == 127) step<1, true>(A, sm, N, k++, w, l, tid, vout, tauout, tprev, cprev);
+        for (; k + 2 < N && k < 191; ++k) step<2, false>(A, sm, N, k, w, l, tid, vout, tauout, tprev, cprev);
+        if (k + 2 < N && k == 191) step<2, true>(A, sm, N, k++, w, l, tid, vout, tauout, tprev, cprev);
+        for (; k + 2 < N; ++k) step<3, false>(A, sm, N, k, w, l, tid, vout, tauout, tprev, cprev);
         __syncthreads();
         // the last two diagonal entries and the last off-diagonal one, from their owners
         if (N >= 2) {

@@ -274,6 +274,7 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     __shared__ uint32_t mt[fkrng::MT_WORDS];
     __shared__ move_prop prop;
     __shared__ int acc_s, chain_s, step_s;
+    __shared__ __align__(16) double bmu_s[2];  // beta, mu_f for the decision after the solve (no register across it)
     __shared__ sweep_args sa;
     const int tid = threadIdx.x;
     const int B = a.B;
@@ -315,11 +316,13 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
     for (int i = tid; i < fkrng::MT_WORDS; i += S::NT) mt[i] = mtg[i];
     for (int i = tid; i < S::NMAX; i += S::NT) f_lds[i] = (i < N) ? a.f[(size_t)ch * cl.ld + i] : 0;
     __syncthreads();
-    const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch], mu_f = a.mu_f[ch];
+    const double beta = a.beta[ch], U = a.U[ch], mu_c = a.mu_c[ch];
     if (tid == 0) {
         sa.step = a.step + s;
         cl.apply(sa);
         prop = propose_move(mt, f_lds, N, a.nf[ch], sa);
+        bmu_s[0] = beta;
+        bmu_s[1] = a.mu_f[ch];
     }
     __syncthreads();
     double lz_new = 0.0;
@@ -328,9 +331,10 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
         double* ws = a.ws ? a.ws + (size_t)ch * a.ws_stride : nullptr;
         lz_new = S::solve(sm, cl.hop, f_lds, N, beta, U, mu_c, ws, nullptr, nullptr, nullptr, nullptr);
     }
+    __syncthreads();
     if (tid == 0) {
         double logw;
-        const int acc = decide_move(prop, prop.sa >= 0 ? f_lds[prop.sa] : 0, lz_new, a.logz[ch], beta, mu_f,
+        const int acc = decide_move(prop, prop.sa >= 0 ? f_lds[prop.sa] : 0, lz_new, a.logz[ch], bmu_s[0], bmu_s[1],
                                     a.nf[ch], &logw);
         record_move(sa, ch, prop, acc, logw, lz_new);
         acc_s = acc;

@@ -81,7 +81,7 @@ static void meas_free(chain_meas_bufs* m) {
                   m->wgrid, m->fermi, m->JV, m->mJ, m->tdiag, m->stiff, m->cond, m->stiff_hist, m->cond_hist,
                   m->stiff_sum, m->cond_sum, m->evec_hist};
     for (void* p : ps)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     delete m;
 }
 
@@ -153,11 +153,11 @@ struct fkgpu_chains {
 // Captured graphs depend on the move set, the f-f terms and the update mode: drop them
 // whenever one of these changes.
 static void drop_graphs(fkgpu_chains* ch) {
-    if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
-    if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+    if (ch->gexec) { (void)hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+    if (ch->graph) { (void)hipGraphDestroy(ch->graph); ch->graph = nullptr; }
     ch->graph_steps = 0;
-    if (ch->mexec) { hipGraphExecDestroy(ch->mexec); ch->mexec = nullptr; }
-    if (ch->mgraph) { hipGraphDestroy(ch->mgraph); ch->mgraph = nullptr; }
+    if (ch->mexec) { (void)hipGraphExecDestroy(ch->mexec); ch->mexec = nullptr; }
+    if (ch->mgraph) { (void)hipGraphDestroy(ch->mgraph); ch->mgraph = nullptr; }
     ch->mgraph_len = 0;
 }
 
@@ -181,7 +181,7 @@ int ensure(fkgpu_ctx* ctx, fkgpu_ctx::buf& b, size_t count, T** out) {
     size_t bytes = count * sizeof(T);
     if (bytes == 0) bytes = sizeof(T);
     if (b.n < bytes) {
-        if (b.p) hipFree(b.p);
+        if (b.p) (void)hipFree(b.p);
         b.p = nullptr;
         b.n = 0;
         hipError_t e = hipMalloc(&b.p, bytes);
@@ -261,10 +261,10 @@ void build_hypercubic(fkgpu_ctx* ctx, int kind, int D, int L, double t, double t
 static const double* hopq_ptr(const fkgpu_ctx* ctx) { return ctx->hopq.empty() ? nullptr : ctx->hopq.data(); }
 
 int upload_lattice(fkgpu_ctx* ctx) {
-    if (ctx->d_hop) hipFree(ctx->d_hop);
-    if (ctx->d_hopv) hipFree(ctx->d_hopv);
-    if (ctx->d_nbr) hipFree(ctx->d_nbr);
-    if (ctx->d_z) hipFree(ctx->d_z);
+    if (ctx->d_hop) (void)hipFree(ctx->d_hop);
+    if (ctx->d_hopv) (void)hipFree(ctx->d_hopv);
+    if (ctx->d_nbr) (void)hipFree(ctx->d_nbr);
+    if (ctx->d_z) (void)hipFree(ctx->d_z);
     ctx->d_hop = ctx->d_hopv = nullptr;
     ctx->d_nbr = ctx->d_z = nullptr;
     size_t n = ctx->N;
@@ -335,19 +335,19 @@ static void mixed_free(mixed_bufs* m);
 
 static void ctx_release(fkgpu_ctx* ctx) {
     if (!ctx || --ctx->refs > 0) return;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     mixed_free(ctx->mx);
     for (auto* b : {&ctx->b_f, &ctx->b_beta, &ctx->b_U, &ctx->b_mu, &ctx->b_ev, &ctx->b_lz, &ctx->b_ce, &ctx->b_cf,
                     &ctx->b_ws, &ctx->b_vh, &ctx->b_tau, &ctx->b_d, &ctx->b_e, &ctx->b_Z, &ctx->b_scr, &ctx->b_m1,
                     &ctx->b_m2, &ctx->b_m3, &ctx->b_m4, &ctx->b_m5, &ctx->b_out, &ctx->b_w, &ctx->b_c1, &ctx->b_c2,
                     &ctx->b_c3, &ctx->b_c4})
-        if (b->p) hipFree(b->p);
-    if (ctx->d_hop) hipFree(ctx->d_hop);
-    if (ctx->d_hopv) hipFree(ctx->d_hopv);
-    if (ctx->d_nbr) hipFree(ctx->d_nbr);
-    if (ctx->d_z) hipFree(ctx->d_z);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (b->p) (void)hipFree(b->p);
+    if (ctx->d_hop) (void)hipFree(ctx->d_hop);
+    if (ctx->d_hopv) (void)hipFree(ctx->d_hopv);
+    if (ctx->d_nbr) (void)hipFree(ctx->d_nbr);
+    if (ctx->d_z) (void)hipFree(ctx->d_z);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -370,7 +370,7 @@ int fkgpu_set_lattice(fkgpu_ctx* ctx, int kind, int ndim, int L, double t, doubl
     long n = 1;
     for (int i = 0; i < ndim; ++i) n *= L;
     if (n > 16384) return fail(ctx, FKGPU_EUNSUPPORTED, "lattice volume above 16384 sites");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     build_hypercubic(ctx, kind, ndim, L, t, tp);
     ctx->lat_kind = kind;
     ctx->lat_ndim = ndim;
@@ -413,7 +413,7 @@ int fkgpu_set_lattice_csr(fkgpu_ctx* ctx, int n, const int32_t* row_ptr, const i
     ctx->lat_kind = 2;
     ctx->lat_ndim = 0;
     ctx->lat_L = 0;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     return upload_lattice(ctx);
 }
 
@@ -508,7 +508,7 @@ static void mixed_free(mixed_bufs* m) {
     void* ptrs[] = {m->ws, m->vh, m->tau, m->d, m->e, m->evh, m->lzh, m->X, m->scr, m->Q, m->ev, m->lz, m->ws64,
                     m->fb, m->nflag};
     for (void* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     delete m;
 }
 
@@ -600,7 +600,7 @@ int fkgpu_eig_logz_mixed(fkgpu_ctx* ctx, int batch, const int32_t* f, const doub
     if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
     const int N = ctx->N;
     if (N > fk_solver_nmax(FK_SOLVER_MIXED32)) return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     const size_t BN = (size_t)batch * N;
     int32_t* df;
     double *dbeta, *dU, *dmu;
@@ -651,7 +651,7 @@ int fkgpu_autotune(fkgpu_ctx* ctx, int batch, double* ms) {
     if (!ctx) return FKGPU_EINVAL;
     if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
     if (batch <= 0) return fail(ctx, FKGPU_EINVAL, "empty batch");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     const int N = ctx->N;
     const size_t BN = (size_t)batch * N;
     // synthetic half-filled configurations (a fixed LCG), beta = 1, U = 2, mu_c = 1
@@ -703,16 +703,16 @@ int fkgpu_autotune(fkgpu_ctx* ctx, int batch, double* ms) {
         float t = 0.f;
         if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
         if (he != hipSuccess) {
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
             return hipfail(ctx, he, "autotune launch");
         }
         const double tv = t / 3.0;
         if (ms) ms[v] = tv;
         if (best < 0 || tv < best_ms) { best = v; best_ms = tv; }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (r) return r;
     if (best < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "no solver variant fits this lattice size");
     ctx->solver = best;
@@ -744,7 +744,7 @@ int fkgpu_eig_logz_device(fkgpu_ctx* ctx, int batch, const int32_t* d_f, const d
         int r = ensure(ctx, ctx->b_ws, (size_t)grid * a.ws_stride, &a.ws);
         if (r) return r;
     }
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     FK_HIP(ctx, fk_launch_eig(kind, a, grid, ctx->stream));
     return FKGPU_OK;
 }
@@ -783,7 +783,7 @@ int fkgpu_eig_logz_device_noalloc(fkgpu_ctx* ctx, int batch, const int32_t* d_f,
         }
         a.ws = ctx->noalloc_scratch;
     }
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     FK_HIP(ctx, fk_launch_eig(kind, a, batch, ctx->stream));
     return FKGPU_OK;
 }
@@ -795,7 +795,7 @@ int fkgpu_eig_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     if (r) return r;
     if (!evals || !logZ) return fail(ctx, FKGPU_EINVAL, "null output");
     if (fk_solver_kind(ctx->N) < 0) return fail(ctx, FKGPU_EUNSUPPORTED, "dense eigensolver implemented for N <= 2048");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     if (want_vectors) {
         if (!evecs) return fail(ctx, FKGPU_EINVAL, "null evecs with want_vectors");
         eig_dev o;
@@ -861,7 +861,7 @@ int fkgpu_measure_ipr(fkgpu_ctx* ctx, int batch, const int32_t* f, const double*
     int r = check_params(ctx, batch, f, beta, U, mu_c);
     if (r) return r;
     if (!ipr) return fail(ctx, FKGPU_EINVAL, "null output");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     eig_dev o;
     if ((r = eig_pairs_dev(ctx, batch, f, beta, U, mu_c, true, o))) return r;
     const int N = ctx->N;
@@ -885,7 +885,7 @@ int fkgpu_measure_stiffness(fkgpu_ctx* ctx, int batch, const int32_t* f, const d
         return fail(ctx, FKGPU_EINVAL, "invalid stiffness arguments");
     if (ctx->lat_kind != FKGPU_LATTICE_HYPERCUBIC || ctx->lat_ndim < 2)
         return fail(ctx, FKGPU_EUNSUPPORTED, "stiffness needs a hypercubic lattice of dimension >= 2");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     eig_dev o;
     if ((r = eig_pairs_dev(ctx, batch, f, beta, U, mu_c, true, o))) return r;
     const int N = ctx->N;
@@ -915,7 +915,7 @@ int fkgpu_dos(fkgpu_ctx* ctx, int nspectra, const double* spectra, int nw, const
     if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
     if (nspectra <= 0 || nw <= 0 || !spectra || !wgrid || !dos || !(eta > 0.0))
         return fail(ctx, FKGPU_EINVAL, "invalid DOS arguments");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     const int N = ctx->N;
     double *sp, *wg, *out;
     int r;
@@ -937,7 +937,7 @@ int fkgpu_ipr_w(fkgpu_ctx* ctx, int nspectra, const double* spectra, const doubl
     if (ctx->N == 0) return fail(ctx, FKGPU_ENOLATTICE, "no lattice set");
     if (nspectra <= 0 || nw <= 0 || !spectra || !ipr || !wgrid || !out || !(eta > 0.0))
         return fail(ctx, FKGPU_EINVAL, "invalid IPR(w) arguments");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     const int N = ctx->N;
     const size_t SN = (size_t)nspectra * N;
     double *sp, *wg, *o, *ip;
@@ -960,7 +960,7 @@ int fkgpu_adjoint(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* bet
     int r = check_params(ctx, batch, f, beta, U, mu_c);
     if (r) return r;
     if (!logZ || !dens || !grads) return fail(ctx, FKGPU_EINVAL, "null output");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     eig_dev o;
     if ((r = eig_pairs_dev(ctx, batch, f, beta, U, mu_c, true, o))) return r;
     const int N = ctx->N;
@@ -1145,7 +1145,7 @@ int fkgpu_cheb_logz(fkgpu_ctx* ctx, const int32_t* f, double beta, double U, dou
     if (N > 16384) return fail(ctx, FKGPU_EUNSUPPORTED, "exact-trace Chebyshev keeps two N x N matrices: N <= 16384");
     cheb_quad cq(cheb_size, grid_size);
     const int M = cq.M;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     const size_t NN = (size_t)N * N;
     double *T0, *T1, *parts, *traces, *diag;
     if ((r = ensure(ctx, ctx->b_c1, NN, &T0)) || (r = ensure(ctx, ctx->b_c2, NN, &T1)) ||
@@ -1206,7 +1206,7 @@ int fkgpu_kpm_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
     const int N = ctx->N;
     if (!logZ || R < 1 || M < 2 || (M & 1) || M > 512) return fail(ctx, FKGPU_EINVAL, "invalid KPM arguments");
     if (N > 4096 || ctx->zmax > 8) return fail(ctx, FKGPU_EUNSUPPORTED, "KPM path needs N <= 4096 and <= 8 neighbours");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     // Gershgorin bounds and Chebyshev-Gauss coefficients of log(1+e^{-beta(a x + b)})
     std::vector<double> abv(2 * (size_t)batch), coef((size_t)batch * M), abo(2 * (size_t)batch);
     const int K = std::max(2 * M, 64);
@@ -1266,8 +1266,8 @@ int fkgpu_kpm_logz(fkgpu_ctx* ctx, int batch, const int32_t* f, const double* be
 void fkgpu_chains_destroy(fkgpu_chains* ch) {
     if (!ch) return;
     if (ch->ctx) {
-        hipSetDevice(ch->ctx->device);
-        hipStreamSynchronize(ch->ctx->stream);
+        (void)hipSetDevice(ch->ctx->device);
+        (void)hipStreamSynchronize(ch->ctx->stream);
     }
     drop_graphs(ch);
     meas_free(ch->ms);
@@ -1276,14 +1276,14 @@ void fkgpu_chains_destroy(fkgpu_chains* ch) {
                     ch->t_logz, ch->t_acc, ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef,
                     ch->k_mom, ch->k_lz, ch->k_lmom, ch->q_ctr, ch->q_ring, ch->q_err, ch->lat};
     for (void* p : ptrs)
-        if (p) hipFree(p);
-    if (ch->ev0) hipEventDestroy(ch->ev0);
-    if (ch->ev1) hipEventDestroy(ch->ev1);
+        if (p) (void)hipFree(p);
+    if (ch->ev0) (void)hipEventDestroy(ch->ev0);
+    if (ch->ev1) (void)hipEventDestroy(ch->ev1);
     mixed_free(ch->mx);
     void* fptrs[] = {ch->fu_V, ch->fu_Q1, ch->fu_Q2, ch->fu_T, ch->fu_vh, ch->fu_tau, ch->fu_d, ch->fu_e, ch->fu_scr,
                      ch->fu_upd, ch->fu_f32, ch->fu_skip, ch->fu_count};
     for (void* p : fptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     fkgpu_ctx* ctx = ch->ctx;
     std::vector<fkgpu_ctx*> others = ch->mctx;
     delete ch;
@@ -1301,7 +1301,7 @@ static int chains_create_impl(fkgpu_ctx* ctx, int N, int kind, int nchains, cons
                               const double* beta, const double* U, const double* mu_c, const double* mu_f,
                               const uint32_t* seeds, bool initial_eig, fkgpu_chains** out) {
     int r = 0;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     const int B = nchains;
     fkgpu_chains* ch = new fkgpu_chains();
     ch->ctx = ctx;
@@ -1741,7 +1741,7 @@ int fkgpu_chains_randomize_f(fkgpu_chains* ch, const int32_t* nf_start) {
             return fail(ctx, FKGPU_EINVAL, "nf_start must lie in [0, msize]");
     if (ch->fu || ch->mixed || ch->kpm || !ch->dense_ok)
         return fail(ctx, FKGPU_EUNSUPPORTED, "randomize_f: call it on a new chain set (FP64 dense mode)");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     int32_t *dnf, *df;
     int r;
@@ -1781,12 +1781,12 @@ int fkgpu_chains_set_steps_per_launch(fkgpu_chains* ch, int steps) {
     if (!ch) return FKGPU_EINVAL;
     if (steps < 1 || (long long)steps * ch->B > 0x7fffffffLL)
         return fail(ch->ctx, FKGPU_EINVAL, "steps per launch must be >= 1 (and steps x chains < 2^31)");
-    hipSetDevice(ch->ctx->device);
+    FK_HIP(ch->ctx, hipSetDevice(ch->ctx->device));
     FK_HIP(ch->ctx, hipStreamSynchronize(ch->ctx->stream));
     if (steps > ch->spl) {
         int32_t* ring = nullptr;
         if (dalloc(&ring, (size_t)steps * ch->B) != hipSuccess) return fail(ch->ctx, FKGPU_ENOMEM, "chain queue");
-        hipFree(ch->q_ring);
+        (void)hipFree(ch->q_ring);
         ch->q_ring = ring;
     }
     ch->spl = steps;
@@ -1801,7 +1801,7 @@ int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
     const int B = ch->B, N = ch->N;
     if (mode != FKGPU_UPDATE_DENSE && mode != FKGPU_UPDATE_RANK1) return fail(ctx, FKGPU_EINVAL, "unknown update mode");
     if (refresh_steps < 0) return fail(ctx, FKGPU_EINVAL, "negative refresh interval");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     FK_HIP(ctx, hipStreamSynchronize(st));
     drop_graphs(ch);
@@ -1839,7 +1839,7 @@ int fkgpu_chains_set_update(fkgpu_chains* ch, int mode, int refresh_steps) {
                              (void**)&ch->fu_scr, (void**)&ch->fu_upd, (void**)&ch->fu_f32, (void**)&ch->fu_skip,
                              (void**)&ch->fu_count};
             for (void** q : bufs) {
-                if (*q) hipFree(*q);
+                if (*q) (void)hipFree(*q);
                 *q = nullptr;
             }
             return fail(ctx, FKGPU_ENOMEM, "rank-one update buffers");
@@ -1865,7 +1865,7 @@ int fkgpu_chains_set_precision(fkgpu_chains* ch, int precision) {
         return fail(ctx, FKGPU_EUNSUPPORTED, "rank-one update mode is FP64; switch it off first");
     if (precision == FKGPU_PRECISION_MIXED && N > fk_solver_nmax(FK_SOLVER_MIXED32))
         return fail(ctx, FKGPU_EUNSUPPORTED, "mixed-precision path implemented for N <= 1024");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     FK_HIP(ctx, hipStreamSynchronize(st));
     drop_graphs(ch);
@@ -1911,7 +1911,7 @@ int fkgpu_chains_fallbacks(const fkgpu_chains* ch, int64_t* n) {
     if (!ch || !n) return FKGPU_EINVAL;
     unsigned long long v = 0;
     if (ch->mx) {
-        hipSetDevice(ch->ctx->device);
+        FK_HIP(ch->ctx, hipSetDevice(ch->ctx->device));
         if (hipStreamSynchronize(ch->ctx->stream) != hipSuccess ||
             hipMemcpy(&v, ch->mx->nflag, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
             return FKGPU_EHIP;
@@ -1927,7 +1927,7 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
     const int B = ch->B, N = ch->N;
     if (R < 1 || M < 2 || (M & 1) || M > 512) return fail(ctx, FKGPU_EINVAL, "invalid KPM arguments");
     if (N > 4096 || ctx->zmax > 8) return fail(ctx, FKGPU_EUNSUPPORTED, "KPM path needs N <= 4096 and <= 8 neighbours");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     std::vector<double> beta(B), U(B), mu(B);
     FK_HIP(ctx, hipMemcpy(beta.data(), ch->beta, sizeof(double) * B, hipMemcpyDeviceToHost));
@@ -1962,7 +1962,7 @@ int fkgpu_chains_set_kpm(fkgpu_chains* ch, int R, int M, uint64_t seed, int jack
     }
     void* old[] = {ch->k_props, ch->k_fprop, ch->k_skip, ch->k_ab, ch->k_coef, ch->k_mom, ch->k_lz};
     for (void* p : old)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     ch->k_props = nullptr; ch->k_fprop = ch->k_skip = nullptr;
     ch->k_ab = ch->k_coef = ch->k_mom = ch->k_lz = nullptr;
     hipError_t e = hipSuccess;
@@ -2022,7 +2022,7 @@ int fkgpu_chains_set_kpm_delta(fkgpu_chains* ch, int mode, int moments, int freq
     if (moments < 0 || moments == 1 || moments > FK_KPM_LOCAL_MMAX || frequencies < 0 || frequencies == 1)
         return fail(ctx, FKGPU_EINVAL, "moments must be 2..2048 and frequencies >= 2 (0: automatic)");
     const int B = ch->B;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<double> beta(B), ab(2 * (size_t)B);
     FK_HIP(ctx, hipMemcpy(beta.data(), ch->beta, sizeof(double) * B, hipMemcpyDeviceToHost));
     FK_HIP(ctx, hipMemcpy(ab.data(), ch->k_ab, sizeof(double) * 2 * B, hipMemcpyDeviceToHost));
@@ -2036,7 +2036,7 @@ int fkgpu_chains_set_kpm_delta(fkgpu_chains* ch, int mode, int moments, int freq
         moments = std::max(64, std::min(FK_KPM_LOCAL_MMAX, 32 * (int)std::ceil(need / 32.0)));
     }
     if (frequencies == 0) frequencies = 512;
-    if (ch->k_lmom) hipFree(ch->k_lmom);
+    if (ch->k_lmom) (void)hipFree(ch->k_lmom);
     ch->k_lmom = nullptr;
     if (dalloc(&ch->k_lmom, (size_t)B * 3 * moments) != hipSuccess) return fail(ctx, FKGPU_ENOMEM, "local moment buffers");
     ch->kl_M = moments;
@@ -2072,7 +2072,7 @@ int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
     fkgpu_ctx* ctx = ch->ctx;
     if (nsteps < 0) return fail(ctx, FKGPU_EINVAL, "negative step count");
     if (int r = kpm_moves_ok(ch)) return r;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     sweep_args a = ch->args;
     const size_t need = (size_t)nsteps * ch->B;
@@ -2080,7 +2080,7 @@ int fkgpu_sweep(fkgpu_chains* ch, int nsteps, const fkgpu_trace* trace) {
         if (ch->trace_cap < need) {
             void* ps[] = {ch->t_kind, ch->t_a, ch->t_b, ch->t_u, ch->t_logw, ch->t_logz, ch->t_acc};
             for (void* p : ps)
-                if (p) hipFree(p);
+                if (p) (void)hipFree(p);
             ch->t_kind = ch->t_a = ch->t_b = nullptr;
             ch->t_u = ch->t_logw = ch->t_logz = nullptr;
             ch->t_acc = nullptr;
@@ -2136,11 +2136,11 @@ int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
     fkgpu_ctx* ctx = ch->ctx;
     if (nsteps <= 0) return fail(ctx, FKGPU_EINVAL, "step count must be positive");
     if (int r = kpm_moves_ok(ch)) return r;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     if (!ch->gexec || ch->graph_steps != nsteps) {
-        if (ch->gexec) { hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
-        if (ch->graph) { hipGraphDestroy(ch->graph); ch->graph = nullptr; }
+        if (ch->gexec) { (void)hipGraphExecDestroy(ch->gexec); ch->gexec = nullptr; }
+        if (ch->graph) { (void)hipGraphDestroy(ch->graph); ch->graph = nullptr; }
         sweep_args a = ch->args;
         a.trace = 0;
         if (!ch->kpm && !ch->dense_ok)
@@ -2149,8 +2149,8 @@ int fkgpu_sweep_graph(fkgpu_chains* ch, int nsteps) {
         hipError_t e = launch_sweep_chunks(ch, a, nsteps, st);
         if (e != hipSuccess) {
             hipGraph_t g = nullptr;
-            hipStreamEndCapture(st, &g);
-            if (g) hipGraphDestroy(g);
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
             return hipfail(ctx, e, "graph capture");
         }
         FK_HIP(ctx, hipStreamEndCapture(st, &ch->graph));
@@ -2177,7 +2177,7 @@ int fkgpu_chains_state(fkgpu_chains* ch, int32_t* f, int32_t* nf, double* logZ, 
                        int64_t* naccept, int64_t* nsolve) {
     if (!ch) return FKGPU_EINVAL;
     fkgpu_ctx* ctx = ch->ctx;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     FK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const size_t BN = (size_t)ch->B * ch->N;
     if (f) {
@@ -2317,7 +2317,7 @@ static hipError_t meas_launch(fkgpu_chains* ch, hipStream_t st) {
 static int meas_count(fkgpu_chains* ch, long long* count) {
     fkgpu_ctx* ctx = ch->ctx;
     if (!ch->ms) return fail(ctx, FKGPU_EINVAL, "measurements are not enabled on this chain set");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     FK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     FK_HIP(ctx, hipMemcpy(count, ch->ms->count, sizeof(long long), hipMemcpyDeviceToHost));
     return FKGPU_OK;
@@ -2336,7 +2336,7 @@ int fkgpu_chains_measure_enable(fkgpu_chains* ch, int capacity, int flags) {
         return fail(ctx, FKGPU_EUNSUPPORTED, "IPR, stiffness and eigenfunctions need eigenvectors, implemented for N <= 1024");
     if ((flags & FKGPU_MEASURE_STIFFNESS) && (ctx->lat_kind != FKGPU_LATTICE_HYPERCUBIC || ctx->lat_ndim < 2))
         return fail(ctx, FKGPU_EUNSUPPORTED, "stiffness needs a hypercubic lattice of dimension >= 2");
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     FK_HIP(ctx, hipStreamSynchronize(st));
     drop_graphs(ch);
@@ -2502,7 +2502,7 @@ int fkgpu_chains_green(fkgpu_chains* ch, double w, double offset, double* g_re, 
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ch->ev0, ch->ev1);
-    hipFree(buf);
+    (void)hipFree(buf);
     FK_HIP(ctx, e);
     ch->last_ms = ms;
     return FKGPU_OK;
@@ -2535,7 +2535,7 @@ int fkgpu_chains_local_dos(fkgpu_chains* ch, int nw, const double* wgrid, double
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ch->ev0, ch->ev1);
-    hipFree(buf);
+    (void)hipFree(buf);
     FK_HIP(ctx, e);
     ch->last_ms = ms;
     return FKGPU_OK;
@@ -2546,7 +2546,7 @@ int fkgpu_chains_measure(fkgpu_chains* ch) {
     fkgpu_ctx* ctx = ch->ctx;
     int r = meas_prepare(ch);
     if (r) return r;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     FK_HIP(ctx, meas_launch(ch, st));
     FK_HIP(ctx, hipStreamSynchronize(st));
@@ -2563,11 +2563,11 @@ int fkgpu_sweep_measure(fkgpu_chains* ch, int nsweeps, int sweep_len) {
     if (r) return r;
     r = meas_prepare(ch);
     if (r) return r;
-    hipSetDevice(ctx->device);
+    FK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     if (!ch->mexec || ch->mgraph_len != sweep_len) {
-        if (ch->mexec) { hipGraphExecDestroy(ch->mexec); ch->mexec = nullptr; }
-        if (ch->mgraph) { hipGraphDestroy(ch->mgraph); ch->mgraph = nullptr; }
+        if (ch->mexec) { (void)hipGraphExecDestroy(ch->mexec); ch->mexec = nullptr; }
+        if (ch->mgraph) { (void)hipGraphDestroy(ch->mgraph); ch->mgraph = nullptr; }
         sweep_args a = ch->args;
         a.trace = 0;
         FK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2575,8 +2575,8 @@ int fkgpu_sweep_measure(fkgpu_chains* ch, int nsweeps, int sweep_len) {
         if (e == hipSuccess) e = meas_launch(ch, st);
         if (e != hipSuccess) {
             hipGraph_t g = nullptr;
-            hipStreamEndCapture(st, &g);
-            if (g) hipGraphDestroy(g);
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
             return hipfail(ctx, e, "graph capture");
         }
         FK_HIP(ctx, hipStreamEndCapture(st, &ch->mgraph));

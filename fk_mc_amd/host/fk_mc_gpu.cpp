@@ -132,6 +132,10 @@ params define_parameters() {
     p.define("dos_offset", "0.05", "DOS offset from real axis");
     p.define("cond_npoints", "150", "number of points to sample conductivity");
     p.define("save_eigenfunctions", "false", "Store eigenfunctions?");
+    p.define("maxtime", "720", "max evaluation time (defined, not used, as in the reference)");
+    p.define("h5lattice", "kwant_lattice.h5", "H5 for lattice (graph lattices: not built)");
+    p.define("exit", "false", "dry_run: print the parameters and exit");
+    p.define("max_time", "1209600", "Maximum running time in seconds");
     // batched program
     p.define("lattice", "cubic2d", "cubic1d | cubic2d | cubic3d | triangular");
     p.define("chains", "1", "Number of chains (chain r = MPI rank r of the reference, seeded seed + r)");
@@ -451,6 +455,18 @@ int run(params& p) {
     // with measure_ipr (calc_spectrum); without the spectrum measures only occupations are kept
     const bool spectra = !cheb || ipr || eig || p.b("measure_stiffness");
 
+    // prog/fk_mc_exec.cpp:97-117: the parameter summary, then `exit` ends a dry run before
+    // anything is computed
+    if (p.b("show_output") || p.b("exit")) {
+        std::printf("Model parameters:\n  System size %d (%s)\n  U %g\n  t %g\n  Temperature %g\n  mu_c %g\n"
+                    "  e_f %g\n  mu_f %g\n  beta %g\n", L, lat.c_str(), U, p.d("t"), 1.0 / beta, mu_c, mu_f - mu_c, mu_f,
+                    beta);
+        std::printf("Monte Carlo parameters:\n  sweeps %ld\n  moves per sweep %ld\n  warmup sweeps %ld\n  seed %ld\n"
+                    "  flip %g, add/remove %g, reshuffle %g, hop %g\n", nsweeps, sweep_len, ntherm, seed,
+                    p.d("mc_flip"), p.d("mc_add_remove"), p.d("mc_reshuffle"), p.d("mc_hop"));
+        std::fflush(stdout);
+    }
+    if (p.b("exit")) return 0;
     // opened before the run, so that an output that cannot be written ends it at once
     archive ar(p.str("output"), R);
     fkgpu_ctx* ctx = nullptr;
@@ -519,19 +535,27 @@ int run(params& p) {
                     L, N, R, seed, seed + R - 1, ntherm, nsweeps, sweep_len);
     auto t0 = std::chrono::steady_clock::now();
     if (ntherm > 0) FK_CHECK(fkgpu_sweep_graph(ch, (int)(ntherm * sweep_len)), ctx);
-    // measured sweeps in blocks, with a progress line per block
+    // measured sweeps in blocks, with a progress line per block; max_time ends them after the
+    // block in which it ran out (the reference's alps::stop_callback, prog/fk_mc_exec.cpp:151)
     const long block = std::max(1L, nsweeps / 10);
-    for (long done = 0; done < nsweeps;) {
+    const double max_time = p.d("max_time");
+    long done = 0;
+    while (done < nsweeps) {
         const long n = std::min(block, nsweeps - done);
         FK_CHECK(fkgpu_sweep_measure(ch, (int)n, (int)sweep_len), ctx);
         done += n;
         if (show) std::printf("  %ld%%\n", 100 * done / std::max(1L, nsweeps)), std::fflush(stdout);
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (done < nsweeps && el >= max_time) {
+            if (show) std::printf("  max_time (%g s) reached after %ld of %ld sweeps\n", max_time, done, nsweeps);
+            break;
+        }
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
-    // results
+    // results (of the sweeps done)
     int64_t count = 0;
-    const size_t S = (size_t)nsweeps, RN = (size_t)R * N;
+    const size_t S = (size_t)done, RN = (size_t)R * N;
     std::vector<double> e(R * S), d2(R * S), ce(R * S), em(R), d2m(R), spec(RN), focc(RN);
     FK_CHECK(fkgpu_chains_energies(ch, &count, e.data(), d2.data(), ce.data()), ctx);
     FK_CHECK(fkgpu_chains_means(ch, &count, em.data(), d2m.data(), spec.data(), focc.data()), ctx);
@@ -637,7 +661,7 @@ int run(params& p) {
         md2 /= std::max<size_t>(S, 1);
         cv[r] = beta * beta * (m2 - md2 - m * m) / N;  // prog/data_save.hxx:185-190
         eN[r] = em[r] / N;
-        accr[r] = (double)nacc[r] / (double)((ntherm + nsweeps) * sweep_len);
+        accr[r] = (double)nacc[r] / (double)((ntherm + done) * sweep_len);
     }
     std::vector<double> nfm(R, 0.0);
     for (int r = 0; r < R; ++r)
@@ -707,12 +731,12 @@ int run(params& p) {
         if (!drec.empty() && nm * (npts + 1) == drec.size()) err_table("dos", drec);
         if (!irec.empty()) err_table("ipr", irec);
     }
-    st << "seconds = " << secs << "\nmoves_per_second = " << (double)R * (ntherm + nsweeps) * sweep_len / secs
+    st << "seconds = " << secs << "\nmoves_per_second = " << (double)R * (ntherm + done) * sweep_len / secs
        << "\n";
     st.close();
     ar.close();
     if (show) std::printf("  %.3f s, %.4g moves/s, output in %s\n", secs,
-                          (double)R * (ntherm + nsweeps) * sweep_len / secs, p.str("output").c_str());
+                          (double)R * (ntherm + done) * sweep_len / secs, p.str("output").c_str());
     return 0;
 }
 

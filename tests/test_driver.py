@@ -117,6 +117,14 @@ def test_driver_parameters_without_device(tmp_path):
     r = subprocess.run([EXE, "--cheb_moves", "true", "--output", str(tmp_path / "x")], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0
+    # prog/fk_mc_exec.cpp:97-117: `exit` prints the parameter summary and ends before any
+    # device work or output; the reference's other program parameters are accepted
+    r = subprocess.run([EXE, "--exit", "true", "--L", "6", "--U", "2.5", "--nsweeps", "7", "--maxtime", "10",
+                        "--h5lattice", "x.h5", "--max_time", "100", "--output", str(tmp_path / "dry.h5")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "System size 6" in r.stdout and "U 2.5" in r.stdout and "sweeps 7" in r.stdout
+    assert not (tmp_path / "dry.h5").exists()
 
 
 @pytest.mark.gpu
@@ -329,3 +337,20 @@ def test_driver_cheb_moves(tmp_path, fk):
                "--output", str(tmp_path / "bad.h5"), "--show_output", "false"] + extra
         r = subprocess.run([str(c) for c in cmd], capture_output=True, text=True, timeout=60)
         assert r.returncode != 0 and "cheb_moves" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_max_time(tmp_path):
+    """max_time ends the measured sweeps after the block in which it ran out (the reference's
+    alps::stop_callback): with max_time 0 one block of nsweeps / 10 sweeps is measured and every
+    series has that length; the chains are the same as in a full run over those sweeps."""
+    L, R, ns = 4, 2, 40
+    common = ["--lattice", "cubic2d", "--L", L, "--beta", 2.0, "--U", 2.0, "--mu_c", 1.0, "--mu_f", 1.0, "--nsweeps", ns,
+              "--sweep_len", 4, "--ntherm_sweeps", 1, "--chains", R, "--seed", 3, "--measure_history", "true"]
+    out = _run(tmp_path, *(common + ["--max_time", 0]), output="short")
+    e = np.load(out / "mc_data" / "energies.npy")
+    fh = np.load(out / "mc_data" / "focc_history.npy")
+    assert e.shape == (R, ns // 10) and fh.shape == (R, ns // 10, L * L)
+    full = _run(tmp_path, *common, output="full")
+    np.testing.assert_array_equal(np.load(full / "mc_data" / "energies.npy")[:, :ns // 10], e)
+    np.testing.assert_array_equal(np.load(full / "mc_data" / "focc_history.npy")[:, :ns // 10], fh)

@@ -222,26 +222,30 @@ struct wrow_solver {
         }
         __syncthreads();
         double* ps = sm.p[pk];
-        if (tau != 0.0) {
-            // p_i = tau (row sum + 16 column partials): four lanes per row; entries of finished
-            // rows and columns hold older partials and are masked
-            const int i = tid >> 2, q4 = tid & 3;
-            double s = (q4 == 0) ? sm.rs[i] : 0.0;
+        constexpr int PW = NP / WAVE;  // waves forming p: one row a lane, one wave a SIMD
+        if (tau != 0.0 && w < PW) {
+            // p_i = tau (row sum + 16 column partials, pairwise in a fixed order); entries of
+            // finished rows and columns hold older partials and are masked.  The other waves go
+            // on to the barrier: this phase is a latency chain, which fewer waves do not lengthen.
+            const int i = tid;
+            double t[4];
 #pragma unroll
-            for (int ww = 0; ww < 4; ++ww) s += sm.u.cpart[4 * q4 + ww][i];
-            s = quad_sum(s);
+            for (int g = 0; g < 4; ++g)
+                t[g] = (sm.u.cpart[4 * g][i] + sm.u.cpart[4 * g + 1][i]) +
+                       (sm.u.cpart[4 * g + 2][i] + sm.u.cpart[4 * g + 3][i]);
+            const double s = sm.rs[i] + ((t[0] + t[1]) + (t[2] + t[3]));
             const double pi = (i > k && i < N) ? tau * s : 0.0;
-            if (q4 == 0) ps[i] = pi;
+            ps[i] = pi;
             const double vi = (i > k) ? vs[i] : 0.0;
-            const double pv = wave_sum(q4 == 0 ? pi * vi : 0.0);
-            if (l == 0) sm.red[tid / WAVE] = pv;
+            const double pv = wave_sum(pi * vi);
+            if (l == 0) sm.red[w] = pv;
         }
         __syncthreads();
         double coef = 0.0;
         if (tau != 0.0) {
             double pvt = 0.0;  // fixed order: bitwise reproducible
 #pragma unroll
-            for (int q = 0; q < NWV; ++q) pvt += sm.red[q];
+            for (int q = 0; q < PW; ++q) pvt += sm.red[q];
             coef = uniform(0.5 * tau * pvt);
             double wc[NBC];
 #pragma unroll

@@ -58,13 +58,15 @@ FP32_PEAK_TFLOPS = 157.3
 # workload (cubic2d L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 +
 # WRITE_SIZE x 1.00 (scales from a 256 MiB copy, bench.py --copy-calibration); keyed by
 # solver variant
-TRAFFIC_BYTES = {5: 124.5e6}
+TRAFFIC_BYTES = {5: 124.5e6, 10: 124.8e6}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the 16-step launch, scaled by a plain copy "
                   "(profiles/k_sweep_q_cubic2d_L16_B1024_hbm_traffic.txt)")
 KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",
                 2: "fkk::k_sweep_q<reg_solver<32,8>>", 3: "fkk::k_sweep_q<glob_solver>",
-                4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver>",
-                6: "fkk::k_sweep_q<wave_solver>"}
+                4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver_t<16>>",
+                6: "fkk::k_sweep_q<wave_solver>", 7: "fkk::k_sweep_q<blocked_solver<double,256>>",
+                8: "fkk::k_sweep_q<blocked_solver<double,1024>>", 9: "fkk::k_sweep_q<blocked_solver<double,512,16>>",
+                10: "fkk::k_sweep_q<wrow_solver_t<8>>"}
 
 
 def workload(L=16, B=1024, seed=20240601):

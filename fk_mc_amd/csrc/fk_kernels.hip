@@ -488,8 +488,10 @@ using namespace fkk;
 // trailing updates (N <= 1024), 5 register-resident one-stage, wave-row layout
 // (N <= 256), 6 register-resident one-wave (N <= 64); 7, 8, 9 the blocked FP64 solver in
 // other tile shapes for the autotuner: 4 and 16 waves per matrix, and panels of at most 16
-// columns; 20 blocked FP32 (first stage of the mixed-precision path, N <= 1024).
+// columns; 10 the wave-row layout with 8 waves per matrix (N <= 256); 20 blocked FP32 (first
+// stage of the mixed-precision path, N <= 1024).
 using fkw::wrow_solver;
+using fkw::wrow8_solver;
 using fkq::wave_solver;
 typedef blocked_solver<double, 512> blk64;
 typedef blocked_solver<double, 256> blk64w4;
@@ -500,7 +502,7 @@ typedef blocked_solver<float, 512> blk32;
 int fk_solver_kind(int N) {
     if (N <= 64) return 0;
     if (N <= 128) return 1;
-    if (N <= 256) return 5;
+    if (N <= 256) return 10;
     if (N <= blk64w16::NMAX) return 8;
     if (N <= glob_solver::NMAX) return 3;
     return -1;
@@ -517,6 +519,7 @@ int fk_solver_nmax(int kind) {
         case 7: return blk64w4::NMAX;
         case 8: return blk64w16::NMAX;
         case 9: return blk64nb16::NMAX;
+        case 10: return wrow8_solver::NMAX;
         case FK_SOLVER_MIXED32: return blk32::NMAX;
     }
     return 0;
@@ -533,6 +536,7 @@ int fk_solver_threads(int kind) {
         case 7: return blk64w4::NT;
         case 8: return blk64w16::NT;
         case 9: return blk64nb16::NT;
+        case 10: return wrow8_solver::NT;
         case FK_SOLVER_MIXED32: return blk32::NT;
     }
     return 0;
@@ -562,6 +566,7 @@ hipError_t fk_launch_eig(int kind, const eig_args& a, int grid, hipStream_t st) 
         case 7: return launch_eig<blk64w4>(a, grid, st);
         case 8: return launch_eig<blk64w16>(a, grid, st);
         case 9: return launch_eig<blk64nb16>(a, grid, st);
+        case 10: return launch_eig<wrow8_solver>(a, grid, st);
         case FK_SOLVER_MIXED32: return launch_eig<blk32>(a, grid, st);
     }
     return hipErrorInvalidValue;
@@ -596,6 +601,7 @@ hipError_t fk_launch_sweep_q(int kind, const sweep_args& a, int nsub, const swee
         case 7: return launch_sweep_q<blk64w4>(a, nsub, q, st);
         case 8: return launch_sweep_q<blk64w16>(a, nsub, q, st);
         case 9: return launch_sweep_q<blk64nb16>(a, nsub, q, st);
+        case 10: return launch_sweep_q<wrow8_solver>(a, nsub, q, st);
     }
     return hipErrorInvalidValue;
 }
@@ -612,6 +618,7 @@ hipError_t fk_launch_sweep(int kind, const sweep_args& a, int grid, hipStream_t 
         case 7: return launch_sweep<blk64w4>(a, grid, st);
         case 8: return launch_sweep<blk64w16>(a, grid, st);
         case 9: return launch_sweep<blk64nb16>(a, grid, st);
+        case 10: return launch_sweep<wrow8_solver>(a, grid, st);
     }
     return hipErrorInvalidValue;
 }

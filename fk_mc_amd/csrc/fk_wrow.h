@@ -1,22 +1,19 @@
 // fk_wrow.h -- register-resident Householder tridiagonalisation, "wave-row" layout,
-// for N <= 256: one workgroup of 16 waves per matrix.
+// for N <= 256: one workgroup of NWV waves (16 or 8) per matrix.
 //
-// Wave w holds the rows i = w + 16a (a = 0..15), lane l the columns j = l + 64b
-// (b = 0..3).  Thread (w, l) keeps A[w + 16a][l + 64b] for b <= a/4 in VGPRs
-// (40 doubles): every 64 x 64 diagonal block is stored in full, the blocks below
-// it once.  For a Householder step (Golub & Van Loan 8.3.1):
-//   * a lane's column values v_j, w_j (j = l + 64b) are its own registers; the
-//     row values v_i (i = w + 16a, uniform over the wave) are read from the lane
-//     that holds them with v_readlane for the column sums, and for the rank-2
-//     update v_i and p_i come from LDS by broadcast reads (w_i formed as the lanes
-//     form w_j), which issues fewer VALU instructions than 4 readlanes per row;
-//   * the row sums of p = A v are reduced across the 64 lanes by a
-//     reduce-scatter on v_permlane32_swap / v_permlane16_swap / DPP (no LDS); the
-//     column sums of the blocks below the diagonal go through LDS (16 partials
-//     per column);
-//   * LDS traffic per step is the column x_k, p and the column partials
-//     (~100 KB), against ~0.5 MB for a block-cyclic grid that broadcasts v and w
-//     through LDS.
+// Wave w holds the rows i = w + NWV a (a = 0 .. 256/NWV - 1), lane l the columns j = l + 64b
+// (b = 0..3).  Thread (w, l) keeps A[w + NWV a][l + 64b] for b <= (row block of a) in VGPRs
+// (40 doubles with 16 waves, 80 with 8): every 64 x 64 diagonal block is stored in full, the
+// blocks below it once.  For a Householder step (Golub & Van Loan 8.3.1):
+//   * a lane's column values v_j, w_j (j = l + 64b) are its own registers; the row values
+//     v_i, p_i (i = w + NWV a, uniform over the wave) come from LDS by broadcast reads;
+//   * the row sums of p = A v are reduced across the 64 lanes by a reduce-scatter on
+//     v_permlane32_swap / v_permlane16_swap / DPP (no LDS); the column sums of the blocks
+//     below the diagonal go through LDS (NWV partials per column);
+//   * each step has two workgroup barriers (see step()).
+// With 16 waves four share a SIMD and the matrix takes 40 of a lane's 128 VGPRs; with 8 two
+// share a SIMD, 80 of 256: half the waves repeat the per-wave work (reflector, reductions)
+// and each has twice the rows to interleave.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -56,14 +53,17 @@ __device__ __forceinline__ double reduce4(double (&r)[4]) {
     return u;
 }
 
-struct wrow_solver {
-    static constexpr int NT = 1024;
-    static constexpr int NWV = NT / WAVE;  // 16 waves = row residues
+template <int NWV_>
+struct wrow_solver_t {
+    static constexpr int NWV = NWV_;       // waves = row residues
+    static constexpr int NT = NWV * WAVE;
     static constexpr int NP = 256;
     static constexpr int NMAX = NP;
-    static constexpr int NA = 16;          // rows per thread
+    static constexpr int NA = NP / NWV;    // rows per thread
+    static constexpr int RB = WAVE / NWV;  // rows per thread in each 64-row block
     static constexpr int NBC = 4;          // column blocks per thread
-    static constexpr int NS = 40;          // slots per thread
+    static constexpr int NS = 10 * RB;     // slots per thread: RB (1 + 2 + 3 + 4)
+    static_assert(NWV == 16 || NWV == 8, "16 or 8 waves per matrix");
     struct eig_area {
         double de[2 * NP];
         double ev[NP];
@@ -87,8 +87,9 @@ struct wrow_solver {
     // partials made the solve 11 % slower)
     static_assert(offsetof(smem, u) % 16 == 0 && offsetof(smem, rs) % 16 == 0 && offsetof(smem, v) % 16 == 0,
                   "LDS buffers 16-byte aligned");
+    // slot of row a (block g = a / RB, g + 1 column blocks) and column block b <= g
     __host__ __device__ static constexpr int sl(int a, int b) {
-        return a < 4 ? a : (a < 8 ? 4 + 2 * (a - 4) + b : (a < 12 ? 12 + 3 * (a - 8) + b : 24 + 4 * (a - 12) + b));
+        return RB * ((a / RB) * (a / RB + 1) / 2) + (a % RB) * (a / RB + 1) + b;
     }
     __device__ static const double* dptr(const smem& sm) { return sm.d; }
     __device__ static const double* eptr(const smem& sm) { return sm.e; }
@@ -96,16 +97,14 @@ struct wrow_solver {
 
     // A[i][j] (a stored slot: i / 64 >= j / 64) from the thread that holds it into *dst
     __device__ __forceinline__ static void put_elem(const double (&A)[NS], int i, int j, int w, int l, double* dst) {
-        if (w != (i & 15) || l != (j & 63)) return;
+        if (w != (i % NWV) || l != (j & 63)) return;
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int b = 0; b <= (a >> 2); ++b)
-                if (a == (i >> 4) && b == (j >> 6)) *dst = A[sl(a, b)];
+            for (int b = 0; b <= a / RB; ++b)
+                if (a == i / NWV && b == (j >> 6)) *dst = A[sl(a, b)];
     }
 
-    // Householder step k with the BM = k / 64 finished column blocks known at compile time:
-    // slots of those blocks take no part, and every other skip is per 64-row group.
     // Householder step k with BM = k / 64 finished column blocks known at compile time
     // (slots of those blocks take no part) and CROSS when column k + 1 lies in block BM + 1.
     //
@@ -184,26 +183,27 @@ struct wrow_solver {
         if (l == ((k + 1) & 63)) {
             double* xn = sm.x[pq];
 #pragma unroll
-            for (int a = 4 * CB; a < NA; ++a) xn[w + 16 * a] = A[sl(a, CB)];
+            for (int a = RB * CB; a < NA; ++a) xn[w + NWV * a] = A[sl(a, CB)];
         }
         if (tau != 0.0) {
-            // row sums over the lanes' columns (four groups of 4 rows); rows of finished
-            // groups are not needed (p is zero there)
+            // row sums over the lanes' columns (groups of 4 rows, all in one 64-row block);
+            // rows of finished blocks are not needed (p is zero there)
 #pragma unroll
-            for (int h = BM; h < 4; ++h) {
+            for (int h = RB * BM / 4; h < NA / 4; ++h) {
+                const int g = 4 * h / RB;
                 double rp[4];
 #pragma unroll
                 for (int aa = 0; aa < 4; ++aa) {
                     const int a = 4 * h + aa;
                     double s = 0.0;
 #pragma unroll
-                    for (int b = BM; b <= h; ++b) s = fma(A[sl(a, b)], vc[b], s);
+                    for (int b = BM; b <= g; ++b) s = fma(A[sl(a, b)], vc[b], s);
                     rp[aa] = s;
                 }
                 const double rsum = reduce4(rp);
                 if ((l & 15) == 0) {
                     const int a = 4 * h + 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
-                    sm.rs[w + 16 * a] = rsum;
+                    sm.rs[w + NWV * a] = rsum;
                 }
             }
             // column sums of the blocks strictly below the diagonal blocks, row values by
@@ -212,10 +212,10 @@ struct wrow_solver {
 #pragma unroll
             for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
 #pragma unroll
-            for (int a = 4 * (BM + 1); a < NA; ++a) {
-                const double vr = vs[w + 16 * a];
+            for (int a = RB * (BM + 1); a < NA; ++a) {
+                const double vr = vs[w + NWV * a];
 #pragma unroll
-                for (int b = BM; b < (a >> 2); ++b) cs[b] = fma(A[sl(a, b)], vr, cs[b]);
+                for (int b = BM; b < a / RB; ++b) cs[b] = fma(A[sl(a, b)], vr, cs[b]);
             }
 #pragma unroll
             for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];
@@ -224,16 +224,17 @@ struct wrow_solver {
         double* ps = sm.p[pk];
         constexpr int PW = NP / WAVE;  // waves forming p: one row a lane, one wave a SIMD
         if (tau != 0.0 && w < PW) {
-            // p_i = tau (row sum + 16 column partials, pairwise in a fixed order); entries of
+            // p_i = tau (row sum + NWV column partials, pairwise in a fixed order); entries of
             // finished rows and columns hold older partials and are masked.  The other waves go
             // on to the barrier: this phase is a latency chain, which fewer waves do not lengthen.
             const int i = tid;
-            double t[4];
+            double t[NWV / 4];
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+            for (int g = 0; g < NWV / 4; ++g)
                 t[g] = (sm.u.cpart[4 * g][i] + sm.u.cpart[4 * g + 1][i]) +
                        (sm.u.cpart[4 * g + 2][i] + sm.u.cpart[4 * g + 3][i]);
-            const double s = sm.rs[i] + ((t[0] + t[1]) + (t[2] + t[3]));
+            const double tt = NWV == 16 ? (t[0] + t[1]) + (t[NWV / 4 - 2] + t[NWV / 4 - 1]) : t[0] + t[1];
+            const double s = sm.rs[i] + tt;
             const double pi = (i > k && i < N) ? tau * s : 0.0;
             ps[i] = pi;
             const double vi = (i > k) ? vs[i] : 0.0;
@@ -251,14 +252,14 @@ struct wrow_solver {
 #pragma unroll
             for (int b = BM; b < NBC; ++b) wc[b] = fma(-coef, vc[b], ps[l + 64 * b]);
 #pragma unroll
-            for (int a = 4 * BM; a < NA; ++a) {
+            for (int a = RB * BM; a < NA; ++a) {
                 // row values by broadcast LDS reads, w_i formed as the lanes form w_j; a
                 // finished row (v_i = p_i = 0) is left as it is
-                const int i = w + 16 * a;
+                const int i = w + NWV * a;
                 const double vra = vs[i];
                 const double wra = fma(-coef, vra, ps[i]);
 #pragma unroll
-                for (int b = BM; b <= (a >> 2); ++b)
+                for (int b = BM; b <= a / RB; ++b)
                     A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
             }
         }
@@ -283,7 +284,7 @@ struct wrow_solver {
         // column 0 for step 0
         if (l == 0) {
 #pragma unroll
-            for (int a = 0; a < NA; ++a) sm.x[0][w + 16 * a] = A[sl(a, 0)];
+            for (int a = 0; a < NA; ++a) sm.x[0][w + NWV * a] = A[sl(a, 0)];
         }
         __syncthreads();
         double tprev = 0.0, cprev = 0.0;
@@ -321,20 +322,20 @@ struct wrow_solver {
         const int l = tid & (WAVE - 1);
         const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
         double A[NS];
-        // all 40 template loads in flight at once (indices clamped, out-of-range entries zeroed
+        // all template loads in flight at once (indices clamped, out-of-range entries zeroed
         // afterwards), then the diagonal
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int b = 0; b <= (a >> 2); ++b) {
-                const int i = min(w + 16 * a, N - 1), j = min(l + 64 * b, N - 1);
+            for (int b = 0; b <= a / RB; ++b) {
+                const int i = min(w + NWV * a, N - 1), j = min(l + 64 * b, N - 1);
                 A[sl(a, b)] = hop[(size_t)i * N + j];
             }
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int b = 0; b <= (a >> 2); ++b) {
-                const int i = w + 16 * a, j = l + 64 * b;
+            for (int b = 0; b <= a / RB; ++b) {
+                const int i = w + NWV * a, j = l + 64 * b;
                 double v = (i < N && j < N) ? A[sl(a, b)] : 0.0;
                 if (i == j && i < N) v += -mu_c + U * (double)f_lds[i];
                 A[sl(a, b)] = v;
@@ -345,5 +346,8 @@ struct wrow_solver {
         return logz_block<NT>(sm.u.ea.ev, N, sm.beta_s[0], cexp, cfermi, sm.u.ea.scratch);
     }
 };
+
+typedef wrow_solver_t<16> wrow_solver;
+typedef wrow_solver_t<8> wrow8_solver;
 
 }  // namespace fkw

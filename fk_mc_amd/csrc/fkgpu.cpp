@@ -1463,7 +1463,7 @@ int fkgpu_chains_create_multi(int ncontexts, fkgpu_ctx* const* contexts, int nch
     }
     // one solver for all chains: the register-resident wave-row solver when every N <= 256,
     // else the blocked HBM-workspace solver (both take N at run time)
-    const int kind = nmax <= 256 ? 5 : 8;
+    const int kind = nmax <= 256 ? 10 : 8;
     const int B = nchains;
     std::vector<int32_t> fpad((size_t)B * nmax, 0);
     for (int c = 0; c < B; ++c) {

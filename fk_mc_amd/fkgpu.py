@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libfkgpu.so")
 OK, EINVAL, ENOLATTICE, EUNSUPPORTED, EHIP, ENOMEM = 0, -1, -2, -3, -4, -5
 LATTICE_HYPERCUBIC, LATTICE_TRIANGULAR = 0, 1
 MOVE_ADDREMOVE, MOVE_FLIP, MOVE_HOP, MOVE_RANDOMIZE = 0, 1, 2, 3
-SOLVER_AUTO, SOLVER_VARIANTS = -1, 10
+SOLVER_AUTO, SOLVER_VARIANTS = -1, 11
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 UPDATE_DENSE, UPDATE_RANK1 = 0, 1
 KPM_DELTA_LOCAL, KPM_DELTA_STOCHASTIC = 0, 1

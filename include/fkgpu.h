@@ -74,9 +74,11 @@ int fkgpu_msize(const fkgpu_ctx* ctx);
  *   6                       register-resident one-stage, one wave (lane = column), N <= 64
  *   7, 8, 9                 variant 4 in other tile shapes: 4 and 16 waves per matrix (256 and
  *                           1024 threads), and panels of at most 16 columns, N <= 1024;
- *                           8 is the default for 256 < N <= 1024 */
+ *                           8 is the default for 256 < N <= 1024
+ *   10                      variant 5 with 8 waves per matrix (80 doubles a lane), N <= 256;
+ *                           the default for 128 < N <= 256 */
 #define FKGPU_SOLVER_AUTO (-1)
-#define FKGPU_SOLVER_VARIANTS 10
+#define FKGPU_SOLVER_VARIANTS 11
 int fkgpu_set_solver(fkgpu_ctx* ctx, int variant);
 int fkgpu_get_solver(const fkgpu_ctx* ctx); /* the variant used for the current lattice */
 /* Autotuner: times every variant that fits the lattice's N on `batch` synthetic

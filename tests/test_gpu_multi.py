@@ -60,7 +60,7 @@ def test_multi_lattice_chains(fk, orc, lats, B, nsteps):
     for c in range(B):
         assert not st["f"][c, Ns[lat[c]]:].any() and not st["spectrum"][c, Ns[lat[c]]:].any()
     # the same chains, one lattice per chain set, the same solver
-    solver = 5 if max(Ns) <= 256 else 8
+    solver = 10 if max(Ns) <= 256 else 8
     for li, (k, D, L, t, tp) in enumerate(lats):
         idx = np.nonzero(lat == li)[0]
         c1 = fk.Context(0).set_lattice(k, D, L, t, tp)

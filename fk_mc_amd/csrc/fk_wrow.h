@@ -124,20 +124,17 @@ struct wrow_solver_t {
         const double* xe = sm.x[pk];
         const double* vq = sm.v[pq];
         const double* prq = sm.p[pq];
+        // A[i][k] after step k - 1: without an update there (tau = 0) v_k = w_k = 0 leave the
+        // stored column as it is (v, p hold finite older values), so no branch keeps the loads
+        // of the lane's four entries from going out together
         const bool upd = tprev != 0.0;  // wave-uniform
         const double cq = cprev;
-        double vk = 0.0, wk = 0.0;
-        if (upd) {
-            vk = vq[k];
-            wk = fma(-cq, vk, prq[k]);
-        }
-        // A[i][k] after step k - 1
+        const double vk = upd ? vq[k] : 0.0;
+        const double wk = upd ? fma(-cq, vk, prq[k]) : 0.0;
         auto col = [&](int i) {
-            const double xi = xe[i];
-            if (!upd) return xi;
             const double vi = vq[i];
             const double wi = fma(-cq, vi, prq[i]);
-            return fma(-vi, wk, fma(-wi, vk, xi));
+            return fma(-vi, wk, fma(-wi, vk, xe[i]));
         };
         // reflector from x = A[k+1:N, k] (every wave redundantly), the lane's entries l + 64b
         const double alpha = col(k + 1);
@@ -251,16 +248,26 @@ struct wrow_solver_t {
             double wc[NBC];
 #pragma unroll
             for (int b = BM; b < NBC; ++b) wc[b] = fma(-coef, vc[b], ps[l + 64 * b]);
+            // row values by broadcast LDS reads, a 64-row block's rows of the wave loaded together
+            // (one LDS round trip a block), w_i formed as the lanes form w_j; a finished row
+            // (v_i = p_i = 0) is left as it is
 #pragma unroll
-            for (int a = RB * BM; a < NA; ++a) {
-                // row values by broadcast LDS reads, w_i formed as the lanes form w_j; a
-                // finished row (v_i = p_i = 0) is left as it is
-                const int i = w + NWV * a;
-                const double vra = vs[i];
-                const double wra = fma(-coef, vra, ps[i]);
+            for (int g = BM; g < NBC; ++g) {
+                double vra[RB], pra[RB];
 #pragma unroll
-                for (int b = BM; b <= a / RB; ++b)
-                    A[sl(a, b)] = fma(-vra, wc[b], fma(-wra, vc[b], A[sl(a, b)]));
+                for (int aa = 0; aa < RB; ++aa) {
+                    const int i = w + NWV * (RB * g + aa);
+                    vra[aa] = vs[i];
+                    pra[aa] = ps[i];
+                }
+#pragma unroll
+                for (int aa = 0; aa < RB; ++aa) {
+                    const int a = RB * g + aa;
+                    const double wra = fma(-coef, vra[aa], pra[aa]);
+#pragma unroll
+                    for (int b = BM; b <= g; ++b)
+                        A[sl(a, b)] = fma(-vra[aa], wc[b], fma(-wra, vc[b], A[sl(a, b)]));
+                }
             }
         }
         tprev = tau;

@@ -488,18 +488,31 @@ __device__ __forceinline__ void rescale2(double& a1, double& a2, int ex) {
     a2 = __builtin_amdgcn_ldexp(a2, -ex);
 }
 
+// Per-step bookkeeping kept in vector registers: a comparison's lane mask would pass through
+// scalar registers and stall the pass's in-order instruction stream once per step.
+// Sign change of consecutive leading minors: their sign bits differ (no minor is zero while
+// the pivot test below holds).
+__device__ __forceinline__ int sign_change(double p, double p1) {
+    return (int)(((unsigned)__double2hiint(p) ^ (unsigned)__double2hiint(p1)) >> 31);
+}
+// Pivot test |p| >= pivmin |p1| of every step as the minimum of |p| - pivmin |p1| (a NaN
+// leaves it unchanged but reaches the final value, which the callers test for finiteness).
+__device__ __forceinline__ double pivot_margin(double mn, double p, double p1, double pivmin) {
+    return fmin(mn, fma(-pivmin, fabs(p1), fabs(p)));
+}
+
 __device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ de, int N, double sigma,
                                                     double pivmin) {
     double p1 = 1.0, p2 = 0.0, d1 = 0.0, d2 = 0.0, h1 = 0.0, h2 = 0.0;
     int cnt = 0;
-    bool ok = true;
+    double mn = 1.0;
     auto step = [&](double2 t) {
         const double c = t.x - sigma;
         const double p = fma(c, p1, -t.y * p2);
         const double dp = fma(c, d1, -fma(t.y, d2, p1));
         const double hp = fma(c, h1, -fma(t.y, h2, 2.0 * d1));
-        ok &= fabs(p) >= pivmin * fabs(p1);  // false for NaN too
-        cnt += (p < 0.0) != (p1 < 0.0);
+        mn = pivot_margin(mn, p, p1, pivmin);
+        cnt += sign_change(p, p1);
         p2 = p1; p1 = p;
         d2 = d1; d1 = dp;
         h2 = h1; h1 = hp;
@@ -519,18 +532,18 @@ __device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ 
     for (; i < N; ++i) step(de[i]);
     const double G = d1 / p1;
     const double H = fma(G, G, -h1 / p1);
-    if (!ok || !isfinite(G) || !isfinite(H)) return sturm_laguerre_q(de, N, sigma, pivmin);
+    if (!(mn >= 0.0) || !isfinite(G) || !isfinite(H)) return sturm_laguerre_q(de, N, sigma, pivmin);
     return {cnt, G, H};
 }
 
 __device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N, double s, double pivmin) {
     double p1 = 1.0, p2 = 0.0;
     int c = 0;
-    bool ok = true;
+    double mn = 1.0;
     auto step = [&](double2 t) {
         const double p = fma(t.x - s, p1, -t.y * p2);
-        ok &= fabs(p) >= pivmin * fabs(p1);
-        c += (p < 0.0) != (p1 < 0.0);
+        mn = pivot_margin(mn, p, p1, pivmin);
+        c += sign_change(p, p1);
         p2 = p1; p1 = p;
     };
     int i = 0;
@@ -543,7 +556,7 @@ __device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N
         rescale2(p1, p2, __builtin_amdgcn_frexp_exp(p1));
     }
     for (; i < N; ++i) step(de[i]);
-    if (!ok || !isfinite(p1)) return sturm_count_q(de, N, s, pivmin);
+    if (!(mn >= 0.0) || !isfinite(p1)) return sturm_count_q(de, N, s, pivmin);
     return c;
 }
 
@@ -552,13 +565,14 @@ __device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int
                                              double pivmin, int& c1, int& c2) {
     double a1 = 1.0, a2 = 0.0, b1 = 1.0, b2 = 0.0;
     int ca = 0, cb = 0;
-    bool ok = true;
+    double mna = 1.0, mnb = 1.0;
     auto step = [&](double2 t) {
         const double pa = fma(t.x - s1, a1, -t.y * a2);
         const double pb = fma(t.x - s2, b1, -t.y * b2);
-        ok &= (fabs(pa) >= pivmin * fabs(a1)) & (fabs(pb) >= pivmin * fabs(b1));
-        ca += (pa < 0.0) != (a1 < 0.0);
-        cb += (pb < 0.0) != (b1 < 0.0);
+        mna = pivot_margin(mna, pa, a1, pivmin);
+        mnb = pivot_margin(mnb, pb, b1, pivmin);
+        ca += sign_change(pa, a1);
+        cb += sign_change(pb, b1);
         a2 = a1; a1 = pa;
         b2 = b1; b1 = pb;
     };
@@ -573,7 +587,7 @@ __device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int
         rescale2(b1, b2, __builtin_amdgcn_frexp_exp(b1));
     }
     for (; i < N; ++i) step(de[i]);
-    if (!ok || !isfinite(a1) || !isfinite(b1)) {
+    if (!(mna >= 0.0) || !(mnb >= 0.0) || !isfinite(a1) || !isfinite(b1)) {
         sturm_count2_q(de, N, s1, s2, pivmin, c1, c2);
         return;
     }

@@ -184,9 +184,14 @@ struct wrow_solver_t {
         }
         if (tau != 0.0) {
             // row sums over the lanes' columns (groups of 4 rows, all in one 64-row block);
-            // rows of finished blocks are not needed (p is zero there)
+            // rows of finished blocks are not needed (p is zero there).  The groups' sums are
+            // stored together after all reductions: a store per group under a lane condition
+            // would split the code into blocks and keep the groups' reductions from
+            // interleaving.
+            constexpr int H0 = RB * BM / 4, NH = NA / 4 - H0;
+            double rsum[NH];
 #pragma unroll
-            for (int h = RB * BM / 4; h < NA / 4; ++h) {
+            for (int h = H0; h < NA / 4; ++h) {
                 const int g = 4 * h / RB;
                 double rp[4];
 #pragma unroll
@@ -197,11 +202,12 @@ struct wrow_solver_t {
                     for (int b = BM; b <= g; ++b) s = fma(A[sl(a, b)], vc[b], s);
                     rp[aa] = s;
                 }
-                const double rsum = reduce4(rp);
-                if ((l & 15) == 0) {
-                    const int a = 4 * h + 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
-                    sm.rs[w + NWV * a] = rsum;
-                }
+                rsum[h - H0] = reduce4(rp);
+            }
+            if ((l & 15) == 0) {
+                const int ar = 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
+#pragma unroll
+                for (int h = H0; h < NA / 4; ++h) sm.rs[w + NWV * (4 * h + ar)] = rsum[h - H0];
             }
             // column sums of the blocks strictly below the diagonal blocks, row values by
             // broadcast LDS reads (v is zero on finished rows)

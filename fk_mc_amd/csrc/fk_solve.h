@@ -595,10 +595,50 @@ __device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int
     c2 = cb;
 }
 
+// counts at four shifts in one pass (four independent chains)
+__device__ __forceinline__ void sturm_count4(const double2* __restrict__ de, int N, const double (&s)[4], double pivmin,
+                                             int (&c)[4]) {
+    double a1[4], a2[4], mn[4];
+    int cc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { a1[u] = 1.0; a2[u] = 0.0; mn[u] = 1.0; cc[u] = 0; }
+    auto step = [&](double2 t) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double p = fma(t.x - s[u], a1[u], -t.y * a2[u]);
+            mn[u] = pivot_margin(mn[u], p, a1[u], pivmin);
+            cc[u] += sign_change(p, a1[u]);
+            a2[u] = a1[u];
+            a1[u] = p;
+        }
+    };
+    int i = 0;
+    for (; i + SPF <= N; i += SPF) {
+        double2 t[SPF];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) t[q] = de[i + q];
+#pragma unroll
+        for (int q = 0; q < SPF; ++q) step(t[q]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) rescale2(a1[u], a2[u], __builtin_amdgcn_frexp_exp(a1[u]));
+    }
+    for (; i < N; ++i) step(de[i]);
+    bool ok = true;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ok &= (mn[u] >= 0.0) && isfinite(a1[u]);
+    if (!ok) {
+        sturm_count2_q(de, N, s[0], s[1], pivmin, c[0], c[1]);
+        sturm_count2_q(de, N, s[2], s[3], pivmin, c[2], c[3]);
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = cc[u];
+}
+
 template <int NT, int MS = 1>
 __device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e, double* de_buf, int* cnt, int N,
                                                 double* ev_out, double* scratch) {
-    static_assert(MS == 1 || MS == 2, "one or two multisection shifts per thread");
+    static_assert(MS == 1 || MS == 2 || MS == 4, "one, two or four multisection shifts per thread");
     constexpr int NC = MS * NT;  // interior multisection points
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1), wid = tid / WAVE;
@@ -637,11 +677,18 @@ __device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e
     // 1. multisection: cnt[m] = count at lo + m*h, m = 0..NC+1
     if (MS == 1) {
         cnt[tid + 1] = sturm_count(de, N, lo + (double)(tid + 1) * h, pivmin);
-    } else {
+    } else if (MS == 2) {
         int c1, c2;
         sturm_count2(de, N, lo + (double)(2 * tid + 1) * h, lo + (double)(2 * tid + 2) * h, pivmin, c1, c2);
         cnt[2 * tid + 1] = c1;
         cnt[2 * tid + 2] = c2;
+    } else {
+        const double sh[4] = {lo + (double)(4 * tid + 1) * h, lo + (double)(4 * tid + 2) * h,
+                              lo + (double)(4 * tid + 3) * h, lo + (double)(4 * tid + 4) * h};
+        int c4[4];
+        sturm_count4(de, N, sh, pivmin, c4);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cnt[4 * tid + u + 1] = c4[u];
     }
     if (tid == 0) { cnt[0] = 0; cnt[NC + 1] = N; }
     __syncthreads();

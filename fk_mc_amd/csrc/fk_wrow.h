@@ -64,10 +64,11 @@ struct wrow_solver_t {
     static constexpr int NBC = 4;          // column blocks per thread
     static constexpr int NS = 10 * RB;     // slots per thread: RB (1 + 2 + 3 + 4)
     static_assert(NWV == 16 || NWV == 8, "16 or 8 waves per matrix");
+    static constexpr int EMS = NT >= 1024 ? 2 : 4;  // multisection shifts per thread: 2048 cells
     struct eig_area {
         double de[2 * NP];
         double ev[NP];
-        int cnt[2 * NT + 2];
+        int cnt[EMS * NT + 2];
         double scratch[3 * NWV + 8];
     };
     struct smem {
@@ -355,7 +356,7 @@ struct wrow_solver_t {
             }
         if (tid == 0) sm.beta_s[0] = beta;
         tridiagonalize(A, sm, N, vout, tauout);
-        tridiag_eigvals<NT, 2>(sm.d, sm.e, sm.u.ea.de, sm.u.ea.cnt, N, sm.u.ea.ev, sm.u.ea.scratch);
+        tridiag_eigvals<NT, EMS>(sm.d, sm.e, sm.u.ea.de, sm.u.ea.cnt, N, sm.u.ea.ev, sm.u.ea.scratch);
         return logz_block<NT>(sm.u.ea.ev, N, sm.beta_s[0], cexp, cfermi, sm.u.ea.scratch);
     }
 };

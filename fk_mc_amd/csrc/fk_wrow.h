@@ -216,10 +216,14 @@ struct wrow_solver_t {
 #pragma unroll
             for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
 #pragma unroll
-            for (int a = RB * (BM + 1); a < NA; ++a) {
-                const double vr = vs[w + NWV * a];
+            for (int g = BM + 1; g < NBC; ++g) {
+                double vr[RB];  // a 64-row block's rows of the wave loaded together
 #pragma unroll
-                for (int b = BM; b < a / RB; ++b) cs[b] = fma(A[sl(a, b)], vr, cs[b]);
+                for (int aa = 0; aa < RB; ++aa) vr[aa] = vs[w + NWV * (RB * g + aa)];
+#pragma unroll
+                for (int aa = 0; aa < RB; ++aa)
+#pragma unroll
+                    for (int b = BM; b < g; ++b) cs[b] = fma(A[sl(RB * g + aa, b)], vr[aa], cs[b]);
             }
 #pragma unroll
             for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];

@@ -184,13 +184,17 @@ struct wrow_solver_t {
             for (int a = RB * CB; a < NA; ++a) xn[w + NWV * a] = A[sl(a, CB)];
         }
         if (tau != 0.0) {
-            // row sums over the lanes' columns (groups of 4 rows, all in one 64-row block);
-            // rows of finished blocks are not needed (p is zero there).  The groups' sums are
-            // stored together after all reductions: a store per group under a lane condition
-            // would split the code into blocks and keep the groups' reductions from
-            // interleaving.
-            constexpr int H0 = RB * BM / 4, NH = NA / 4 - H0;
-            double rsum[NH];
+            // p = A v as column sums over the diagonal blocks and the blocks below them (the
+            // diagonal blocks are stored in full, so their column sums are complete; partials
+            // per wave through LDS) plus row sums over the blocks left of the diagonal blocks
+            // (reduced over the lanes).  Rows of a finished block are not needed (p is zero
+            // there), and the rows of block BM have no blocks left of theirs, so only rows
+            // in blocks BM + 1 .. 3 are reduced across lanes.
+            // The groups' row sums are stored together after all reductions: a store per group
+            // under a lane condition would split the code into blocks and keep the groups'
+            // reductions from interleaving.
+            constexpr int H0 = RB * (BM + 1) / 4, NH = NA / 4 - H0;
+            double rsum[NH > 0 ? NH : 1];
 #pragma unroll
             for (int h = H0; h < NA / 4; ++h) {
                 const int g = 4 * h / RB;
@@ -200,30 +204,28 @@ struct wrow_solver_t {
                     const int a = 4 * h + aa;
                     double s = 0.0;
 #pragma unroll
-                    for (int b = BM; b <= g; ++b) s = fma(A[sl(a, b)], vc[b], s);
+                    for (int b = BM; b < g; ++b) s = fma(A[sl(a, b)], vc[b], s);
                     rp[aa] = s;
                 }
                 rsum[h - H0] = reduce4(rp);
             }
-            if ((l & 15) == 0) {
+            if (NH > 0 && (l & 15) == 0) {
                 const int ar = 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
 #pragma unroll
                 for (int h = H0; h < NA / 4; ++h) sm.rs[w + NWV * (4 * h + ar)] = rsum[h - H0];
             }
-            // column sums of the blocks strictly below the diagonal blocks, row values by
-            // broadcast LDS reads (v is zero on finished rows)
             double cs[NBC];
 #pragma unroll
             for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
 #pragma unroll
-            for (int g = BM + 1; g < NBC; ++g) {
+            for (int g = BM; g < NBC; ++g) {
                 double vr[RB];  // a 64-row block's rows of the wave loaded together
 #pragma unroll
                 for (int aa = 0; aa < RB; ++aa) vr[aa] = vs[w + NWV * (RB * g + aa)];
 #pragma unroll
                 for (int aa = 0; aa < RB; ++aa)
 #pragma unroll
-                    for (int b = BM; b < g; ++b) cs[b] = fma(A[sl(RB * g + aa, b)], vr[aa], cs[b]);
+                    for (int b = BM; b <= g; ++b) cs[b] = fma(A[sl(RB * g + aa, b)], vr[aa], cs[b]);
             }
 #pragma unroll
             for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];
@@ -242,7 +244,7 @@ struct wrow_solver_t {
                 t[g] = (sm.u.cpart[4 * g][i] + sm.u.cpart[4 * g + 1][i]) +
                        (sm.u.cpart[4 * g + 2][i] + sm.u.cpart[4 * g + 3][i]);
             const double tt = NWV == 16 ? (t[0] + t[1]) + (t[NWV / 4 - 2] + t[NWV / 4 - 1]) : t[0] + t[1];
-            const double s = sm.rs[i] + tt;
+            const double s = ((i >> 6) > BM ? sm.rs[i] : 0.0) + tt;
             const double pi = (i > k && i < N) ? tau * s : 0.0;
             ps[i] = pi;
             const double vi = (i > k) ? vs[i] : 0.0;

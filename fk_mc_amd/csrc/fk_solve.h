@@ -703,12 +703,14 @@ __device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e
         int cl = cnt[ml], cu = cnt[mh];
         double s = 0.5 * (l + u);
         double res = s;
+        double prev = INFINITY;  // the last Laguerre step taken
         for (int it = 0; it < 200; ++it) {
             if (u - l <= atol + 2.0 * eps * fmax(fabs(l), fabs(u))) { res = 0.5 * (l + u); break; }
             const sturm_res sr = sturm_laguerre(de, N, s, pivmin);
             const bool left = sr.count > j;  // eigenvalue j lies below s
             if (left) { u = s; cu = sr.count; } else { l = s; cl = sr.count; }
             double next = 0.5 * (l + u);
+            bool took = false;  // a Laguerre step
             if (cu - cl == 1) {
                 const double disc = fmax((nd - 1.0) * (nd * sr.H - sr.G * sr.G), 0.0);
                 const double sq = sqrt(disc);
@@ -726,12 +728,34 @@ __device__ __forceinline__ void tridiag_eigvals(const double* d, const double* e
                             if (ca > j) { u = sn - vtol; cu = ca; next = u; }
                             else if (cb <= j) { l = sn + vtol; cl = cb; next = l; }
                             else next = 0.5 * (l + u);
-                        } else {
+                        } else if (!(fabs(sn - s) > 0.5 * prev)) {
+                            // (steps that do not shrink: s sits next to a neighbouring eigenvalue,
+                            // which holds Laguerre's steps towards eigenvalue j to a doubling
+                            // sequence; those bisect instead)
                             next = sn;
+                            took = true;
                         }
+                    } else if (fabs(sn - s) <= small && (sn >= u || sn <= l)) {
+                        // a short step onto or past a bracket edge: the eigenvalue lies within
+                        // rounding of that edge (typically the edge a failed window test left
+                        // just beside it), where Laguerre's step no longer resolves it; one count
+                        // tests the window of width 2 vtol inside the edge (bisection from here
+                        // takes some 30 passes)
+                        if (sn >= u) {
+                            const int c = sturm_count(de, N, u - 2.0 * vtol, pivmin);
+                            if (c <= j) { res = u - vtol; break; }
+                            u = u - 2.0 * vtol; cu = c;
+                        } else {
+                            const int c = sturm_count(de, N, l + 2.0 * vtol, pivmin);
+                            if (c > j) { res = l + vtol; break; }
+                            l = l + 2.0 * vtol; cl = c;
+                        }
+                        next = 0.5 * (l + u);
                     }
                 }
             }
+            if (took) prev = fabs(next - s);
+            else if (next == 0.5 * (l + u)) prev = INFINITY;
             if (next == s) { res = s; break; }
             s = next;
             res = s;

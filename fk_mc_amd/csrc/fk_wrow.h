@@ -133,8 +133,12 @@ struct wrow_solver_t {
         // column k + 1 before this step's update, for the next step and for A v
         if (l == ((k + 1) & 63)) {
             double* xn = sm.x[pq];
+            // rows <= k of block CB are not read (a wave-uniform test per row)
 #pragma unroll
-            for (int a = RB * CB; a < NA; ++a) xn[w + NWV * a] = A[sl(a, CB)];
+            for (int a = RB * CB; a < RB * CB + RB; ++a)
+                if (w + NWV * a > k) xn[w + NWV * a] = A[sl(a, CB)];
+#pragma unroll
+            for (int a = RB * CB + RB; a < NA; ++a) xn[w + NWV * a] = A[sl(a, CB)];
         }
         const double* xe = sm.x[pk];
         const double* vq = sm.v[pq];

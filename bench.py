@@ -58,7 +58,7 @@ FP32_PEAK_TFLOPS = 157.3
 # workload (cubic2d L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 +
 # WRITE_SIZE x 1.00 (scales from a 256 MiB copy, bench.py --copy-calibration); keyed by
 # solver variant
-TRAFFIC_BYTES = {5: 124.5e6, 10: 124.8e6}
+TRAFFIC_BYTES = {5: 124.5e6, 10: 125.4e6}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the 16-step launch, scaled by a plain copy "
                   "(profiles/k_sweep_q_cubic2d_L16_B1024_hbm_traffic.txt)")
 KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",

@@ -108,38 +108,17 @@ struct wrow_solver_t {
                 if (a == i / NWV && b == (j >> 6)) *dst = A[sl(a, b)];
     }
 
-    // Householder step k with BM = k / 64 finished column blocks known at compile time
-    // (slots of those blocks take no part) and CROSS when column k + 1 lies in block BM + 1.
-    //
-    // Look-ahead: column k of A after step k - 1 is not handed over after that step's update
-    // (which would need a barrier between the update and this step's reflector); every wave
-    // forms it itself from column k before step k - 1's update (stored during step k - 1,
-    // before its updates) and step k - 1's v, p and coef, as the update forms the registers.
-    // So a wave that has finished step k - 1's update goes on into step k while others still
-    // update, and a step has two barriers.  tprev / cprev carry step k - 1's tau and coef.
-    //
-    // The product A v is formed from the unscaled column y (y_j = A[j][k], j > k), known before
-    // the reflector's norm, square root and reciprocals: with v = scal y + (1 - scal alpha)
-    // e_{k+1}, A v = scal (A y) + (1 - scal alpha) A[:, k+1], and A[:, k+1] is the column this
-    // step stores for the next one.  The reflector's latency chain and the product's FMAs are
-    // one branch-free stretch of code, which the compiler interleaves.
+    // The step as 16 waves take it (128 registers a lane leave no room to overlap the sums with
+    // the reflector's chain): the reflector first, then A v on v.  Same phases, barriers and
+    // buffers as step(), which describes them; the column handed to the next step is stored
+    // after the reflector and d, e right after it.
     template <int BM, bool CROSS>
-    __device__ __forceinline__ static void step(double (&A)[NS], smem& sm, int N, int k, int w, int l, int tid,
+    __device__ __forceinline__ static void step_v(double (&A)[NS], smem& sm, int N, int k, int w, int l, int tid,
                                                 double* __restrict__ vout, double* __restrict__ tauout,
                                                 double& tprev, double& cprev) {
         static_assert(!CROSS || BM < 3, "column k + 1 lies in block BM or BM + 1 <= 3");
         constexpr int CB = CROSS ? BM + 1 : BM;
         const int pk = k & 1, pq = pk ^ 1;
-        // column k + 1 before this step's update, for the next step and for A v
-        if (l == ((k + 1) & 63)) {
-            double* xn = sm.x[pq];
-            // rows <= k of block CB are not read (a wave-uniform test per row)
-#pragma unroll
-            for (int a = RB * CB; a < RB * CB + RB; ++a)
-                if (w + NWV * a > k) xn[w + NWV * a] = A[sl(a, CB)];
-#pragma unroll
-            for (int a = RB * CB + RB; a < NA; ++a) xn[w + NWV * a] = A[sl(a, CB)];
-        }
         const double* xe = sm.x[pk];
         const double* vq = sm.v[pq];
         const double* prq = sm.p[pq];
@@ -155,84 +134,37 @@ struct wrow_solver_t {
             const double wi = fma(-cq, vi, prq[i]);
             return fma(-vi, wk, fma(-wi, vk, xe[i]));
         };
-        // y at the lane's entries l + 64b (every wave redundantly; every wave stores the same
-        // values to sm.y, from where the wave reads y at its rows: its own stores are ordered
-        // before its loads)
+        // reflector from x = A[k+1:N, k] (every wave redundantly), the lane's entries l + 64b
         const double alpha = col(k + 1);
-        const double dk = col(k);  // A[k][k] is final after step k - 1
-        double* ys = sm.y;
-        double yc[NBC], part = 0.0;
+        double xb[NBC], part = 0.0;
 #pragma unroll
-        for (int b = 0; b < NBC; ++b) {
+        for (int b = BM; b < NBC; ++b) {
             const int j = l + 64 * b;
-            if (b < BM) {
-                yc[b] = 0.0;
-            } else {
-                const double x = col(j);
-                yc[b] = (j > k && j < N) ? x : 0.0;
-                part += (j >= k + 2 && j < N) ? x * x : 0.0;
-                ys[j] = yc[b];
-            }
+            xb[b] = col(j);
+            part += (j >= k + 2 && j < N) ? xb[b] * xb[b] : 0.0;
         }
-        // reflector (selects instead of branches)
         const double xn2 = wave_sum(part);
-        const bool refl = xn2 != 0.0;
-        const double bt = -copysign(sqrt(alpha * alpha + xn2), alpha);
-        const double tau = uniform(refl ? (bt - alpha) * rcp_refined(bt) : 0.0);
-        const double scal = uniform(refl ? rcp_refined(alpha - bt) : 0.0);
-        const double beta = refl ? bt : alpha;
-        // A y as column sums over the diagonal blocks and the blocks below them (the diagonal
-        // blocks are stored in full, so their column sums are complete; partials per wave
-        // through LDS) plus row sums over the blocks left of the diagonal blocks (reduced over
-        // the lanes).  Rows of a finished block are not needed (p is zero there), and the rows
-        // of block BM have no blocks left of theirs, so only rows in blocks BM + 1 .. 3 are
-        // reduced across lanes.  The groups' row sums are stored together after all
-        // reductions: a store per group under a lane condition would split the code into
-        // blocks and keep the groups' reductions from interleaving.
-        constexpr int H0 = RB * (BM + 1) / 4, NH = NA / 4 - H0;
-        double rsum[NH > 0 ? NH : 1];
-#pragma unroll
-        for (int h = H0; h < NA / 4; ++h) {
-            const int g = 4 * h / RB;
-            double rp[4];
-#pragma unroll
-            for (int aa = 0; aa < 4; ++aa) {
-                const int a = 4 * h + aa;
-                double t = 0.0;
-#pragma unroll
-                for (int b = BM; b < g; ++b) t = fma(A[sl(a, b)], yc[b], t);
-                rp[aa] = t;
-            }
-            rsum[h - H0] = reduce4(rp);
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xn2 != 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            tau = (beta - alpha) * rcp_refined(beta);
+            scal = rcp_refined(alpha - beta);
         }
-        double cs[NBC];
-#pragma unroll
-        for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
-#pragma unroll
-        for (int g = BM; g < NBC; ++g) {
-            double vr[RB];  // a 64-row block's rows of the wave loaded together
-#pragma unroll
-            for (int aa = 0; aa < RB; ++aa) vr[aa] = ys[w + NWV * (RB * g + aa)];
-#pragma unroll
-            for (int aa = 0; aa < RB; ++aa)
-#pragma unroll
-                for (int b = BM; b <= g; ++b) cs[b] = fma(A[sl(RB * g + aa, b)], vr[aa], cs[b]);
+        if (tid == 0) {
+            sm.e[k] = beta;
+            sm.d[k] = col(k);  // A[k][k] is final after step k - 1
         }
-#pragma unroll
-        for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];
-        // v at the lane's columns, to sm.v (read at the wave's rows after the first barrier)
+        tau = uniform(tau);
+        scal = uniform(scal);
+        // v at the lane's columns; every wave stores the same values to sm.v, from where the
+        // wave reads v at its rows (its own stores are ordered before its loads)
         double* vs = sm.v[pk];
         double vc[NBC];
 #pragma unroll
         for (int b = 0; b < NBC; ++b) {
             const int j = l + 64 * b;
-            vc[b] = b < BM ? 0.0 : ((j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? yc[b] * scal : 0.0));
+            vc[b] = b < BM ? 0.0 : ((j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? xb[b] * scal : 0.0));
             if (b >= BM) vs[j] = vc[b];
-        }
-        if (NH > 0 && (l & 15) == 0) {
-            const int ar = 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
-#pragma unroll
-            for (int h = H0; h < NA / 4; ++h) sm.rs[w + NWV * (4 * h + ar)] = rsum[h - H0];
         }
         if (vout) {
             if (w == 0) {
@@ -242,16 +174,66 @@ struct wrow_solver_t {
             }
             if (tid == 0) tauout[k] = tau;
         }
+        // column k + 1 before this step's update, for the next step
+        if (l == ((k + 1) & 63)) {
+            double* xn = sm.x[pq];
+#pragma unroll
+            for (int a = RB * CB; a < NA; ++a) xn[w + NWV * a] = A[sl(a, CB)];
+        }
+        if (tau != 0.0) {
+            // p = A v as column sums over the diagonal blocks and the blocks below them (the
+            // diagonal blocks are stored in full, so their column sums are complete; partials
+            // per wave through LDS) plus row sums over the blocks left of the diagonal blocks
+            // (reduced over the lanes).  Rows of a finished block are not needed (p is zero
+            // there), and the rows of block BM have no blocks left of theirs, so only rows
+            // in blocks BM + 1 .. 3 are reduced across lanes.
+            // The groups' row sums are stored together after all reductions: a store per group
+            // under a lane condition would split the code into blocks and keep the groups'
+            // reductions from interleaving.
+            constexpr int H0 = RB * (BM + 1) / 4, NH = NA / 4 - H0;
+            double rsum[NH > 0 ? NH : 1];
+#pragma unroll
+            for (int h = H0; h < NA / 4; ++h) {
+                const int g = 4 * h / RB;
+                double rp[4];
+#pragma unroll
+                for (int aa = 0; aa < 4; ++aa) {
+                    const int a = 4 * h + aa;
+                    double s = 0.0;
+#pragma unroll
+                    for (int b = BM; b < g; ++b) s = fma(A[sl(a, b)], vc[b], s);
+                    rp[aa] = s;
+                }
+                rsum[h - H0] = reduce4(rp);
+            }
+            if (NH > 0 && (l & 15) == 0) {
+                const int ar = 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
+#pragma unroll
+                for (int h = H0; h < NA / 4; ++h) sm.rs[w + NWV * (4 * h + ar)] = rsum[h - H0];
+            }
+            double cs[NBC];
+#pragma unroll
+            for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
+#pragma unroll
+            for (int g = BM; g < NBC; ++g) {
+                double vr[RB];  // a 64-row block's rows of the wave loaded together
+#pragma unroll
+                for (int aa = 0; aa < RB; ++aa) vr[aa] = vs[w + NWV * (RB * g + aa)];
+#pragma unroll
+                for (int aa = 0; aa < RB; ++aa)
+#pragma unroll
+                    for (int b = BM; b <= g; ++b) cs[b] = fma(A[sl(RB * g + aa, b)], vr[aa], cs[b]);
+            }
+#pragma unroll
+            for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];
+        }
         __syncthreads();
         double* ps = sm.p[pk];
-        const double c1 = fma(-scal, alpha, 1.0);  // v_{k+1} - scal y_{k+1}
         constexpr int PW = NP / WAVE;  // waves forming p: one row a lane, one wave a SIMD
-        if (w < PW) {
-            // p_i = tau (scal (A y)_i + (1 - scal alpha) A[i][k+1]), (A y)_i = row sum + NWV
-            // column partials, pairwise in a fixed order; entries of finished rows and columns
-            // hold older partials and are masked, and without a reflector p = 0.  The other
-            // waves go on to the barrier: this phase is a latency chain, which fewer waves do
-            // not lengthen.
+        if (tau != 0.0 && w < PW) {
+            // p_i = tau (row sum + NWV column partials, pairwise in a fixed order); entries of
+            // finished rows and columns hold older partials and are masked.  The other waves go
+            // on to the barrier: this phase is a latency chain, which fewer waves do not lengthen.
             const int i = tid;
             double t[NWV / 4];
 #pragma unroll
@@ -260,15 +242,11 @@ struct wrow_solver_t {
                        (sm.u.cpart[4 * g + 2][i] + sm.u.cpart[4 * g + 3][i]);
             const double tt = NWV == 16 ? (t[0] + t[1]) + (t[NWV / 4 - 2] + t[NWV / 4 - 1]) : t[0] + t[1];
             const double s = ((i >> 6) > BM ? sm.rs[i] : 0.0) + tt;
-            const double pi = (tau != 0.0 && i > k && i < N) ? tau * fma(scal, s, c1 * sm.x[pq][i]) : 0.0;
+            const double pi = (i > k && i < N) ? tau * s : 0.0;
             ps[i] = pi;
             const double vi = (i > k) ? vs[i] : 0.0;
             const double pv = wave_sum(pi * vi);
             if (l == 0) sm.red[w] = pv;
-            if (tid == 0) {  // read after the tridiagonalisation only
-                sm.e[k] = beta;
-                sm.d[k] = dk;
-            }
         }
         __syncthreads();
         double coef = 0.0;
@@ -305,8 +283,213 @@ struct wrow_solver_t {
         tprev = tau;
         cprev = coef;
         // no barrier here: this step's buffers that the next step writes before its first
-        // barrier (x[k & 1], y, v[(k + 1) & 1], rs, cpart) were last read before this step's
+        // barrier (x[k & 1], v[(k + 1) & 1], rs, cpart) were last read before this step's
         // second barrier, and the next step writes p, red only after its first barrier
+    }
+
+    // Householder step k with BM = k / 64 finished column blocks known at compile time
+    // (slots of those blocks take no part) and CROSS when column k + 1 lies in block BM + 1.
+    //
+    // Look-ahead: column k of A after step k - 1 is not handed over after that step's update
+    // (which would need a barrier between the update and this step's reflector); every wave
+    // forms it itself from column k before step k - 1's update (stored during step k - 1,
+    // before its updates) and step k - 1's v, p and coef, as the update forms the registers.
+    // So a wave that has finished step k - 1's update goes on into step k while others still
+    // update, and a step has two barriers.  tprev / cprev carry step k - 1's tau and coef.
+    //
+    // The product A v is formed from the unscaled column y (y_j = A[j][k], j > k), known before
+    // the reflector's norm, square root and reciprocals: with v = scal y + (1 - scal alpha)
+    // e_{k+1}, A v = scal (A y) + (1 - scal alpha) A[:, k+1], and A[:, k+1] is the column this
+    // step stores for the next one.  The reflector's latency chain and the product's FMAs are
+    // one branch-free stretch of code, which the compiler interleaves.
+    template <int BM, bool CROSS>
+    __device__ __forceinline__ static void step(double (&A)[NS], smem& sm, int N, int k, int w, int l, int tid,
+                                                double* __restrict__ vout, double* __restrict__ tauout,
+                                                double& tprev, double& cprev) {
+        if constexpr (NWV == 16) {
+            step_v<BM, CROSS>(A, sm, N, k, w, l, tid, vout, tauout, tprev, cprev);
+        } else {
+            static_assert(!CROSS || BM < 3, "column k + 1 lies in block BM or BM + 1 <= 3");
+            constexpr int CB = CROSS ? BM + 1 : BM;
+            const int pk = k & 1, pq = pk ^ 1;
+            // column k + 1 before this step's update, for the next step and for A v
+            if (l == ((k + 1) & 63)) {
+                double* xn = sm.x[pq];
+                // rows <= k of block CB are not read (a wave-uniform test per row)
+#pragma unroll
+                for (int a = RB * CB; a < RB * CB + RB; ++a)
+                    if (w + NWV * a > k) xn[w + NWV * a] = A[sl(a, CB)];
+#pragma unroll
+                for (int a = RB * CB + RB; a < NA; ++a) xn[w + NWV * a] = A[sl(a, CB)];
+            }
+            const double* xe = sm.x[pk];
+            const double* vq = sm.v[pq];
+            const double* prq = sm.p[pq];
+            // A[i][k] after step k - 1: without an update there (tau = 0) v_k = w_k = 0 leave the
+            // stored column as it is (v, p hold finite older values), so no branch keeps the loads
+            // of the lane's four entries from going out together
+            const bool upd = tprev != 0.0;  // wave-uniform
+            const double cq = cprev;
+            const double vk = upd ? vq[k] : 0.0;
+            const double wk = upd ? fma(-cq, vk, prq[k]) : 0.0;
+            auto col = [&](int i) {
+                const double vi = vq[i];
+                const double wi = fma(-cq, vi, prq[i]);
+                return fma(-vi, wk, fma(-wi, vk, xe[i]));
+            };
+            // y at the lane's entries l + 64b (every wave redundantly; every wave stores the same
+            // values to sm.y, from where the wave reads y at its rows: its own stores are ordered
+            // before its loads)
+            const double alpha = col(k + 1);
+            const double dk = col(k);  // A[k][k] is final after step k - 1
+            double* ys = sm.y;
+            double yc[NBC], part = 0.0;
+#pragma unroll
+            for (int b = 0; b < NBC; ++b) {
+                const int j = l + 64 * b;
+                if (b < BM) {
+                    yc[b] = 0.0;
+                } else {
+                    const double x = col(j);
+                    yc[b] = (j > k && j < N) ? x : 0.0;
+                    part += (j >= k + 2 && j < N) ? x * x : 0.0;
+                    ys[j] = yc[b];
+                }
+            }
+            // reflector (selects instead of branches)
+            const double xn2 = wave_sum(part);
+            const bool refl = xn2 != 0.0;
+            const double bt = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            const double tau = uniform(refl ? (bt - alpha) * rcp_refined(bt) : 0.0);
+            const double scal = uniform(refl ? rcp_refined(alpha - bt) : 0.0);
+            const double beta = refl ? bt : alpha;
+            // A y as column sums over the diagonal blocks and the blocks below them (the diagonal
+            // blocks are stored in full, so their column sums are complete; partials per wave
+            // through LDS) plus row sums over the blocks left of the diagonal blocks (reduced over
+            // the lanes).  Rows of a finished block are not needed (p is zero there), and the rows
+            // of block BM have no blocks left of theirs, so only rows in blocks BM + 1 .. 3 are
+            // reduced across lanes.  The groups' row sums are stored together after all
+            // reductions: a store per group under a lane condition would split the code into
+            // blocks and keep the groups' reductions from interleaving.
+            constexpr int H0 = RB * (BM + 1) / 4, NH = NA / 4 - H0;
+            double rsum[NH > 0 ? NH : 1];
+#pragma unroll
+            for (int h = H0; h < NA / 4; ++h) {
+                const int g = 4 * h / RB;
+                double rp[4];
+#pragma unroll
+                for (int aa = 0; aa < 4; ++aa) {
+                    const int a = 4 * h + aa;
+                    double t = 0.0;
+#pragma unroll
+                    for (int b = BM; b < g; ++b) t = fma(A[sl(a, b)], yc[b], t);
+                    rp[aa] = t;
+                }
+                rsum[h - H0] = reduce4(rp);
+            }
+            double cs[NBC];
+#pragma unroll
+            for (int b = 0; b < NBC; ++b) cs[b] = 0.0;
+#pragma unroll
+            for (int g = BM; g < NBC; ++g) {
+                double vr[RB];  // a 64-row block's rows of the wave loaded together
+#pragma unroll
+                for (int aa = 0; aa < RB; ++aa) vr[aa] = ys[w + NWV * (RB * g + aa)];
+#pragma unroll
+                for (int aa = 0; aa < RB; ++aa)
+#pragma unroll
+                    for (int b = BM; b <= g; ++b) cs[b] = fma(A[sl(RB * g + aa, b)], vr[aa], cs[b]);
+            }
+#pragma unroll
+            for (int b = BM; b < NBC; ++b) sm.u.cpart[w][l + 64 * b] = cs[b];
+            // v at the lane's columns, to sm.v (read at the wave's rows after the first barrier)
+            double* vs = sm.v[pk];
+            double vc[NBC];
+#pragma unroll
+            for (int b = 0; b < NBC; ++b) {
+                const int j = l + 64 * b;
+                vc[b] = b < BM ? 0.0 : ((j == k + 1) ? 1.0 : ((j > k + 1 && j < N) ? yc[b] * scal : 0.0));
+                if (b >= BM) vs[j] = vc[b];
+            }
+            if (NH > 0 && (l & 15) == 0) {
+                const int ar = 2 * ((l >> 5) & 1) + ((l >> 4) & 1);
+#pragma unroll
+                for (int h = H0; h < NA / 4; ++h) sm.rs[w + NWV * (4 * h + ar)] = rsum[h - H0];
+            }
+            if (vout) {
+                if (w == 0) {
+#pragma unroll
+                    for (int b = 0; b < NBC; ++b)
+                        if (l + 64 * b < N) vout[(size_t)k * N + l + 64 * b] = vc[b];
+                }
+                if (tid == 0) tauout[k] = tau;
+            }
+            __syncthreads();
+            double* ps = sm.p[pk];
+            const double c1 = fma(-scal, alpha, 1.0);  // v_{k+1} - scal y_{k+1}
+            constexpr int PW = NP / WAVE;  // waves forming p: one row a lane, one wave a SIMD
+            if (w < PW) {
+                // p_i = tau (scal (A y)_i + (1 - scal alpha) A[i][k+1]), (A y)_i = row sum + NWV
+                // column partials, pairwise in a fixed order; entries of finished rows and columns
+                // hold older partials and are masked, and without a reflector p = 0.  The other
+                // waves go on to the barrier: this phase is a latency chain, which fewer waves do
+                // not lengthen.
+                const int i = tid;
+                double t[NWV / 4];
+#pragma unroll
+                for (int g = 0; g < NWV / 4; ++g)
+                    t[g] = (sm.u.cpart[4 * g][i] + sm.u.cpart[4 * g + 1][i]) +
+                           (sm.u.cpart[4 * g + 2][i] + sm.u.cpart[4 * g + 3][i]);
+                const double tt = NWV == 16 ? (t[0] + t[1]) + (t[NWV / 4 - 2] + t[NWV / 4 - 1]) : t[0] + t[1];
+                const double s = ((i >> 6) > BM ? sm.rs[i] : 0.0) + tt;
+                const double pi = (tau != 0.0 && i > k && i < N) ? tau * fma(scal, s, c1 * sm.x[pq][i]) : 0.0;
+                ps[i] = pi;
+                const double vi = (i > k) ? vs[i] : 0.0;
+                const double pv = wave_sum(pi * vi);
+                if (l == 0) sm.red[w] = pv;
+                if (tid == 0) {  // read after the tridiagonalisation only
+                    sm.e[k] = beta;
+                    sm.d[k] = dk;
+                }
+            }
+            __syncthreads();
+            double coef = 0.0;
+            if (tau != 0.0) {
+                double pvt = 0.0;  // fixed order: bitwise reproducible
+#pragma unroll
+                for (int q = 0; q < PW; ++q) pvt += sm.red[q];
+                coef = uniform(0.5 * tau * pvt);
+                double wc[NBC];
+#pragma unroll
+                for (int b = BM; b < NBC; ++b) wc[b] = fma(-coef, vc[b], ps[l + 64 * b]);
+                // row values by broadcast LDS reads, a 64-row block's rows of the wave loaded together
+                // (one LDS round trip a block), w_i formed as the lanes form w_j; a finished row
+                // (v_i = p_i = 0) is left as it is
+#pragma unroll
+                for (int g = BM; g < NBC; ++g) {
+                    double vra[RB], pra[RB];
+#pragma unroll
+                    for (int aa = 0; aa < RB; ++aa) {
+                        const int i = w + NWV * (RB * g + aa);
+                        vra[aa] = vs[i];
+                        pra[aa] = ps[i];
+                    }
+#pragma unroll
+                    for (int aa = 0; aa < RB; ++aa) {
+                        const int a = RB * g + aa;
+                        const double wra = fma(-coef, vra[aa], pra[aa]);
+#pragma unroll
+                        for (int b = BM; b <= g; ++b)
+                            A[sl(a, b)] = fma(-vra[aa], wc[b], fma(-wra, vc[b], A[sl(a, b)]));
+                    }
+                }
+            }
+            tprev = tau;
+            cprev = coef;
+            // no barrier here: this step's buffers that the next step writes before its first
+            // barrier (x[k & 1], y, v[(k + 1) & 1], rs, cpart) were last read before this step's
+            // second barrier, and the next step writes p, red only after its first barrier
+        }
     }
 
     __device__ __forceinline__ static void tridiagonalize(double (&A)[NS], smem& sm, int N, double* __restrict__ vout,

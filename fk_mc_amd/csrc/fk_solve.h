@@ -479,8 +479,8 @@ __device__ __forceinline__ int sturm_count_q(const double2* __restrict__ de, int
 // recurrence).  A step is one dependent FMA instead of a reciprocal, and its rounding
 // is that of the ratio form (relative perturbations of d_i - s and of e^2, one each),
 // so the counts are those of a nearby matrix as before.  The carried values are
-// rescaled by a power of two every 4 steps (exact); a pivot |q_i| < pivmin or a value
-// out of range sends the whole pass to the ratio form above.
+// rescaled by a power of two every SPF steps (exact); a vanishing pivot (the test below)
+// or a value out of range sends the whole pass to the ratio form above.
 constexpr int SPF = 8;  // steps between rescalings
 
 __device__ __forceinline__ void rescale2(double& a1, double& a2, int ex) {
@@ -495,10 +495,19 @@ __device__ __forceinline__ void rescale2(double& a1, double& a2, int ex) {
 __device__ __forceinline__ int sign_change(double p, double p1) {
     return (int)(((unsigned)__double2hiint(p) ^ (unsigned)__double2hiint(p1)) >> 31);
 }
-// Pivot test |p| >= pivmin |p1| of every step as the minimum of |p| - pivmin |p1| (a NaN
-// leaves it unchanged but reaches the final value, which the callers test for finiteness).
+// Pivot test of every step as the minimum of |P_i| over the pass, against a floor: every
+// SPF steps the carried minors are rescaled to |P| in [1/2, 1), and in 8 steps a normal pass
+// moves them by far less than 2^-900 (ratios |q| of 10^-3 to 10^2), so a minor below the floor
+// is an exact or underflowing pivot (s on an eigenvalue of a leading block to the last bits),
+// where the signs of the minors that follow stop counting (a split e_i = 0 then zeroes all of
+// them); such a pass goes to the ratio form.  One min a step (the relative test
+// |P_i| >= pivmin |P_{i-1}| took an FMA more).  A NaN leaves the minimum unchanged but reaches
+// the final value, which the callers test for finiteness.
+constexpr double PIVOT_FLOOR = 0x1p-900;
 __device__ __forceinline__ double pivot_margin(double mn, double p, double p1, double pivmin) {
-    return fmin(mn, fma(-pivmin, fabs(p1), fabs(p)));
+    (void)p1;
+    (void)pivmin;
+    return fmin(mn, fabs(p));
 }
 
 __device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ de, int N, double sigma,
@@ -532,7 +541,7 @@ __device__ __forceinline__ sturm_res sturm_laguerre(const double2* __restrict__ 
     for (; i < N; ++i) step(de[i]);
     const double G = d1 / p1;
     const double H = fma(G, G, -h1 / p1);
-    if (!(mn >= 0.0) || !isfinite(G) || !isfinite(H)) return sturm_laguerre_q(de, N, sigma, pivmin);
+    if (!(mn >= PIVOT_FLOOR) || !isfinite(G) || !isfinite(H)) return sturm_laguerre_q(de, N, sigma, pivmin);
     return {cnt, G, H};
 }
 
@@ -556,7 +565,7 @@ __device__ __forceinline__ int sturm_count(const double2* __restrict__ de, int N
         rescale2(p1, p2, __builtin_amdgcn_frexp_exp(p1));
     }
     for (; i < N; ++i) step(de[i]);
-    if (!(mn >= 0.0) || !isfinite(p1)) return sturm_count_q(de, N, s, pivmin);
+    if (!(mn >= PIVOT_FLOOR) || !isfinite(p1)) return sturm_count_q(de, N, s, pivmin);
     return c;
 }
 
@@ -587,7 +596,7 @@ __device__ __forceinline__ void sturm_count2(const double2* __restrict__ de, int
         rescale2(b1, b2, __builtin_amdgcn_frexp_exp(b1));
     }
     for (; i < N; ++i) step(de[i]);
-    if (!(mna >= 0.0) || !(mnb >= 0.0) || !isfinite(a1) || !isfinite(b1)) {
+    if (!(mna >= PIVOT_FLOOR) || !(mnb >= PIVOT_FLOOR) || !isfinite(a1) || !isfinite(b1)) {
         sturm_count2_q(de, N, s1, s2, pivmin, c1, c2);
         return;
     }
@@ -625,7 +634,7 @@ __device__ __forceinline__ void sturm_count4(const double2* __restrict__ de, int
     for (; i < N; ++i) step(de[i]);
     bool ok = true;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) ok &= (mn[u] >= 0.0) && isfinite(a1[u]);
+    for (int u = 0; u < 4; ++u) ok &= (mn[u] >= PIVOT_FLOOR) && isfinite(a1[u]);
     if (!ok) {
         sturm_count2_q(de, N, s[0], s[1], pivmin, c[0], c[1]);
         sturm_count2_q(de, N, s[2], s[3], pivmin, c[2], c[3]);

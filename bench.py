@@ -54,13 +54,13 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per launch (16 move steps) of the headline move-step kernel on the default
-# workload (cubic2d L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 +
-# WRITE_SIZE x 1.00 (scales from a 256 MiB copy, bench.py --copy-calibration); keyed by
-# solver variant
-TRAFFIC_BYTES = {5: 124.5e6, 10: 125.4e6}
-TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the 16-step launch, scaled by a plain copy "
-                  "(profiles/k_sweep_q_cubic2d_L16_B1024_hbm_traffic.txt)")
+# HBM bytes per move step of the headline move-step kernel on the default workload
+# (cubic2d L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
+# (scales from a 256 MiB copy, bench.py --copy-calibration) over whole launches; the traffic is
+# about proportional to the (step, chain) items.  Keyed by solver variant.
+TRAFFIC_BYTES_PER_STEP = {5: 124.5e6 / 16, 10: 1.197e9 / 160}
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of whole launches, scaled by a plain copy, per "
+                  "move step x the launch's steps (profiles/k_sweep_q_cubic2d_L16_B1024_hbm_traffic.txt)")
 KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",
                 2: "fkk::k_sweep_q<reg_solver<32,8>>", 3: "fkk::k_sweep_q<glob_solver>",
                 4: "fkk::k_sweep_q<blocked_solver<double,512>>", 5: "fkk::k_sweep_q<wrow_solver_t<16>>",
@@ -128,10 +128,12 @@ def main():
     ch = fk.Chains(ctx, f, beta, U, U / 2, U / 2, seeds)
     ch.set_moves([(fk.MOVE_ADDREMOVE, 0.5), (fk.MOVE_HOP, 0.5)])
 
-    # warmup (untimed), then exactly `steps` move steps between synchronisations.  The
-    # timed steps are issued as calls of steps_per_launch steps (one fused launch each,
-    # plus the remainder), so the HIP-event time of every launch is known; each call is
-    # synchronous, which adds a host round trip per launch (tens of us against ~80 ms).
+    # warmup (untimed), then exactly `steps` move steps between synchronisations, as one
+    # persistent launch (a launch ends with a tail in which the last chains finish their steps
+    # while other CUs idle: 1.6 % of the time of a 16-step launch).  The calls below are
+    # generic in the launch size, so the HIP-event time of every launch is known; each call
+    # is synchronous, which adds a host round trip per launch.
+    ch.set_steps_per_launch(max(args.steps, 1))
     if args.warmup:
         ch.sweep(args.warmup)
     spl = ch.steps_per_launch
@@ -153,17 +155,16 @@ def main():
     flop = solves * (4.0 / 3.0) * N ** 3
     achieved = flop / (kern_ms / 1e3) / 1e12
     full = [ms for k, ms in zip(chunks, launch_ms) if k == spl]
-    # HBM bytes of one launch of steps_per_launch steps at the default workload, from the
-    # rocprofv3 counters (profiles/, FETCH_SIZE and WRITE_SIZE scaled by a copy kernel);
-    # the traffic is about proportional to the (step, chain) items, so the mean over the
-    # timed launches scales with the steps each launch ran
-    per_launch_bytes = TRAFFIC_BYTES.get(ctx.get_solver()) if (L, B, spl) == (16, 1024, 16) else None
-    traffic = per_launch_bytes * (args.steps / spl) / len(chunks) if per_launch_bytes else None
+    # HBM bytes per launch at the default workload: the profiled bytes per move step
+    # (profiles/, FETCH_SIZE and WRITE_SIZE scaled by a copy kernel) times the mean steps of
+    # the timed launches
+    per_step_bytes = TRAFFIC_BYTES_PER_STEP.get(ctx.get_solver()) if (L, B) == (16, 1024) else None
+    traffic = per_step_bytes * args.steps / len(chunks) if per_step_bytes else None
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
                 "traffic": traffic,
                 "traffic_source": TRAFFIC_SOURCE if traffic else None,
-                "traffic_per_full_launch": per_launch_bytes,
+                "traffic_per_step": per_step_bytes,
                 "kernel": KERNEL_NAMES.get(ctx.get_solver(), "k_sweep_q"), "steps_per_launch": spl,
                 "launches": len(chunks), "steps_per_timed_launch": chunks,
                 "kernel_ms_per_launch": round(float(np.mean(full)) if full else kern_ms / len(chunks), 4),

@@ -58,7 +58,7 @@ FP32_PEAK_TFLOPS = 157.3
 # (cubic2d L=16, 1024 chains), measured with rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
 # (scales from a 256 MiB copy, bench.py --copy-calibration) over whole launches; the traffic is
 # about proportional to the (step, chain) items.  Keyed by solver variant.
-TRAFFIC_BYTES_PER_STEP = {5: 124.5e6 / 16, 10: 1.197e9 / 160}
+TRAFFIC_BYTES_PER_STEP = {5: 124.5e6 / 16, 10: 1.198e9 / 160}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of whole launches, scaled by a plain copy, per "
                   "move step x the launch's steps (profiles/k_sweep_q_cubic2d_L16_B1024_hbm_traffic.txt)")
 KERNEL_NAMES = {0: "fkk::k_sweep_q<reg_solver<16,4>>", 1: "fkk::k_sweep_q<reg_solver<16,8>>",

@@ -42,6 +42,26 @@ def test_hypercubic_random(fk, orc, D, L):
     _check_batch(fk, orc, fk.LATTICE_HYPERCUBIC, D, L, f, beta, U, U / 2)
 
 
+def test_refinement_long_tail_matrices(fk, orc):
+    """cubic2d L=16 matrices whose eigenvalue refinement once took 18 and 33 passes for its
+    slowest eigenvalue: a Laguerre iterate that missed the verification window left a bracket
+    edge within rounding of the eigenvalue, and every later Laguerre step landed on that edge
+    (tridiag_eigvals now tests the window inside the edge).  The row-0 occupations of the 10th
+    and 12th 256 x 256 draws of default_rng(1), U = 1, mu = 1/2; the eigenvalues must still meet
+    the window's accuracy (8 eps |T| ~ 1e-14), well inside the module's 1e-10."""
+    print("case refinement long tail cubic2d L=16", flush=True)
+    rng = np.random.default_rng(1)
+    draws = [rng.integers(0, 2, (256, 256)).astype(np.int32) for _ in range(12)]
+    f = np.stack([draws[9][0], draws[11][0], draws[9][1], draws[11][1]])
+    ctx = fk.Context(0).set_lattice(fk.LATTICE_HYPERCUBIC, 2, 16, 1.0, 0.0)
+    out = ctx.eig_logz(f, 2.0, 1.0, 0.5)
+    for b in range(f.shape[0]):
+        ref = orc.calc_ed(orc.KIND_HYPERCUBIC, 2, 16, f[b], 2.0, 1.0, 0.5, t=1.0, tp=0.0)
+        err = np.abs(out["evals"][b] - ref["evals"]).max() / np.abs(ref["evals"]).max()
+        assert err < 1e-12, (b, err)
+    ctx.close()
+
+
 def test_triangular_L12(fk, orc):
     print("case triangular L=12", flush=True)
     rng = np.random.default_rng(5)

@@ -39,11 +39,17 @@ struct wave_solver {
     __device__ static const double* eptr(const smem& sm) { return sm.e; }
     __device__ static double* evptr(smem& sm) { return sm.ev; }
 
-    // column `c` of the lane that holds it into sm.x
-    __device__ __forceinline__ static void put_column(const double (&a)[NP], smem& sm, int c) {
+    // column `c` of the lane that holds it into sm.x, rows >= from (8-row groups: the groups
+    // above the first needed row are not stored; a single lane's LDS stores are the costliest
+    // part of a column step)
+    __device__ __forceinline__ static void put_column(const double (&a)[NP], smem& sm, int c, int from) {
         if ((int)threadIdx.x == c) {
 #pragma unroll
-            for (int i = 0; i < NP; ++i) sm.x[i] = a[i];
+            for (int g = 0; g < NP / 8; ++g)
+                if (8 * g + 7 >= from) {
+#pragma unroll
+                    for (int i = 8 * g; i < 8 * g + 8; ++i) sm.x[i] = a[i];
+                }
         }
         __syncthreads();
     }
@@ -52,7 +58,7 @@ struct wave_solver {
                                                           double* __restrict__ tauout) {
         const int j = threadIdx.x;
         for (int k = 0; k + 2 < N; ++k) {
-            put_column(a, sm, k);
+            put_column(a, sm, k, k);  // rows >= k are read
             const double xj = sm.x[j];  // A[j][k]
             const double alpha = sm.x[k + 1];
             if (j == 0) sm.d[k] = sm.x[k];
@@ -99,13 +105,13 @@ struct wave_solver {
         }
         // trailing 2 x 2 (or 1 x 1) block
         if (N >= 2) {
-            put_column(a, sm, N - 2);
+            put_column(a, sm, N - 2, N - 2);
             if (j == 0) {
                 sm.d[N - 2] = sm.x[N - 2];
                 sm.e[N - 2] = sm.x[N - 1];
             }
             __syncthreads();
-            put_column(a, sm, N - 1);
+            put_column(a, sm, N - 1, N - 1);
             if (j == 0) sm.d[N - 1] = sm.x[N - 1];
             if (vout) {
                 if (j < N) {
@@ -115,7 +121,7 @@ struct wave_solver {
                 if (j == 0) { tauout[N - 2] = 0.0; tauout[N - 1] = 0.0; }
             }
         } else {
-            put_column(a, sm, 0);
+            put_column(a, sm, 0, 0);
             if (j == 0) sm.d[0] = sm.x[0];
             if (vout && j == 0) { vout[0] = 0.0; tauout[0] = 0.0; }
         }

@@ -186,31 +186,44 @@ __global__ void k_ipr_w(int S, int N, const double* __restrict__ spec, const dou
     out[(size_t)s * nw + w] = nom / den;
 }
 
-// One thread per site i: rho_ii and the bond sums of row i.  Per-matrix
-// reductions of the per-site values are done by k_adjoint_reduce.
-__global__ void k_adjoint_sites(int B, int N, const double* __restrict__ V, const double* __restrict__ fermi,
-                                const int32_t* __restrict__ nbr, const int32_t* __restrict__ z, int zmax,
-                                int nt_bonds, double* __restrict__ dens, double* __restrict__ bond_t,
-                                double* __restrict__ bond_tp) {
+// One wave per site i: rho_ii and the bond sums of row i, rho_ij = sum_k V_ik f_k V_jk.
+// The lanes stride over k, so row i of V and its neighbours' rows are read coalesced
+// (row-major V: row i holds component i of every vector); the lane partial sums are
+// reduced across the wave.  Per-matrix reductions of the per-site values are done by
+// k_adjoint_reduce.
+constexpr int ADJ_SITES = 4;  // sites (waves) per workgroup
+__global__ void __launch_bounds__(WAVE * ADJ_SITES) k_adjoint_sites(int B, int N, const double* __restrict__ V,
+                                                                    const double* __restrict__ fermi,
+                                                                    const int32_t* __restrict__ nbr,
+                                                                    const int32_t* __restrict__ z, int zmax,
+                                                                    int nt_bonds, double* __restrict__ dens,
+                                                                    double* __restrict__ bond_t,
+                                                                    double* __restrict__ bond_tp) {
     const int b = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B || i >= N) return;
+    const int i = blockIdx.x * ADJ_SITES + (int)(threadIdx.x / WAVE);
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (b >= B || i >= N) return;  // uniform over the wave: the reductions below need all 64 lanes
     const double* v = V + (size_t)b * N * N;
     const double* f = fermi + (size_t)b * N;
     const double* vi = v + (size_t)i * N;
     double rii = 0.0;
-    for (int k = 0; k < N; ++k) rii = fma(vi[k] * vi[k], f[k], rii);
+    for (int k = lane; k < N; k += WAVE) rii = fma(vi[k] * vi[k], f[k], rii);
     double bt = 0.0, btp = 0.0;
     const int zi = z[i];
     for (int s = 0; s < zi; ++s) {
         const double* vj = v + (size_t)nbr[(size_t)i * zmax + s] * N;
         double rij = 0.0;
-        for (int k = 0; k < N; ++k) rij = fma(vi[k] * vj[k], f[k], rij);
+        for (int k = lane; k < N; k += WAVE) rij = fma(vi[k] * vj[k], f[k], rij);
         if (s < nt_bonds) bt += rij; else btp += rij;
     }
-    dens[(size_t)b * N + i] = rii;
-    bond_t[(size_t)b * N + i] = bt;
-    bond_tp[(size_t)b * N + i] = btp;
+    rii = wave_sum(rii);
+    bt = wave_sum(bt);
+    btp = wave_sum(btp);
+    if (lane == 0) {
+        dens[(size_t)b * N + i] = rii;
+        bond_t[(size_t)b * N + i] = bt;
+        bond_tp[(size_t)b * N + i] = btp;
+    }
 }
 
 // out[b*8 + ...] = {dOmega/dU, dOmega/dt, dOmega/dtp, dOmega/dmu_c, N_c, dN_c/dmu_c}
@@ -429,8 +442,8 @@ hipError_t fk_launch_adjoint(int B, int N, const double* V, const double* fermi,
                              const double* beta, const int32_t* nbr, const int32_t* z, int zmax, int nt_bonds,
                              double* dens, double* bond_t, double* bond_tp, double* out, double* rho_full,
                              hipStream_t st) {
-    hipLaunchKernelGGL(k_adjoint_sites, dim3((N + 127) / 128, B), dim3(128), 0, st, B, N, V, fermi, nbr, z, zmax,
-                       nt_bonds, dens, bond_t, bond_tp);
+    hipLaunchKernelGGL(k_adjoint_sites, dim3((N + ADJ_SITES - 1) / ADJ_SITES, B), dim3(WAVE * ADJ_SITES), 0, st, B,
+                       N, V, fermi, nbr, z, zmax, nt_bonds, dens, bond_t, bond_tp);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     hipLaunchKernelGGL(k_adjoint_reduce, dim3(B), dim3(256), 0, st, B, N, f, fermi, beta, dens, bond_t, bond_tp, out);

@@ -4,9 +4,10 @@
 //   k_tri_evecs     eigenvectors of the tridiagonal T by inverse iteration, one
 //                   thread per eigenvalue (LU of T - lambda with partial
 //                   pivoting fused with the right-hand side, O(N) per solve);
-//                   eigenvalues closer than ortol form clusters whose members
-//                   are computed in rounds and orthogonalised (modified
-//                   Gram-Schmidt) against the earlier members.
+//                   eigenvalues closer than ortol form clusters: members of
+//                   tight clusters are computed in rounds and orthogonalised
+//                   (modified Gram-Schmidt) against the earlier members, those
+//                   of loose clusters projected off the earlier members at once.
 //   k_bt_tfac, k_bt_apply
 //                   V = Q Z with Q = H_0 H_1 ... H_{N-3} the Householder
 //                   reflectors stored by the tridiagonaliser, applied in
@@ -45,8 +46,11 @@ struct start_vec {
 // rule).  With gen the right-hand side is the start vector of eigenvalue j and x is not
 // read.  Loads are issued in chunks ahead of their use, branch-free so that a chunk is in
 // flight as a whole (the recurrences themselves are sequential).  Returns ||z||^2; *zbig gets the entry of largest magnitude.
-constexpr int TS_CF = 8, TS_CB = 4;  // chunks of the forward and backward sweeps
-constexpr int TS_CV = 8;  // chunk of the vector passes (orthogonalisation, final scaling)
+// Chunks (CF, CB forward and backward, CV the single-vector passes -- final scaling and the
+// tight-cluster Gram-Schmidt --, CL the loose-cluster Gram-Schmidt with OL + 1 loads an
+// element) are template arguments: a wave holds at most 63 vector loads in flight, and the
+// kernel's register budget (its workgroup size) bounds how many values a chunk may stage.
+template <int CF, int CB>
 __device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int N, double lam,
                             double tiny, double* x, double s, bool gen, int j, double* u0, double* u1, double* u2,
                             size_t ld, double* zbig) {
@@ -54,19 +58,19 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
     double a = d[0] - lam;                 // current pivot-row diagonal
     double b = N > 1 ? e[0] : 0.0;         // current pivot-row superdiagonal
     double xk = gen ? sv.next() : s * x[0];
-    for (int k0 = 0; k0 + 1 < N; k0 += TS_CF) {
-        double xb[TS_CF];
+    for (int k0 = 0; k0 + 1 < N; k0 += CF) {
+        double xb[CF];
         // branch-free loads (index clamped to the last row; the surplus values are not used),
         // so that all of them are in flight before the first is waited for
         if (gen) {
 #pragma unroll
-            for (int q = 0; q < TS_CF; ++q) xb[q] = k0 + 1 + q < N ? sv.next() : 0.0;
+            for (int q = 0; q < CF; ++q) xb[q] = k0 + 1 + q < N ? sv.next() : 0.0;
         } else {
 #pragma unroll
-            for (int q = 0; q < TS_CF; ++q) xb[q] = s * x[(size_t)min(k0 + 1 + q, N - 1) * ld];
+            for (int q = 0; q < CF; ++q) xb[q] = s * x[(size_t)min(k0 + 1 + q, N - 1) * ld];
         }
 #pragma unroll
-        for (int q = 0; q < TS_CF; ++q) {
+        for (int q = 0; q < CF; ++q) {
             const int k = k0 + q;
             if (k + 1 >= N) break;
             const double c = e[k];             // subdiagonal below the pivot
@@ -97,10 +101,10 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
     // back substitution with U (diagonals u0, u1, u2)
     double z1 = 0.0, z2 = 0.0;  // z[k+1], z[k+2]
     double nrm2 = 0.0, big = 0.0;
-    for (int k0 = N - 1; k0 >= 0; k0 -= TS_CB) {
-        double xb[TS_CB], p0[TS_CB], p1[TS_CB], p2[TS_CB];
+    for (int k0 = N - 1; k0 >= 0; k0 -= CB) {
+        double xb[CB], p0[CB], p1[CB], p2[CB];
 #pragma unroll
-        for (int q = 0; q < TS_CB; ++q) {
+        for (int q = 0; q < CB; ++q) {
             const size_t o = (size_t)max(k0 - q, 0) * ld;  // branch-free (row 0 repeated past the end)
             xb[q] = x[o];
             p0[q] = u0[o];
@@ -108,7 +112,7 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
             p2[q] = u2[o];
         }
 #pragma unroll
-        for (int q = 0; q < TS_CB; ++q) {
+        for (int q = 0; q < CB; ++q) {
             const int k = k0 - q;
             if (k < 0) break;
             double t = xb[q];
@@ -129,34 +133,35 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
 // Orthogonalise x (column stride N) against the columns p in [p0, p1) of Z (unit vectors),
 // modified Gram-Schmidt, loads in branch-free chunks.  Returns ||x||^2; *xbig gets the entry
 // of largest magnitude.
+template <int CV>
 __device__ double orth_against(double* x, const double* Z, int p0, int p1, int N, double nrm2, double* xbig) {
     for (int p = p0; p < p1; ++p) {
         const double* y = Z + p;
         double dot = 0.0;
-        for (int i0 = 0; i0 < N; i0 += TS_CV) {
-            double xb[TS_CV], yb[TS_CV];
+        for (int i0 = 0; i0 < N; i0 += CV) {
+            double xb[CV], yb[CV];
 #pragma unroll
-            for (int q = 0; q < TS_CV; ++q) {
+            for (int q = 0; q < CV; ++q) {
                 const size_t o = (size_t)min(i0 + q, N - 1) * N;
                 xb[q] = x[o];
                 yb[q] = y[o];
             }
 #pragma unroll
-            for (int q = 0; q < TS_CV; ++q)
+            for (int q = 0; q < CV; ++q)
                 if (i0 + q < N) dot = fma(xb[q], yb[q], dot);
         }
         nrm2 = 0.0;
         double big = 0.0;
-        for (int i0 = 0; i0 < N; i0 += TS_CV) {
-            double xb[TS_CV], yb[TS_CV];
+        for (int i0 = 0; i0 < N; i0 += CV) {
+            double xb[CV], yb[CV];
 #pragma unroll
-            for (int q = 0; q < TS_CV; ++q) {
+            for (int q = 0; q < CV; ++q) {
                 const size_t o = (size_t)min(i0 + q, N - 1) * N;
                 xb[q] = x[o];
                 yb[q] = y[o];
             }
 #pragma unroll
-            for (int q = 0; q < TS_CV; ++q) {
+            for (int q = 0; q < CV; ++q) {
                 if (i0 + q >= N) break;
                 const double v = fma(-dot, yb[q], xb[q]);
                 x[(size_t)(i0 + q) * N] = v;
@@ -170,20 +175,89 @@ __device__ double orth_against(double* x, const double* Z, int p0, int p1, int N
 }
 
 // x <- x sign(big) / sqrt(nrm2): unit norm, largest-magnitude component positive.
+template <int CV>
 __device__ void finish_vec(double* x, int N, double nrm2, double big) {
     const double sc = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0;
     const double sg = big < 0.0 ? -sc : sc;
-    for (int i0 = 0; i0 < N; i0 += TS_CV) {
-        double xb[TS_CV];
+    for (int i0 = 0; i0 < N; i0 += CV) {
+        double xb[CV];
 #pragma unroll
-        for (int q = 0; q < TS_CV; ++q) xb[q] = x[(size_t)min(i0 + q, N - 1) * N];
+        for (int q = 0; q < CV; ++q) xb[q] = x[(size_t)min(i0 + q, N - 1) * N];
 #pragma unroll
-        for (int q = 0; q < TS_CV; ++q)
+        for (int q = 0; q < CV; ++q)
             if (i0 + q < N) x[(size_t)(i0 + q) * N] = sg * xb[q];
     }
 }
 
-// One workgroup per matrix.  d, e, ev: B x N.  Z: B x N x N output.  scratch: B x 3 x N x N.
+// Gram-Schmidt step of a loose-cluster member, every member at once: x (column stride N, unit
+// norm, largest-magnitude component positive) against the columns [p0, p1) of Z as inverse
+// iteration left them (unit norm).  Inverse iteration has separated x from them already, up to
+// delta = eps ||T|| / gap <= 2e-9 (gap > 1e-7 ||T|| outside the tight clusters; a tight-cluster
+// member is orthogonal to its tight predecessors already), so projecting on these columns
+// instead of their orthogonalised forms leaves overlaps of order r^2 delta^2 (r members before
+// x: 2e-16 at r = 7), and no member has to wait for an earlier one.  Blocks of OL columns in
+// turn: one pass forms a block's OL dot products, one subtracts them (classical Gram-Schmidt
+// within a block, which at overlaps this small loses nothing against the modified form).  The
+// result goes to w (stride N; the other members read Z meanwhile); returns its squared norm.
+// The sign stays: a change of 2e-9 per component cannot turn the largest-magnitude component
+// (>= N^-1/2) negative.
+constexpr int OL = 4;
+template <int CL>
+__device__ double orth_loose(const double* x, const double* Z, int p0, int p1, int N, double* w) {
+    double nrm2 = 0.0;
+    for (int pb = p0; pb < p1; pb += OL) {
+        const int nb = min(OL, p1 - pb);
+        const double* src = pb == p0 ? x : w;
+        double dot[OL];
+#pragma unroll
+        for (int m = 0; m < OL; ++m) dot[m] = 0.0;
+        for (int i0 = 0; i0 < N; i0 += CL) {
+            double xb[CL], yb[OL][CL];
+            // branch-free loads: rows clamped to the last one, columns past the block's end
+            // to its last column (their products are dropped below)
+#pragma unroll
+            for (int q = 0; q < CL; ++q) {
+                const size_t o = (size_t)min(i0 + q, N - 1) * N;
+                xb[q] = src[o];
+#pragma unroll
+                for (int m = 0; m < OL; ++m) yb[m][q] = Z[o + pb + min(m, nb - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < CL; ++q) {
+                if (i0 + q >= N) break;
+#pragma unroll
+                for (int m = 0; m < OL; ++m) dot[m] = fma(xb[q], yb[m][q], dot[m]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < OL; ++m)
+            if (m >= nb) dot[m] = 0.0;
+        nrm2 = 0.0;
+        for (int i0 = 0; i0 < N; i0 += CL) {
+            double xb[CL], yb[OL][CL];
+#pragma unroll
+            for (int q = 0; q < CL; ++q) {
+                const size_t o = (size_t)min(i0 + q, N - 1) * N;
+                xb[q] = src[o];
+#pragma unroll
+                for (int m = 0; m < OL; ++m) yb[m][q] = Z[o + pb + min(m, nb - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < CL; ++q) {
+                if (i0 + q >= N) break;
+                double v = xb[q];
+#pragma unroll
+                for (int m = 0; m < OL; ++m) v = fma(-dot[m], yb[m][q], v);
+                w[(size_t)(i0 + q) * N] = v;
+                nrm2 = fma(v, v, nrm2);
+            }
+        }
+    }
+    return nrm2;
+}
+
+// Grid (B, G): G workgroups per matrix, each taking the eigenvalues of whole clusters.  d, e,
+// ev: B x N.  Z: B x N x N output.  scratch: B x 3 x N x N.
 // Two cluster levels (by gaps between consecutive eigenvalues):
 //   * tight (gap <= 1e-7 ||T||): inverse iteration cannot separate the members, so they are
 //     computed in rounds (position in the cluster), three solves each, every iterate
@@ -191,17 +265,22 @@ __device__ void finish_vec(double* x, int N, double nrm2, double big) {
 //   * loose (gap <= 1e-3 ||T||, dstein's threshold): inverse iteration separates them by a
 //     factor ~ eps ||T|| / gap <= 2e-9 per solve, so every eigenvalue outside a tight
 //     cluster is computed at once (two solves), and the members of each loose cluster are
-//     orthogonalised afterwards in cheap rounds of Gram-Schmidt against the earlier members.
-__global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* __restrict__ dg,
+//     then projected off the earlier members, all at once (orth_loose).
+template <int NTB, int CF, int CB, int CV, int CL>
+__global__ void __launch_bounds__(NTB) k_tri_evecs(int B, int N, const double* __restrict__ dg,
                                                     const double* __restrict__ eg, const double* __restrict__ evg,
                                                     double* __restrict__ Zg, double* __restrict__ scr,
                                                     const int32_t* __restrict__ skip) {
-    __shared__ double d[2048], e[2048];
+    __shared__ double d[2048], e[2048], evs[2048];
     __shared__ short cstart[2048], tstart[2048];
-    __shared__ int maxpos_l, maxpos_t;
+    __shared__ int maxpos_l, maxpos_t, jlo, jhi;
     const int b = blockIdx.x;
     const int tid = threadIdx.x, NT = blockDim.x;
     if (b >= B || (skip && skip[b])) return;
+    // this workgroup's share of the eigenvalues: the loose clusters that start in
+    // [c0, c1) (a tight cluster lies inside a loose one), a contiguous range [jlo, jhi)
+    const int G = gridDim.y, chunk = (N + G - 1) / G;
+    const int c0 = (int)blockIdx.y * chunk, c1 = c0 + chunk;
     const double* ev = evg + (size_t)b * N;
     double* Z = Zg + (size_t)b * N * N;
     const size_t NN = (size_t)N * N;
@@ -212,6 +291,7 @@ __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* 
     for (int i = tid; i < N; i += NT) {
         d[i] = dg[(size_t)b * N + i];
         e[i] = eg[(size_t)b * N + i];
+        evs[i] = ev[i];
     }
     __syncthreads();
     for (int i = 0; i < N; ++i) tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < N ? fabs(e[i]) : 0.0));
@@ -221,32 +301,32 @@ __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* 
     const double tiny = eps * fmax(tn, 1e-300);
     // cluster starts (sequential scan by one thread over eigenvalues staged in LDS)
     if (tid == 0) {
-        int cs = 0, ts = 0, mpl = 0, mpt = 0;
+        int cs = 0, ts = 0, mpl = 0, mpt = 0, lo = 0, hi = 0;
         double prev = 0.0;
-        for (int j0 = 0; j0 < N; j0 += TS_CV) {
-            double eb[TS_CV];
-#pragma unroll
-            for (int q = 0; q < TS_CV; ++q) eb[q] = ev[min(j0 + q, N - 1)];
-#pragma unroll
-            for (int q = 0; q < TS_CV; ++q) {
-                const int j = j0 + q;
-                if (j >= N) break;
-                if (j > 0 && eb[q] - prev > ortol) cs = j;
-                if (j > 0 && eb[q] - prev > ttol) ts = j;
-                cstart[j] = (short)cs;
-                tstart[j] = (short)ts;
+        for (int j = 0; j < N; ++j) {
+            const double ej = evs[j];
+            if (j > 0 && ej - prev > ortol) cs = j;
+            if (j > 0 && ej - prev > ttol) ts = j;
+            cstart[j] = (short)cs;
+            tstart[j] = (short)ts;
+            lo += cs < c0;
+            hi += cs < c1;
+            if (cs >= c0 && cs < c1) {
                 mpl = max(mpl, j - cs);
                 mpt = max(mpt, j - ts);
-                prev = eb[q];
             }
+            prev = ej;
         }
         maxpos_l = mpl;
         maxpos_t = mpt;
+        jlo = lo;
+        jhi = hi;
     }
     __syncthreads();
+    const int j0 = jlo, j1 = jhi;
     // inverse iteration: tight clusters in rounds, everything else in round 0
     for (int round = 0; round <= maxpos_t; ++round) {
-        for (int j = tid; j < N; j += NT) {
+        for (int j = j0 + tid; j < j1; j += NT) {
             const int pos = j - tstart[j];
             if (pos != round) continue;
             const bool tight = pos > 0 || (j + 1 < N && tstart[j + 1] == j);
@@ -255,27 +335,38 @@ __global__ void __launch_bounds__(1024) k_tri_evecs(int B, int N, const double* 
             const int its = tight ? 3 : 2;
             for (int it = 0; it < its; ++it) {
                 const double sc = it == 0 ? 1.0 : (nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0);
-                nrm2 = tri_solve(d, e, N, ev[j], tiny, x, sc, it == 0, j, u0 + j, u1 + j, u2 + j, N, &big);
-                if (pos > 0) nrm2 = orth_against(x, Z, tstart[j], j, N, nrm2, &big);
+                nrm2 = tri_solve<CF, CB>(d, e, N, ev[j], tiny, x, sc, it == 0, j, u0 + j, u1 + j, u2 + j, N, &big);
+                if (pos > 0) nrm2 = orth_against<CV>(x, Z, tstart[j], j, N, nrm2, &big);
             }
-            finish_vec(x, N, nrm2, big);
+            finish_vec<CV>(x, N, nrm2, big);
         }
         __threadfence();
         __syncthreads();
     }
-    // loose clusters: Gram-Schmidt against the earlier members, by position (members of a
-    // tight cluster are already orthogonal to their tight predecessors; the pass repeats that
-    // harmlessly)
-    for (int round = 1; round <= maxpos_l; ++round) {
-        for (int j = tid; j < N; j += NT) {
-            if (j - cstart[j] != round) continue;
-            double* x = Z + j;
-            double big = 0.0;
-            const double nrm2 = orth_against(x, Z, cstart[j], j, N, 1.0, &big);
-            finish_vec(x, N, nrm2, big);
+    // loose clusters: every member after the first projected off the earlier members at once
+    // (orth_loose; members of a tight cluster are already orthogonal to their tight
+    // predecessors, which the projection repeats harmlessly) into the free solve scratch u0,
+    // its squared norm to u1[j]; copied back normalised once every member has read the columns
+    // it needs
+    if (maxpos_l > 0) {
+        for (int j = j0 + tid; j < j1; j += NT) {
+            if (j == cstart[j]) continue;
+            u1[j] = orth_loose<CL>(Z + j, Z, cstart[j], j, N, u0 + j);
         }
-        __threadfence();
         __syncthreads();
+        for (int j = j0 + tid; j < j1; j += NT) {
+            if (j == cstart[j]) continue;
+            const double nrm2 = u1[j];
+            const double sc = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 1.0;
+            for (int i0 = 0; i0 < N; i0 += CV) {
+                double xb[CV];
+#pragma unroll
+                for (int q = 0; q < CV; ++q) xb[q] = u0[(size_t)min(i0 + q, N - 1) * N + j];
+#pragma unroll
+                for (int q = 0; q < CV; ++q)
+                    if (i0 + q < N) Z[(size_t)(i0 + q) * N + j] = sc * xb[q];
+            }
+        }
     }
 }
 
@@ -444,8 +535,22 @@ size_t fk_evecs_scratch_doubles(int B, int N) {
 hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const double* ev, const double* vh,
                            const double* tau, double* Z, double* scratch, hipStream_t st, const int32_t* skip) {
     if (N > 2048) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fkv::k_tri_evecs, dim3(B), dim3(N <= 256 ? 256 : 1024), 0, st, B, N, d, e, ev, Z, scratch,
-                       skip);
+    // A batch of fewer matrices than CUs splits each matrix's eigenvalues over G workgroups
+    // (whole clusters each, so the results do not depend on G): the solves of one matrix are
+    // bound by its CU's memory throughput, not by their sequential length.
+    // Such a workgroup holds few solves, each a sequence of dependent memory round trips, so it
+    // runs with one wave a SIMD and stages longer chunks (fewer round trips) in the registers
+    // that frees; the chunk sizes change how loads are grouped, not the arithmetic.
+    const int G = B < 256 ? std::min((256 + B - 1) / B, std::max(1, N / 64)) : 1;
+    if (G > 1)
+        hipLaunchKernelGGL((fkv::k_tri_evecs<256, 16, 12, 16, 8>), dim3(B, G), dim3(256), 0, st, B, N, d, e, ev, Z,
+                           scratch, skip);
+    else if (N <= 256)
+        hipLaunchKernelGGL((fkv::k_tri_evecs<256, 8, 4, 8, 8>), dim3(B, 1), dim3(256), 0, st, B, N, d, e, ev, Z,
+                           scratch, skip);
+    else
+        hipLaunchKernelGGL((fkv::k_tri_evecs<1024, 8, 4, 8, 8>), dim3(B, 1), dim3(1024), 0, st, B, N, d, e, ev, Z,
+                           scratch, skip);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     if (N < 3) return hipSuccess;  // no reflectors

@@ -18,7 +18,7 @@ def _rand_f(rng, n):
     return rng.integers(0, 2, n).astype(np.int32)
 
 
-@pytest.mark.parametrize("kind,D,L", [(0, 2, 8), (0, 1, 30), (0, 2, 16), (1, 2, 12), (0, 3, 8)])
+@pytest.mark.parametrize("kind,D,L", [(0, 2, 8), (0, 1, 30), (0, 2, 16), (1, 2, 12), (0, 3, 8), (1, 2, 24)])
 def test_eigenvectors(fk, orc, kind, D, L):
     print(f"case eigenvectors kind={kind} D={D} L={L}", flush=True)
     rng = np.random.default_rng(L + 3 * D)

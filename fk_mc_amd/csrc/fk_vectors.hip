@@ -565,6 +565,9 @@ hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const
     else if (N <= 512)
         hipLaunchKernelGGL((fkv::k_bt_apply<2, 2>), dim3(B, (N + 31) / 32), dim3(1024), 0, st, B, N, nblk, vh, scratch,
                            Z, skip);
+    else if (N <= 768)  // three row tiles: four would spill at this workgroup size
+        hipLaunchKernelGGL((fkv::k_bt_apply<3, 2>), dim3(B, (N + 31) / 32), dim3(1024), 0, st, B, N, nblk, vh, scratch,
+                           Z, skip);
     else if (N <= 1024)
         hipLaunchKernelGGL((fkv::k_bt_apply<4, 1>), dim3(B, (N + 15) / 16), dim3(1024), 0, st, B, N, nblk, vh, scratch,
                            Z, skip);

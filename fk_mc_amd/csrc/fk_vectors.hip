@@ -266,13 +266,13 @@ __device__ double orth_loose(const double* x, const double* Z, int p0, int p1, i
 //     factor ~ eps ||T|| / gap <= 2e-9 per solve, so every eigenvalue outside a tight
 //     cluster is computed at once (two solves), and the members of each loose cluster are
 //     then projected off the earlier members, all at once (orth_loose).
-template <int NTB, int CF, int CB, int CV, int CL>
+template <int NTB, int NL, int CF, int CB, int CV, int CL>
 __global__ void __launch_bounds__(NTB) k_tri_evecs(int B, int N, const double* __restrict__ dg,
                                                     const double* __restrict__ eg, const double* __restrict__ evg,
                                                     double* __restrict__ Zg, double* __restrict__ scr,
                                                     const int32_t* __restrict__ skip) {
-    __shared__ double d[2048], e[2048], evs[2048];
-    __shared__ short cstart[2048], tstart[2048];
+    __shared__ double d[NL], e[NL], evs[NL];  // NL >= N
+    __shared__ short cstart[NL], tstart[NL];
     __shared__ int maxpos_l, maxpos_t, jlo, jhi;
     const int b = blockIdx.x;
     const int tid = threadIdx.x, NT = blockDim.x;
@@ -541,16 +541,21 @@ hipError_t fk_launch_evecs(int B, int N, const double* d, const double* e, const
     // Such a workgroup holds few solves, each a sequence of dependent memory round trips, so it
     // runs with one wave a SIMD and stages longer chunks (fewer round trips) in the registers
     // that frees; the chunk sizes change how loads are grouped, not the arithmetic.
+    // Up to N = 256 the LDS tables are sized for 256 eigenvalues (7 KB instead of 56), so that
+    // LDS no longer caps a CU at two of these workgroups.
     const int G = B < 256 ? std::min((256 + B - 1) / B, std::max(1, N / 64)) : 1;
-    if (G > 1)
-        hipLaunchKernelGGL((fkv::k_tri_evecs<256, 16, 12, 16, 8>), dim3(B, G), dim3(256), 0, st, B, N, d, e, ev, Z,
-                           scratch, skip);
+    if (G > 1 && N <= 256)
+        hipLaunchKernelGGL((fkv::k_tri_evecs<256, 256, 16, 12, 16, 8>), dim3(B, G), dim3(256), 0, st, B, N, d, e, ev,
+                           Z, scratch, skip);
+    else if (G > 1)
+        hipLaunchKernelGGL((fkv::k_tri_evecs<256, 2048, 16, 12, 16, 8>), dim3(B, G), dim3(256), 0, st, B, N, d, e, ev,
+                           Z, scratch, skip);
     else if (N <= 256)
-        hipLaunchKernelGGL((fkv::k_tri_evecs<256, 8, 4, 8, 8>), dim3(B, 1), dim3(256), 0, st, B, N, d, e, ev, Z,
+        hipLaunchKernelGGL((fkv::k_tri_evecs<256, 256, 8, 4, 8, 8>), dim3(B, 1), dim3(256), 0, st, B, N, d, e, ev, Z,
                            scratch, skip);
     else
-        hipLaunchKernelGGL((fkv::k_tri_evecs<1024, 8, 4, 8, 8>), dim3(B, 1), dim3(1024), 0, st, B, N, d, e, ev, Z,
-                           scratch, skip);
+        hipLaunchKernelGGL((fkv::k_tri_evecs<1024, 2048, 8, 4, 8, 8>), dim3(B, 1), dim3(1024), 0, st, B, N, d, e, ev,
+                           Z, scratch, skip);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
     if (N < 3) return hipSuccess;  // no reflectors

@@ -286,13 +286,19 @@ __global__ void __launch_bounds__(S::NT) k_sweep_q(sweep_args a, int nsub, unsig
             if (q < (unsigned)nsub * (unsigned)B) {
                 const long long t0 = wall_clock64();
                 int v;
-                while ((v = __hip_atomic_load(&ring[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < 0) {
+                // relaxed polls, one acquire fence after the poll that sees the slot filled:
+                // an agent-scope acquire invalidates the XCD's L2 (buffer_inv sc1), and a
+                // workgroup polling with acquires would keep stripping the L2 under the
+                // workgroups that solve on its XCD (with few chains per CU most workgroups
+                // of a launch wait; a solver with an HBM workspace then ran 2-3x slower)
+                while ((v = __hip_atomic_load(&ring[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < 0) {
                     if (wall_clock64() - t0 > wait_ticks) {
                         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(2);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 if (v >= 0) {
                     c = v % B;
                     st = v / B;

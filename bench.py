@@ -17,8 +17,8 @@ Printed: one JSON line with the contract fields plus
   plain         the single-chain drop-in path (calc_hamiltonian + calc_ed
                 through the C ABI with host buffers, one call per move);
   single_chain_cubic2d_L8
-                one chain of 10^4 addremove moves at L=8 (N=64) on the device, and the
-                plain drop-in at that size;
+                one chain of 10^4 addremove moves at L=8 (N=64) on the device (dense
+                FP64, and with rank-one updates), and the plain drop-in at that size;
   measured_sweeps
                 sweeps of 16 moves each followed by the on-device measures
                 (energy, spectrum mean, spectrum / occupation / IPR histories) of
@@ -455,7 +455,7 @@ def c3_block(fk, steps):
 def c1_block(fk, nmoves=10000, nplain=2000):
     """The single-chain configuration: cubic2d L=8 (N=64), U=1, beta=1, mu_c = mu_f = 0.5,
     one chain of move_addremove only, 10^4 moves (1 chain x 10^4 steps in 625 launches of
-    16 steps, replayed from a graph), and the plain drop-in (calc_hamiltonian + calc_ed
+    16 steps, replayed from a graph), dense FP64 and with rank-one updates, and the plain drop-in (calc_hamiltonian + calc_ed
     through the C ABI, host buffers, one call per move) at the same size."""
     L, N = 8, 64
     f = (np.random.default_rng(8).random((1, N)) < 0.5).astype(np.int32)
@@ -465,7 +465,18 @@ def c1_block(fk, nmoves=10000, nplain=2000):
     ch.sweep_graph(nmoves)  # capture + warm-up
     ch.sweep_graph(nmoves)
     ms = ch.last_ms()
-    acc = int(ch.state()["naccept"][0])
+    st = ch.state()
+    acc = int(st["naccept"][0])
+    ch.close()
+    # the same chain with rank-one spectral updates (secular equation per move, FP64
+    # re-decomposition every 256 steps): same draws, same decisions
+    ch = fk.Chains(ctx, f, 1.0, 1.0, 0.5, 0.5, np.array([1], np.uint32))
+    ch.set_moves([(fk.MOVE_ADDREMOVE, 1.0)])
+    ch.set_update(fk.UPDATE_RANK1, 256)
+    ch.sweep_graph(nmoves)
+    ch.sweep_graph(nmoves)
+    ms_r1 = ch.last_ms()
+    st_r1 = ch.state()
     ch.close()
     fp = f.copy()
     rng = np.random.default_rng(9)
@@ -478,6 +489,10 @@ def c1_block(fk, nmoves=10000, nplain=2000):
     ctx.close()
     return {"N": N, "moves": nmoves, "chain_moves_per_s": round(nmoves / ms * 1e3, 1),
             "chain_us_per_move": round(ms * 1e3 / nmoves, 2), "accept_rate": round(acc / (2 * nmoves), 4),
+            "rank1_chain_moves_per_s": round(nmoves / ms_r1 * 1e3, 1),
+            "rank1_chain_us_per_move": round(ms_r1 * 1e3 / nmoves, 2),
+            "rank1_same_decisions": bool(int(st_r1["naccept"][0]) == acc and np.array_equal(st_r1["f"], st["f"])),
+            "rank1_logZ_abs_diff": float(abs(st_r1["logZ"][0] - st["logZ"][0])),
             "plain_moves_per_s": round(plain, 1)}
 
 
